@@ -1,0 +1,49 @@
+"""Builds libfomo_mi355x.so in-tree with hipcc for gfx950.
+
+    python -m fomo_julia_amd.build
+
+Flags: -ffp-contract=off keeps the kernels' arithmetic exactly as written
+(explicit fmaf only); packed-FP32 instructions are disabled; float division
+stays IEEE (hipcc's default correctly rounded divide).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_NAME = "libfomo_mi355x.so"
+LIB_PATH = os.path.join(HERE, LIB_NAME)
+SOURCES = [os.path.join(HERE, "csrc", "fomo_mi355x.hip")]
+DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("kernels.h", "fomo_device.h")] + [
+    os.path.join(os.path.dirname(HERE), "include", "fomo_mi355x.h")]
+
+
+def hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def command(out: str) -> list[str]:
+    return [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+            "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-command-line-argument", "-o", out] + SOURCES
+
+
+def build(verbose: bool = False) -> str:
+    tmp = LIB_PATH + ".tmp"
+    cmd = command(tmp)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB_PATH)

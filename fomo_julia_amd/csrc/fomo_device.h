@@ -1,0 +1,91 @@
+// Device-side data layout of the MI355X time-stepping path.
+//
+// Every field (wavefield component or medium coefficient) is a pitched 2D array,
+// x fastest: element (i, j) (0-based; Julia's (i+1, j+1)) lives at
+// base[j * pitch + i].  pitch is a multiple of 256 floats (1 KiB), so every
+// 64-lane x 4-column tile row starts on a 1 KiB boundary, and it exceeds nx by
+// at least 8, so x-halo reads of up to 8 cells past either end of a row stay
+// inside the allocation (they land in zero padding).  GUARD_ROWS zero rows sit
+// above row 0 and below row nz-1 for the z-halo, plus SLACK floats before and
+// after the whole block.  Padding and guards are zeroed at allocation and
+// never written, so stencil loads need no bounds checks.
+#pragma once
+#include <stdint.h>
+
+namespace fomo {
+
+constexpr int WAVE = 64;
+constexpr int GUARD_ROWS = 10;  // >= max half-order (8) + 2
+constexpr int SLACK = 256;      // floats before/after a field block
+constexpr int MAX_M = 8;        // fd_order <= 16
+constexpr int MAX_NBC = 63;     // one wavefront spans an HABC strip's width
+
+enum SrcType { SRC_PRESSURE = 0, SRC_FORCE_Z = 1, SRC_FORCE_X = 2 };
+enum RecType { REC_VZ = 0, REC_VX = 1, REC_P = 2 };
+enum Field { F_VX = 0, F_VZ = 1, F_TXX = 2, F_TZZ = 3, F_TXZ = 4 };
+enum Ramp { RAMP_VX = 0, RAMP_VZ = 1, RAMP_TAU = 2 };
+
+// Work-partition kinds; each has a prefix-sum table over the shots of a batch.
+// HABC runs in three dependency phases per field (program order of the
+// reference's regions): phase 1 = left, right, bottom edges; phase 2 = top
+// edge (no free surface) and the bottom corners; phase 3 = top corners (no
+// free surface).  Every piece of a phase reads only cells that no other piece
+// of the same phase writes.
+enum Part {
+  PART_STENCIL = 0,   // velocity / stress tiles
+  PART_HABC1_V,       // phase-1 pieces x 2 velocity fields
+  PART_HABC1_S,       // phase-1 pieces x 3 stress fields
+  PART_HABC2_V,
+  PART_HABC2_S,
+  PART_HABC3_V,
+  PART_HABC3_S,
+  PART_FSREC,         // free-surface rows + receivers
+  PART_BACKUP,        // standalone strip backup (per-stage API)
+  PART_SHOT,          // one item per shot
+  PART_COUNT
+};
+
+// Everything a kernel needs to know about one shot.  Lives in device memory
+// (a table indexed by shot), so a captured graph stays valid when the host
+// rewrites the table between replays.
+struct ShotDev {
+  int nx, nz, pitch, nbc, pad, fs;
+  int tz;               // rows per stencil tile
+  int tiles_x, tiles_z; // stencil tiling
+  int edge_r0;          // first row (0-based) of the left/right HABC edges
+  int edge_rows;        // rows of the left/right HABC edges
+  int edge_cols;        // columns of the bottom/top HABC edges
+  int n_l, n_r, n_b, n_t;  // edge pieces per field
+  float dtx, dtz;
+  float* f[5];          // vx vz txx tzz txz   (pointer to cell (0,0))
+  float* fo[5];         // strip backups (*_old)
+  const float* bvx_x;   // dtx / rho_vx
+  const float* bvx_z;   // dtz / rho_vx
+  const float* bvz_x;   // dtx / rho_vz
+  const float* bvz_z;   // dtz / rho_vz
+  const float* lam;
+  const float* l2m;     // lam + 2 mu
+  const float* mu_txz;
+  const float* ramp;    // [3][ramp_len]: weight as a function of edge distance
+  int ramp_len;         // nbc + 2
+  float a[MAX_M];
+  float qx, qz, qt_x, qt_z, qxt;
+  int si, sj, stype;    // source cell (0-based) and type
+  const float* wavelet;
+  int nw;
+  float src_scale;      // dt / rho at the source (force sources)
+  const int* rec;       // [n_rec][3]: i, j (0-based), type
+  int n_rec;
+  float* gather;        // [nt][n_rec]
+  int nt;
+  int* kp;              // device step counter: step of k_off == 0 (1-based)
+};
+
+struct BatchArgs {
+  const ShotDev* shots;
+  const int* prefix;    // prefix-sum table of this kernel's partition kind
+  const int* n_shots_p; // number of live table rows
+  int k_off;            // step offset inside the captured chunk
+};
+
+}  // namespace fomo

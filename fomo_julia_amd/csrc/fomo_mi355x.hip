@@ -1,0 +1,821 @@
+// Host side of libfomo_mi355x.so: the C ABI declared in include/fomo_mi355x.h.
+//
+// Objects
+//   fomo_ctx        device + one HIP stream + last error text
+//   fomo_model      Medium + HABCConfig + SimParams on the device: the
+//                   coefficient arrays the kernels read (dt/rho per staggered
+//                   point and direction, lambda, lambda+2mu, mu_txz), the HABC
+//                   weight ramps, FD coefficients and scalars
+//   fomo_wavefield  the 10 fields of Wavefield (5 current + 5 strip backups)
+//   fomo_shot       source cell + wavelet, receivers, gather [nt][n_rec], step counter
+//   fomo_batch      a device table of ShotDev rows + per-kernel work lists;
+//                   single-shot calls use a one-row batch cached on the wavefield
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fomo_mi355x.h"
+#include "kernels.h"
+
+using namespace fomo;
+
+struct fomo_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int cpt = 2;         // columns per lane of the stencil tiles
+  int tz = 32;         // rows per stencil tile
+  int graph_steps = 16;
+};
+
+struct fomo_model {
+  fomo_ctx* ctx = nullptr;
+  fomo_model_desc d{};
+  int M = 0;
+  int pitch = 0;
+  size_t fstride = 0;  // floats per field block (guards + slack included)
+  float* dcoef = nullptr;
+  const float* coef[7] = {};  // bvx_x bvx_z bvz_x bvz_z lam l2m mu_txz
+  float* dramp = nullptr;
+  std::vector<float> rho_vx, rho_vz;  // host copies, for force-source scaling
+  float a[MAX_M] = {};
+  int64_t bytes = 0;
+};
+
+struct fomo_shot;
+struct fomo_batch;
+
+struct fomo_wavefield {
+  fomo_model* m = nullptr;
+  float* dbuf = nullptr;
+  float* f[10] = {};
+  int64_t bytes = 0;
+  fomo_shot* placeholder = nullptr;  // idle shot for stage calls that take none
+  // one-shot batches cached per wavefield: [0] placeholder, [1] last real shot
+  fomo_batch* single[2] = {nullptr, nullptr};
+  uint64_t single_id[2] = {0, 0};
+};
+
+struct fomo_shot {
+  fomo_model* m = nullptr;
+  int si = 0, sj = 0, stype = 0;
+  float* dwav = nullptr;
+  int nw = 0, nt = 0, n_rec = 0;
+  int* drec = nullptr;
+  float* dgather = nullptr;
+  int* dk = nullptr;
+  float src_scale = 0.f;
+  uint64_t id = 0;  // unique per shot object, keys the cached batches
+};
+
+static uint64_t g_next_shot_id = 1;
+
+struct fomo_batch {
+  fomo_ctx* ctx = nullptr;
+  int n = 0, M = 0, cpt = 2, tz = 32, graph_steps = 16;
+  bool any_nofs = false;
+  std::vector<ShotDev> hs;
+  ShotDev* ds = nullptr;
+  int* dn = nullptr;
+  int* dprefix = nullptr;  // [PART_COUNT][n+1]
+  std::vector<int> hprefix;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+};
+
+static thread_local std::string g_err;
+
+static int fail(fomo_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, call)                                                         \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail((ctx), FOMO_ERR_HIP,                                             \
+                  std::string(#call) + ": " + hipGetErrorString(e_));              \
+  } while (0)
+
+static int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// ------------------------------------------------------------------ layout
+static int pitch_for(int nx) { return round_up(nx + 8, 256); }
+static size_t field_stride(int nz, int pitch) {
+  return (size_t)(nz + 2 * GUARD_ROWS) * pitch + 2 * SLACK;
+}
+// Pointer to cell (0,0) of field block b inside a buffer of blocks.
+static float* block_origin(float* buf, size_t stride, int b, int pitch) {
+  return buf + (size_t)b * stride + SLACK + (size_t)GUARD_ROWS * pitch;
+}
+
+// ------------------------------------------------------------------ context
+extern "C" int32_t fomo_ctx_create(int32_t device, fomo_ctx** out) {
+  if (!out) return fail(nullptr, FOMO_ERR_ARG, "out is NULL");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return fail(nullptr, FOMO_ERR_HIP, std::string("no HIP device: ") + hipGetErrorString(e));
+  if (device < 0 || device >= ndev) return fail(nullptr, FOMO_ERR_ARG, "device out of range");
+  auto c = std::make_unique<fomo_ctx>();
+  c->device = device;
+  HIP_TRY(nullptr, hipSetDevice(device));
+  HIP_TRY(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c.release();
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_ctx_destroy(fomo_ctx* c) {
+  if (!c) return FOMO_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FOMO_OK;
+}
+
+extern "C" const char* fomo_ctx_last_error(const fomo_ctx* c) {
+  return c ? c->err.c_str() : g_err.c_str();
+}
+
+extern "C" int32_t fomo_ctx_synchronize(fomo_ctx* c) {
+  if (!c) return fail(nullptr, FOMO_ERR_ARG, "ctx is NULL");
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FOMO_OK;
+}
+
+extern "C" void* fomo_ctx_stream(fomo_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+extern "C" int32_t fomo_ctx_set_tiling(fomo_ctx* c, int32_t cpt, int32_t tz, int32_t gsteps) {
+  if (!c) return fail(nullptr, FOMO_ERR_ARG, "ctx is NULL");
+  if (cpt != 1 && cpt != 2 && cpt != 4) return fail(c, FOMO_ERR_ARG, "cols_per_lane must be 1, 2 or 4");
+  if (tz < 1 || tz > 4096) return fail(c, FOMO_ERR_ARG, "rows_per_tile out of range");
+  if (gsteps < 1 || gsteps > 256) return fail(c, FOMO_ERR_ARG, "steps_per_graph out of range");
+  c->cpt = cpt;
+  c->tz = tz;
+  c->graph_steps = gsteps;
+  return FOMO_OK;
+}
+
+// ------------------------------------------------------------------ model
+// Extracts the weight-vs-distance ramp from an (nz, nx) matrix and checks that
+// the whole matrix is that ramp of dist = min(i-1, nx-i, j-1, nz-j).
+static bool ramp_from_matrix(const float* w, int nx, int nz, int nbc, float* ramp) {
+  const int jm = nz / 2;  // 0-based middle row: its distance to top/bottom exceeds nbc+1
+  for (int d = 0; d <= nbc + 1; ++d) ramp[d] = w[(size_t)jm + (size_t)d * nz];
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < nz; ++j) {
+      int d = std::min(std::min(i, nx - 1 - i), std::min(j, nz - 1 - j));
+      if (w[(size_t)j + (size_t)i * nz] != ramp[std::min(d, nbc + 1)]) return false;
+    }
+  return true;
+}
+
+static void ramp_closed_form(int nbc, double off, float* ramp) {
+  for (int d = 0; d <= nbc + 1; ++d) {
+    double v = ((double)d - off) / (double)nbc;
+    ramp[d] = (float)std::min(1.0, std::max(0.0, v));
+  }
+}
+
+extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, const float* a,
+                                     const float* rho_vx, const float* rho_vz, const float* lam,
+                                     const float* mu_txx, const float* mu_txz, const float* w_vx,
+                                     const float* w_vz, const float* w_tau, fomo_model** out) {
+  if (!c || !d || !a || !out) return fail(c, FOMO_ERR_ARG, "NULL argument");
+  if (!rho_vx || !rho_vz || !lam || !mu_txx || !mu_txz)
+    return fail(c, FOMO_ERR_ARG, "medium array is NULL");
+  if (d->fd_order < 2 || d->fd_order > 2 * MAX_M || d->fd_order % 2)
+    return fail(c, FOMO_ERR_UNSUPPORTED, "fd_order must be even, 2..16");
+  if (d->nbc < 1 || d->nbc > MAX_NBC)
+    return fail(c, FOMO_ERR_UNSUPPORTED, "nbc must be in 1..63");
+  const int M = d->fd_order / 2;
+  const int nmin = 2 * (d->nbc + 3) + 2 * M;
+  if (d->nx < nmin || d->nz < nmin)
+    return fail(c, FOMO_ERR_ARG, "grid too small for nbc and fd_order");
+  if (d->nx > (1 << 20) || d->nz > (1 << 20) || (int64_t)d->nx * d->nz > (int64_t)1 << 31)
+    return fail(c, FOMO_ERR_ARG, "grid too large");
+  if (d->storage != FOMO_STORAGE_F32) return fail(c, FOMO_ERR_UNSUPPORTED, "storage type");
+  const bool any_w = w_vx || w_vz || w_tau;
+  if (any_w && !(w_vx && w_vz && w_tau))
+    return fail(c, FOMO_ERR_ARG, "pass all three HABC weight matrices or none");
+
+  auto m = std::make_unique<fomo_model>();
+  m->ctx = c;
+  m->d = *d;
+  m->M = M;
+  for (int l = 0; l < M; ++l) m->a[l] = a[l];
+  const int nx = d->nx, nz = d->nz, rl = d->nbc + 2;
+  std::vector<float> ramps(3 * rl);
+  if (any_w) {
+    const float* ws[3] = {w_vx, w_vz, w_tau};
+    for (int r = 0; r < 3; ++r)
+      if (!ramp_from_matrix(ws[r], nx, nz, d->nbc, &ramps[r * rl]))
+        return fail(c, FOMO_ERR_UNSUPPORTED,
+                    "HABC weights are not a function of the edge distance (init_habc ramps)");
+  } else {
+    ramp_closed_form(d->nbc, 0.0, &ramps[0]);
+    ramp_closed_form(d->nbc, 0.5, &ramps[rl]);
+    ramp_closed_form(d->nbc, 0.75, &ramps[2 * rl]);
+  }
+
+  // Derived coefficients, IEEE Float32 on the host: identical bits to the
+  // per-cell expressions of velocity.jl:41-42 (dtx / rho) and stress.jl:124-125
+  // (lam + 2mu; 2mu is exact).
+  const size_t n = (size_t)nx * nz;
+  const bool same = d->dtx == d->dtz;
+  std::vector<float> h[7];
+  const float* src_direct[7] = {nullptr, nullptr, nullptr, nullptr, lam, nullptr, mu_txz};
+  for (int k = 0; k < 7; ++k)
+    if (!src_direct[k]) h[k].resize(n);
+  for (size_t o = 0; o < n; ++o) {
+    h[0][o] = d->dtx / rho_vx[o];
+    h[1][o] = d->dtz / rho_vx[o];
+    h[2][o] = d->dtx / rho_vz[o];
+    h[3][o] = d->dtz / rho_vz[o];
+    h[5][o] = lam[o] + 2.0f * mu_txx[o];
+  }
+  const int P = pitch_for(nx);
+  const size_t fs = field_stride(nz, P);
+  const int nblocks = same ? 5 : 7;  // dtx == dtz: the z-scaled arrays alias the x-scaled ones
+  m->pitch = P;
+  m->fstride = fs;
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMalloc(&m->dcoef, nblocks * fs * sizeof(float)));
+  HIP_TRY(c, hipMemset(m->dcoef, 0, nblocks * fs * sizeof(float)));
+  // block order: bvx_x, bvz_x, lam, l2m, mu_txz, [bvx_z, bvz_z]
+  const int blk_of[7] = {0, same ? 0 : 5, 1, same ? 1 : 6, 2, 3, 4};
+  for (int k = 0; k < 7; ++k) {
+    float* dst = block_origin(m->dcoef, fs, blk_of[k], P);
+    m->coef[k] = dst;
+    if (same && (k == 1 || k == 3)) continue;
+    const float* src = src_direct[k] ? src_direct[k] : h[k].data();
+    HIP_TRY(c, hipMemcpy2D(dst, P * sizeof(float), src, nx * sizeof(float), nx * sizeof(float), nz,
+                           hipMemcpyHostToDevice));
+  }
+  HIP_TRY(c, hipMalloc(&m->dramp, ramps.size() * sizeof(float)));
+  HIP_TRY(c, hipMemcpy(m->dramp, ramps.data(), ramps.size() * sizeof(float), hipMemcpyHostToDevice));
+  m->rho_vx.assign(rho_vx, rho_vx + n);
+  m->rho_vz.assign(rho_vz, rho_vz + n);
+  m->bytes = (int64_t)nblocks * fs * sizeof(float) + (int64_t)ramps.size() * sizeof(float);
+  *out = m.release();
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_model_destroy(fomo_model* m) {
+  if (!m) return FOMO_OK;
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
+  (void)hipFree(m->dcoef);
+  (void)hipFree(m->dramp);
+  delete m;
+  return FOMO_OK;
+}
+
+extern "C" int64_t fomo_model_device_bytes(const fomo_model* m) { return m ? m->bytes : 0; }
+
+// ------------------------------------------------------------------ wavefield
+static void drop_single(fomo_wavefield* w);
+
+extern "C" int32_t fomo_wavefield_create(fomo_model* m, fomo_wavefield** out) {
+  if (!m || !out) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_ctx* c = m->ctx;
+  auto w = std::make_unique<fomo_wavefield>();
+  w->m = m;
+  const size_t bytes = 10 * m->fstride * sizeof(float);
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMalloc(&w->dbuf, bytes));
+  HIP_TRY(c, hipMemset(w->dbuf, 0, bytes));
+  for (int k = 0; k < 10; ++k) w->f[k] = block_origin(w->dbuf, m->fstride, k, m->pitch);
+  w->bytes = (int64_t)bytes;
+  *out = w.release();
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_wavefield_destroy(fomo_wavefield* w) {
+  if (!w) return FOMO_OK;
+  (void)hipSetDevice(w->m->ctx->device);
+  (void)hipStreamSynchronize(w->m->ctx->stream);
+  drop_single(w);
+  (void)hipFree(w->dbuf);
+  delete w;
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_wavefield_reset(fomo_wavefield* w) {
+  if (!w) return fail(nullptr, FOMO_ERR_ARG, "NULL wavefield");
+  fomo_ctx* c = w->m->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMemsetAsync(w->dbuf, 0, (size_t)w->bytes, c->stream));
+  return FOMO_OK;
+}
+
+extern "C" int64_t fomo_wavefield_device_bytes(const fomo_wavefield* w) { return w ? w->bytes : 0; }
+
+extern "C" int32_t fomo_wavefield_download(fomo_wavefield* w, int32_t field, float* host) {
+  if (!w || !host) return fail(w ? w->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_ctx* c = w->m->ctx;
+  if (field < 0 || field > 9) return fail(c, FOMO_ERR_ARG, "field id out of range");
+  const int nx = w->m->d.nx, nz = w->m->d.nz, P = w->m->pitch;
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMemcpy2DAsync(host, nx * sizeof(float), w->f[field], P * sizeof(float),
+                              nx * sizeof(float), nz, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_wavefield_upload(fomo_wavefield* w, int32_t field, const float* host) {
+  if (!w || !host) return fail(w ? w->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_ctx* c = w->m->ctx;
+  if (field < 0 || field > 9) return fail(c, FOMO_ERR_ARG, "field id out of range");
+  const int nx = w->m->d.nx, nz = w->m->d.nz, P = w->m->pitch;
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMemcpy2DAsync(w->f[field], P * sizeof(float), host, nx * sizeof(float),
+                              nx * sizeof(float), nz, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FOMO_OK;
+}
+
+// ------------------------------------------------------------------ shot
+extern "C" int32_t fomo_shot_create(fomo_model* m, int32_t src_i, int32_t src_j, int32_t src_type,
+                                    const float* wavelet, int32_t nw, int32_t nt,
+                                    const int32_t* rec_i, const int32_t* rec_j,
+                                    const int32_t* rec_type, int32_t n_rec, fomo_shot** out) {
+  if (!m || !out) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_ctx* c = m->ctx;
+  const int nx = m->d.nx, nz = m->d.nz;
+  if (src_i < 1 || src_i > nx || src_j < 1 || src_j > nz)
+    return fail(c, FOMO_ERR_ARG, "source index outside the padded grid");
+  if (src_type < FOMO_SRC_PRESSURE || src_type > FOMO_SRC_FORCE_X)
+    return fail(c, FOMO_ERR_ARG, "unknown source type");
+  if (nw < 0 || (nw > 0 && !wavelet)) return fail(c, FOMO_ERR_ARG, "bad wavelet");
+  if (nt < 1) return fail(c, FOMO_ERR_ARG, "nt must be >= 1");
+  if (n_rec < 0 || (n_rec > 0 && (!rec_i || !rec_j))) return fail(c, FOMO_ERR_ARG, "bad receivers");
+  if ((int64_t)nt * std::max(n_rec, 1) > ((int64_t)1 << 31))
+    return fail(c, FOMO_ERR_ARG, "gather too large");
+  std::vector<int> rec(3 * (size_t)std::max(n_rec, 1), 0);
+  for (int r = 0; r < n_rec; ++r) {
+    const int ti = rec_type ? rec_type[r] : FOMO_REC_VZ;
+    if (rec_i[r] < 1 || rec_i[r] > nx || rec_j[r] < 1 || rec_j[r] > nz)
+      return fail(c, FOMO_ERR_ARG, "receiver index outside the padded grid");
+    if (ti < FOMO_REC_VZ || ti > FOMO_REC_P) return fail(c, FOMO_ERR_ARG, "unknown receiver type");
+    rec[3 * r] = rec_i[r] - 1;
+    rec[3 * r + 1] = rec_j[r] - 1;
+    rec[3 * r + 2] = ti;
+  }
+  auto s = std::make_unique<fomo_shot>();
+  s->m = m;
+  s->si = src_i - 1;
+  s->sj = src_j - 1;
+  s->stype = src_type;
+  s->nw = nw;
+  s->nt = nt;
+  s->n_rec = n_rec;
+  s->id = __atomic_fetch_add(&g_next_shot_id, 1, __ATOMIC_RELAXED);
+  const size_t o = (size_t)(src_i - 1) + (size_t)(src_j - 1) * nx;
+  if (src_type == FOMO_SRC_FORCE_Z) s->src_scale = m->d.dt / m->rho_vz[o];
+  if (src_type == FOMO_SRC_FORCE_X) s->src_scale = m->d.dt / m->rho_vx[o];
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMalloc(&s->dwav, std::max(nw, 1) * sizeof(float)));
+  if (nw > 0) HIP_TRY(c, hipMemcpy(s->dwav, wavelet, nw * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMalloc(&s->drec, rec.size() * sizeof(int)));
+  HIP_TRY(c, hipMemcpy(s->drec, rec.data(), rec.size() * sizeof(int), hipMemcpyHostToDevice));
+  const size_t gbytes = (size_t)nt * std::max(n_rec, 1) * sizeof(float);
+  HIP_TRY(c, hipMalloc(&s->dgather, gbytes));
+  HIP_TRY(c, hipMemset(s->dgather, 0, gbytes));
+  HIP_TRY(c, hipMalloc(&s->dk, sizeof(int)));
+  HIP_TRY(c, hipMemset(s->dk, 0, sizeof(int)));
+  *out = s.release();
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_shot_destroy(fomo_shot* s) {
+  if (!s) return FOMO_OK;
+  (void)hipSetDevice(s->m->ctx->device);
+  (void)hipStreamSynchronize(s->m->ctx->stream);
+  (void)hipFree(s->dwav);
+  (void)hipFree(s->drec);
+  (void)hipFree(s->dgather);
+  (void)hipFree(s->dk);
+  delete s;
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_shot_reset(fomo_shot* s) {
+  if (!s) return fail(nullptr, FOMO_ERR_ARG, "NULL shot");
+  fomo_ctx* c = s->m->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMemsetAsync(s->dgather, 0, (size_t)s->nt * std::max(s->n_rec, 1) * sizeof(float),
+                            c->stream));
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_shot_gather(fomo_shot* s, float* out) {
+  if (!s || !out) return fail(s ? s->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_ctx* c = s->m->ctx;
+  if (s->n_rec == 0) return FOMO_OK;
+  std::vector<float> tmp((size_t)s->nt * s->n_rec);
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMemcpyAsync(tmp.data(), s->dgather, tmp.size() * sizeof(float),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  // device [k][r]  ->  Julia [nt x n_rec] column-major: out[k + r*nt]
+  for (int k = 0; k < s->nt; ++k)
+    for (int r = 0; r < s->n_rec; ++r) out[(size_t)k + (size_t)r * s->nt] = tmp[(size_t)k * s->n_rec + r];
+  return FOMO_OK;
+}
+
+// ------------------------------------------------------------------ batches
+static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const fomo_shot* s,
+                            int cpt, int tz) {
+  ShotDev sd;
+  std::memset(&sd, 0, sizeof(sd));
+  const fomo_model_desc& d = m->d;
+  sd.nx = d.nx;
+  sd.nz = d.nz;
+  sd.pitch = m->pitch;
+  sd.nbc = d.nbc;
+  sd.pad = d.pad;
+  sd.fs = d.free_surface ? 1 : 0;
+  sd.tz = tz;
+  sd.tiles_x = (d.nx + 64 * cpt - 1) / (64 * cpt);
+  sd.tiles_z = (d.nz + tz - 1) / tz;
+  const int n = d.nbc;
+  sd.edge_r0 = sd.fs ? n : 1;
+  sd.edge_rows = (d.nz - n - 1) - sd.edge_r0;
+  sd.edge_cols = d.nx - 2 * n - 2;
+  sd.n_l = (sd.edge_rows + LROWS - 1) / LROWS;
+  sd.n_r = (sd.edge_rows + 63) / 64;
+  sd.n_b = (sd.edge_cols + 63) / 64;
+  sd.n_t = sd.fs ? 0 : sd.n_b;
+  sd.dtx = d.dtx;
+  sd.dtz = d.dtz;
+  for (int k = 0; k < 5; ++k) {
+    sd.f[k] = w->f[k];
+    sd.fo[k] = w->f[5 + k];
+  }
+  sd.bvx_x = m->coef[0];
+  sd.bvx_z = m->coef[1];
+  sd.bvz_x = m->coef[2];
+  sd.bvz_z = m->coef[3];
+  sd.lam = m->coef[4];
+  sd.l2m = m->coef[5];
+  sd.mu_txz = m->coef[6];
+  sd.ramp = m->dramp;
+  sd.ramp_len = d.nbc + 2;
+  for (int l = 0; l < MAX_M; ++l) sd.a[l] = m->a[l];
+  sd.qx = d.qx;
+  sd.qz = d.qz;
+  sd.qt_x = d.qt_x;
+  sd.qt_z = d.qt_z;
+  sd.qxt = d.qxt;
+  sd.si = s->si;
+  sd.sj = s->sj;
+  sd.stype = s->stype;
+  sd.wavelet = s->dwav;
+  sd.nw = s->nw;
+  sd.src_scale = s->src_scale;
+  sd.rec = s->drec;
+  sd.n_rec = s->n_rec;
+  sd.gather = s->dgather;
+  sd.nt = s->nt;
+  sd.kp = s->dk;
+  return sd;
+}
+
+static int part_items(const ShotDev& sd, int part) {
+  const int edges = sd.n_l + sd.n_r + sd.n_b;
+  switch (part) {
+    case PART_STENCIL: return sd.tiles_x * sd.tiles_z;
+    case PART_HABC1_V: return 2 * edges;
+    case PART_HABC1_S: return 3 * edges;
+    case PART_HABC2_V: return 2 * (sd.n_t + 2);
+    case PART_HABC2_S: return 3 * (sd.n_t + 2);
+    case PART_HABC3_V: return sd.fs ? 0 : 2 * 2;
+    case PART_HABC3_S: return sd.fs ? 0 : 3 * 2;
+    case PART_FSREC: return (6 * sd.nx + sd.n_rec + 63) / 64;
+    case PART_BACKUP: return (int)(((int64_t)sd.nx * sd.nz + 63) / 64);
+    case PART_SHOT: return 1;
+  }
+  return 0;
+}
+
+static void destroy_batch(fomo_batch* b) {
+  if (!b) return;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  (void)hipFree(b->ds);
+  (void)hipFree(b->dn);
+  (void)hipFree(b->dprefix);
+  delete b;
+}
+
+static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield* const* ws,
+                       fomo_shot* const* ss, fomo_batch** out) {
+  if (n < 1) return fail(c, FOMO_ERR_ARG, "empty batch");
+  auto b = std::unique_ptr<fomo_batch, void (*)(fomo_batch*)>(new fomo_batch, destroy_batch);
+  b->ctx = c;
+  b->n = n;
+  b->cpt = c->cpt;
+  b->tz = c->tz;
+  b->graph_steps = c->graph_steps;
+  b->M = ms[0]->M;
+  for (int i = 0; i < n; ++i) {
+    if (!ms[i] || !ws[i] || !ss[i]) return fail(c, FOMO_ERR_ARG, "NULL batch member");
+    if (ms[i]->ctx != c || ws[i]->m != ms[i] || ss[i]->m != ms[i])
+      return fail(c, FOMO_ERR_STATE, "batch member belongs to another model or context");
+    if (ms[i]->M != b->M) return fail(c, FOMO_ERR_UNSUPPORTED, "batch mixes FD orders");
+    b->hs.push_back(make_shotdev(ms[i], ws[i], ss[i], b->cpt, b->tz));
+    if (!ms[i]->d.free_surface) b->any_nofs = true;
+  }
+  b->hprefix.assign((size_t)PART_COUNT * (n + 1), 0);
+  for (int p = 0; p < PART_COUNT; ++p) {
+    int acc = 0;
+    for (int i = 0; i < n; ++i) {
+      b->hprefix[(size_t)p * (n + 1) + i] = acc;
+      const int64_t next = (int64_t)acc + part_items(b->hs[i], p);
+      if (next > (1 << 30)) return fail(c, FOMO_ERR_ARG, "batch work list too large");
+      acc = (int)next;
+    }
+    b->hprefix[(size_t)p * (n + 1) + n] = acc;
+  }
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMalloc(&b->ds, n * sizeof(ShotDev)));
+  HIP_TRY(c, hipMemcpy(b->ds, b->hs.data(), n * sizeof(ShotDev), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMalloc(&b->dn, sizeof(int)));
+  HIP_TRY(c, hipMemcpy(b->dn, &n, sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMalloc(&b->dprefix, b->hprefix.size() * sizeof(int)));
+  HIP_TRY(c, hipMemcpy(b->dprefix, b->hprefix.data(), b->hprefix.size() * sizeof(int),
+                       hipMemcpyHostToDevice));
+  *out = b.release();
+  return FOMO_OK;
+}
+
+static void drop_single(fomo_wavefield* w) {
+  for (int q = 0; q < 2; ++q) {
+    if (w->single[q]) destroy_batch(w->single[q]);
+    w->single[q] = nullptr;
+    w->single_id[q] = 0;
+  }
+  if (w->placeholder) fomo_shot_destroy(w->placeholder);
+  w->placeholder = nullptr;
+}
+
+// The one-row batch for (m, w, s); s == NULL selects an idle placeholder shot
+// (no source, no receivers, nt = 1) for the stages that take no shot.
+static int single_batch(fomo_model* m, fomo_wavefield* w, fomo_shot* s, fomo_batch** out) {
+  fomo_ctx* c = m->ctx;
+  if (w->m != m || (s && s->m != m)) return fail(c, FOMO_ERR_STATE, "wavefield/shot of another model");
+  if (!s) {
+    if (!w->placeholder) {
+      int rc = fomo_shot_create(m, 1, 1, FOMO_SRC_PRESSURE, nullptr, 0, 1, nullptr, nullptr,
+                                nullptr, 0, &w->placeholder);
+      if (rc) return rc;
+    }
+    s = w->placeholder;
+  }
+  const int q = (s == w->placeholder) ? 0 : 1;
+  fomo_batch* b = w->single[q];
+  if (b && w->single_id[q] == s->id && b->cpt == c->cpt && b->tz == c->tz &&
+      b->graph_steps == c->graph_steps) {
+    *out = b;
+    return FOMO_OK;
+  }
+  if (b) destroy_batch(b);
+  w->single[q] = nullptr;
+  w->single_id[q] = 0;
+  int rc = build_batch(c, 1, &m, &w, &s, &w->single[q]);
+  if (rc) return rc;
+  w->single_id[q] = s->id;
+  *out = w->single[q];
+  return FOMO_OK;
+}
+
+// ------------------------------------------------------------------ launches
+constexpr int WPB = 4;  // wavefronts per block
+
+static int blocks_for(const fomo_batch* b, int part) {
+  const int total = b->hprefix[(size_t)part * (b->n + 1) + b->n];
+  return std::max(1, (total + WPB - 1) / WPB);
+}
+
+static BatchArgs args_for(const fomo_batch* b, int part, int k_off) {
+  BatchArgs a;
+  a.shots = b->ds;
+  a.prefix = b->dprefix + (size_t)part * (b->n + 1);
+  a.n_shots_p = b->dn;
+  a.k_off = k_off;
+  return a;
+}
+
+template <int M, int CPT>
+static void launch_stencil_mc(const fomo_batch* b, bool stress, int flags, int k_off) {
+  dim3 grid(blocks_for(b, PART_STENCIL)), block(64 * WPB);
+  BatchArgs a = args_for(b, PART_STENCIL, k_off);
+  if (stress) hipLaunchKernelGGL((k_stress<M, CPT, WPB>), grid, block, 0, b->ctx->stream, a, flags);
+  else hipLaunchKernelGGL((k_velocity<M, CPT, WPB>), grid, block, 0, b->ctx->stream, a, flags);
+}
+
+template <int M>
+static void launch_stencil_m(const fomo_batch* b, bool stress, int flags, int k_off) {
+  switch (b->cpt) {
+    case 1: launch_stencil_mc<M, 1>(b, stress, flags, k_off); break;
+    case 2: launch_stencil_mc<M, 2>(b, stress, flags, k_off); break;
+    default: launch_stencil_mc<M, 4>(b, stress, flags, k_off); break;
+  }
+}
+
+static void launch_stencil(const fomo_batch* b, bool stress, int flags, int k_off) {
+  switch (b->M) {
+    case 1: launch_stencil_m<1>(b, stress, flags, k_off); break;
+    case 2: launch_stencil_m<2>(b, stress, flags, k_off); break;
+    case 3: launch_stencil_m<3>(b, stress, flags, k_off); break;
+    case 4: launch_stencil_m<4>(b, stress, flags, k_off); break;
+    case 5: launch_stencil_m<5>(b, stress, flags, k_off); break;
+    case 6: launch_stencil_m<6>(b, stress, flags, k_off); break;
+    case 7: launch_stencil_m<7>(b, stress, flags, k_off); break;
+    default: launch_stencil_m<8>(b, stress, flags, k_off); break;
+  }
+}
+
+static void launch_habc(const fomo_batch* b, bool stress, int k_off) {
+  const int s = stress ? 1 : 0;
+  dim3 block(64 * WPB);
+  const int p1 = stress ? PART_HABC1_S : PART_HABC1_V;
+  const int p2 = stress ? PART_HABC2_S : PART_HABC2_V;
+  const int p3 = stress ? PART_HABC3_S : PART_HABC3_V;
+  hipLaunchKernelGGL(k_habc1<WPB>, dim3(blocks_for(b, p1)), block, 0, b->ctx->stream,
+                     args_for(b, p1, k_off), s);
+  hipLaunchKernelGGL(k_habc2<WPB>, dim3(blocks_for(b, p2)), block, 0, b->ctx->stream,
+                     args_for(b, p2, k_off), s);
+  if (b->any_nofs)
+    hipLaunchKernelGGL(k_habc3<WPB>, dim3(blocks_for(b, p3)), block, 0, b->ctx->stream,
+                       args_for(b, p3, k_off), s);
+}
+
+static void launch_fsrec(const fomo_batch* b, int mode, int k_off) {
+  hipLaunchKernelGGL(k_fsrec<WPB>, dim3(blocks_for(b, PART_FSREC)), dim3(64 * WPB), 0,
+                     b->ctx->stream, args_for(b, PART_FSREC, k_off), mode);
+}
+
+// One fused step (time_stepper.jl:39-63): backup and injection ride in the
+// stencil kernels; record follows the free surface.
+static void launch_step(const fomo_batch* b, int k_off) {
+  launch_stencil(b, false, FLAG_BACKUP | FLAG_INJECT, k_off);
+  launch_habc(b, false, k_off);
+  launch_stencil(b, true, FLAG_BACKUP | FLAG_INJECT, k_off);
+  launch_habc(b, true, k_off);
+  launch_fsrec(b, MODE_FS | MODE_RECORD, k_off);
+}
+
+static void launch_set_step(const fomo_batch* b, int k) {
+  hipLaunchKernelGGL(k_set_step, dim3((b->n + 255) / 256), dim3(256), 0, b->ctx->stream, b->ds,
+                     b->dn, k);
+}
+static void launch_advance(const fomo_batch* b, int by) {
+  hipLaunchKernelGGL(k_advance, dim3((b->n + 255) / 256), dim3(256), 0, b->ctx->stream, b->ds,
+                     b->dn, by);
+}
+
+static int check_launch(fomo_ctx* c, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, FOMO_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return FOMO_OK;
+}
+
+static int run_batch(fomo_batch* b, int k_first, int n_steps) {
+  fomo_ctx* c = b->ctx;
+  if (n_steps < 0) return fail(c, FOMO_ERR_ARG, "n_steps < 0");
+  if (n_steps == 0) return FOMO_OK;
+  HIP_TRY(c, hipSetDevice(c->device));
+  launch_set_step(b, k_first);
+  const int G = b->graph_steps;
+  const int chunks = n_steps / G, rem = n_steps % G;
+  if (chunks > 0 && !b->gexec) {
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int g = 0; g < G; ++g) launch_step(b, g);
+    launch_advance(b, G);
+    hipGraph_t graph = nullptr;
+    HIP_TRY(c, hipStreamEndCapture(c->stream, &graph));
+    b->graph = graph;
+    HIP_TRY(c, hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0));
+  }
+  for (int i = 0; i < chunks; ++i) HIP_TRY(c, hipGraphLaunch(b->gexec, c->stream));
+  for (int g = 0; g < rem; ++g) launch_step(b, g);
+  if (rem) launch_advance(b, rem);
+  return check_launch(c, "time loop");
+}
+
+extern "C" int32_t fomo_run_time_loop(fomo_model* m, fomo_wavefield* w, fomo_shot* s,
+                                      int32_t k_first, int32_t n_steps) {
+  if (!m || !w || !s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_batch* b = nullptr;
+  int rc = single_batch(m, w, s, &b);
+  if (rc) return rc;
+  return run_batch(b, k_first, n_steps);
+}
+
+extern "C" int32_t fomo_time_step(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t k) {
+  if (!m || !w || !s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_batch* b = nullptr;
+  int rc = single_batch(m, w, s, &b);
+  if (rc) return rc;
+  HIP_TRY(m->ctx, hipSetDevice(m->ctx->device));
+  launch_set_step(b, k);
+  launch_step(b, 0);
+  return check_launch(m->ctx, "time_step");
+}
+
+// Per-stage entry points: the unfused kernels, one stage each.
+static int stage(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int k, int which) {
+  if (!m || !w) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_batch* b = nullptr;
+  int rc = single_batch(m, w, s, &b);
+  if (rc) return rc;
+  fomo_ctx* c = m->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  launch_set_step(b, s ? k : 1);
+  switch (which) {
+    case 0:
+      hipLaunchKernelGGL(k_backup<WPB>, dim3(blocks_for(b, PART_BACKUP)), dim3(64 * WPB), 0,
+                         c->stream, args_for(b, PART_BACKUP, 0));
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_inject, dim3(1), dim3(64), 0, c->stream, args_for(b, PART_SHOT, 0));
+      break;
+    case 2: launch_stencil(b, false, 0, 0); break;
+    case 3: launch_habc(b, false, 0); break;
+    case 4: launch_stencil(b, true, 0, 0); break;
+    case 5: launch_habc(b, true, 0); break;
+    case 6: launch_fsrec(b, MODE_FS, 0); break;
+    case 7: launch_fsrec(b, MODE_RECORD, 0); break;
+  }
+  return check_launch(c, "stage");
+}
+
+extern "C" int32_t fomo_backup_boundary(fomo_model* m, fomo_wavefield* w) { return stage(m, w, nullptr, 1, 0); }
+extern "C" int32_t fomo_inject_source(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t k) {
+  if (!s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL shot");
+  return stage(m, w, s, k, 1);
+}
+extern "C" int32_t fomo_update_velocity(fomo_model* m, fomo_wavefield* w) { return stage(m, w, nullptr, 1, 2); }
+extern "C" int32_t fomo_apply_habc_velocity(fomo_model* m, fomo_wavefield* w) { return stage(m, w, nullptr, 1, 3); }
+extern "C" int32_t fomo_update_stress(fomo_model* m, fomo_wavefield* w) { return stage(m, w, nullptr, 1, 4); }
+extern "C" int32_t fomo_apply_habc_stress(fomo_model* m, fomo_wavefield* w) { return stage(m, w, nullptr, 1, 5); }
+extern "C" int32_t fomo_apply_free_surface(fomo_model* m, fomo_wavefield* w) { return stage(m, w, nullptr, 1, 6); }
+extern "C" int32_t fomo_record_receivers(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t k) {
+  if (!s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL shot");
+  return stage(m, w, s, k, 7);
+}
+
+extern "C" int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t which,
+                                     int32_t k, int32_t reps) {
+  if (!m || !w || !s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  if (which < 0 || which > 4 || reps < 0) return fail(m->ctx, FOMO_ERR_ARG, "bad stage or reps");
+  fomo_batch* b = nullptr;
+  int rc = single_batch(m, w, s, &b);
+  if (rc) return rc;
+  HIP_TRY(m->ctx, hipSetDevice(m->ctx->device));
+  launch_set_step(b, k);
+  for (int r = 0; r < reps; ++r) {
+    switch (which) {
+      case 0: launch_stencil(b, false, FLAG_BACKUP | FLAG_INJECT, 0); break;
+      case 1: launch_stencil(b, true, FLAG_BACKUP | FLAG_INJECT, 0); break;
+      case 2: launch_habc(b, false, 0); break;
+      case 3: launch_habc(b, true, 0); break;
+      case 4: launch_fsrec(b, MODE_FS | MODE_RECORD, 0); break;
+    }
+  }
+  return check_launch(m->ctx, "launch_stage");
+}
+
+// ------------------------------------------------------------------ batch ABI
+extern "C" int32_t fomo_batch_create(fomo_ctx* c, int32_t n, fomo_model* const* ms,
+                                     fomo_wavefield* const* ws, fomo_shot* const* ss,
+                                     fomo_batch** out) {
+  if (!c || !ms || !ws || !ss || !out) return fail(c, FOMO_ERR_ARG, "NULL argument");
+  HIP_TRY(c, hipSetDevice(c->device));
+  return build_batch(c, n, ms, ws, ss, out);
+}
+
+extern "C" int32_t fomo_batch_destroy(fomo_batch* b) {
+  if (!b) return FOMO_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  destroy_batch(b);
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_batch_run(fomo_batch* b, int32_t k_first, int32_t n_steps) {
+  if (!b) return fail(nullptr, FOMO_ERR_ARG, "NULL batch");
+  return run_batch(b, k_first, n_steps);
+}

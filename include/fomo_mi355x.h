@@ -1,0 +1,183 @@
+/*
+ * fomo_mi355x.h -- C ABI of the MI355X (gfx950) time-stepping path of the
+ * Fomo_julia 2D elastic solver.  Plain pointers, sizes and opaque handles;
+ * no GPU or framework types cross this boundary, so Julia binds it with
+ * `ccall` (the reference's only FFI) and Python with ctypes.
+ *
+ * Conventions
+ *  - Every function returns int32: FOMO_OK (0) or a negative FOMO_ERR_* code;
+ *    fomo_ctx_last_error() returns the message of the last failure on that
+ *    context (owned by the context, valid until the next call on it).
+ *  - Arrays follow the reference's Julia layout: a field is (nx, nz)
+ *    column-major Float32 (element (i, j), 1-based, at (i-1) + (j-1)*nx);
+ *    HABC weights are (nz, nx) (src/utils/init.jl:193-195); a gather is
+ *    [nt x n_rec] column-major (src/simulation/shot_manager.jl:17-28).
+ *  - Grid indices passed in (source, receivers) are 1-based, as in Julia.
+ *  - Host pointers are borrowed for the duration of the call only.
+ *  - A context owns one HIP stream; all work is queued on it in call order.
+ *    A context and its objects are used by one host thread at a time.
+ *  - Device memory: one model (medium) is shared by any number of
+ *    wavefields and shots; a shot holds its wavelet, receivers and gather.
+ */
+#ifndef FOMO_MI355X_H
+#define FOMO_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FOMO_OK 0
+#define FOMO_ERR_ARG (-1)         /* invalid argument or size */
+#define FOMO_ERR_HIP (-2)         /* HIP runtime failure (message has the HIP error) */
+#define FOMO_ERR_UNSUPPORTED (-3) /* valid for the reference, not supported here */
+#define FOMO_ERR_STATE (-4)       /* object used with a mismatched model/context */
+
+/* Source types: MultiShotConfig.source_type (src/simulation/shot_manager.jl:99-102). */
+#define FOMO_SRC_PRESSURE 0 /* txx += w, tzz += w   (src/kernels/source_receiver.jl:18-30) */
+#define FOMO_SRC_FORCE_Z 1  /* vz += w*dt/rho_vz    (src/backend/cpu.jl:180-181) */
+#define FOMO_SRC_FORCE_X 2  /* vx += w*dt/rho_vx    (src/backend/cpu.jl:182-183) */
+
+/* Receiver types: Receivers.type (src/core/Structures.jl:131). */
+#define FOMO_REC_VZ 0
+#define FOMO_REC_VX 1
+#define FOMO_REC_P 2 /* (txx+tzz)/2, the CPU path's :p (source_receiver.jl:72-73) */
+
+/* Field ids for download/upload: Wavefield's components (Structures.jl:18-32). */
+#define FOMO_FIELD_VX 0
+#define FOMO_FIELD_VZ 1
+#define FOMO_FIELD_TXX 2
+#define FOMO_FIELD_TZZ 3
+#define FOMO_FIELD_TXZ 4
+#define FOMO_FIELD_VX_OLD 5
+#define FOMO_FIELD_VZ_OLD 6
+#define FOMO_FIELD_TXX_OLD 7
+#define FOMO_FIELD_TZZ_OLD 8
+#define FOMO_FIELD_TXZ_OLD 9
+
+#define FOMO_STORAGE_F32 0
+
+typedef struct fomo_ctx fomo_ctx;
+typedef struct fomo_model fomo_model;
+typedef struct fomo_wavefield fomo_wavefield;
+typedef struct fomo_shot fomo_shot;
+typedef struct fomo_batch fomo_batch;
+
+/* The scalars of Medium, HABCConfig and SimParams that the step needs. */
+typedef struct fomo_model_desc {
+  int32_t nx, nz;       /* padded grid: Medium.nx, Medium.nz (Structures.jl:67-68) */
+  int32_t pad;          /* Medium.pad = nbc + fd_order/2 (init.jl:72-73) */
+  int32_t nbc;          /* HABCConfig.nbc (Structures.jl:95); 1 <= nbc <= 63 */
+  int32_t fd_order;     /* SimParams.fd_order (Structures.jl:148); even, 2..16 */
+  int32_t free_surface; /* Medium.is_free_surface (Structures.jl:75) */
+  float dx, dz;         /* Medium.dx, Medium.dz */
+  float dt, dtx, dtz;   /* SimParams.dt, .dtx, .dtz (Structures.jl:144-147) */
+  float qx, qz, qt_x, qt_z, qxt; /* HABCConfig (Structures.jl:96-100) */
+  int32_t storage;      /* FOMO_STORAGE_F32 */
+} fomo_model_desc;
+
+/* ---- context ------------------------------------------------------------ */
+/* One HIP device + stream.  Replaces CUDA.jl's implicit device/stream of the
+ * CUDABackend (src/backends/backend.jl:20,68-79). */
+int32_t fomo_ctx_create(int32_t device, fomo_ctx** out);
+int32_t fomo_ctx_destroy(fomo_ctx* ctx);
+const char* fomo_ctx_last_error(const fomo_ctx* ctx);
+/* synchronize(::CUDABackend) (src/backends/backend.jl:78-79). */
+int32_t fomo_ctx_synchronize(fomo_ctx* ctx);
+/* The hipStream_t all work of this context is queued on (for event timing). */
+void* fomo_ctx_stream(fomo_ctx* ctx);
+
+/* ---- model: Medium + HABCConfig + SimParams + FD coefficients ----------- */
+/* Replaces to_device of the Medium arrays (init.jl:90-98), of the HABC
+ * weights (init.jl:197-202) and of the coefficient vector (run.jl:93).
+ * fd_coeffs: fd_order/2 values (get_fd_coefficients, init.jl:26-29).
+ * rho_vx..mu_txz: (nx, nz) arrays of Medium (Structures.jl:78-82).
+ * w_vx, w_vz, w_tau: (nz, nx) HABC weights; they must be the distance ramps
+ * init_habc builds (init.jl:188-195), else FOMO_ERR_UNSUPPORTED.  Passing
+ * NULL for all three builds those ramps from nbc directly. */
+int32_t fomo_model_create(fomo_ctx* ctx, const fomo_model_desc* desc, const float* fd_coeffs,
+                          const float* rho_vx, const float* rho_vz, const float* lam,
+                          const float* mu_txx, const float* mu_txz, const float* w_vx,
+                          const float* w_vz, const float* w_tau, fomo_model** out);
+int32_t fomo_model_destroy(fomo_model* m);
+/* Bytes of device memory the model holds. */
+int64_t fomo_model_device_bytes(const fomo_model* m);
+
+/* ---- wavefield ---------------------------------------------------------- */
+/* Wavefield(nx, nz, backend) (Structures.jl:39-55): 10 zeroed fields. */
+int32_t fomo_wavefield_create(fomo_model* m, fomo_wavefield** out);
+int32_t fomo_wavefield_destroy(fomo_wavefield* w);
+/* reset!(backend, W) (src/kernels/source_receiver.jl:115-129). */
+int32_t fomo_wavefield_reset(fomo_wavefield* w);
+/* Copy one field (FOMO_FIELD_*) to / from a host (nx, nz) array. */
+int32_t fomo_wavefield_download(fomo_wavefield* w, int32_t field, float* host);
+int32_t fomo_wavefield_upload(fomo_wavefield* w, int32_t field, const float* host);
+int64_t fomo_wavefield_device_bytes(const fomo_wavefield* w);
+
+/* ---- shot: Source + Receivers + gather ----------------------------------- */
+/* _create_source / _create_receivers (shot_manager.jl:196-222): source cell
+ * (src_i, src_j) as _coord_to_index returns it, wavelet of n_wavelet samples
+ * (steps beyond it inject nothing, source_receiver.jl:19-21), nt steps of
+ * gather for n_rec receivers at (rec_i[r], rec_j[r]) of type rec_type[r]
+ * (NULL = all FOMO_REC_VZ).  The gather starts zeroed. */
+int32_t fomo_shot_create(fomo_model* m, int32_t src_i, int32_t src_j, int32_t src_type,
+                         const float* wavelet, int32_t n_wavelet, int32_t nt,
+                         const int32_t* rec_i, const int32_t* rec_j, const int32_t* rec_type,
+                         int32_t n_rec, fomo_shot** out);
+int32_t fomo_shot_destroy(fomo_shot* s);
+/* fill!(rec.data, 0f0) (shot_manager.jl:54). */
+int32_t fomo_shot_reset(fomo_shot* s);
+/* _get_gather (shot_manager.jl:71-72): [nt x n_rec] column-major into out. */
+int32_t fomo_shot_gather(fomo_shot* s, float* out);
+
+/* ---- the time step ------------------------------------------------------ */
+/* time_step!(backend, W, M, H, a, src, rec, k, params) (time_stepper.jl:39-63), k 1-based. */
+int32_t fomo_time_step(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t k);
+/* run_time_loop! (time_stepper.jl:97-128) for steps k_first .. k_first+n_steps-1,
+ * replayed as captured HIP graphs of whole steps; asynchronous (call
+ * fomo_ctx_synchronize). */
+int32_t fomo_run_time_loop(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t k_first,
+                           int32_t n_steps);
+
+/* Per-stage entry points, each one kernel of the step (Elastic2D.jl:55-58). */
+int32_t fomo_backup_boundary(fomo_model* m, fomo_wavefield* w);             /* boundary.jl:45-56 */
+int32_t fomo_inject_source(fomo_model* m, fomo_wavefield* w, fomo_shot* s,
+                           int32_t k);                                      /* source_receiver.jl:18-30 */
+int32_t fomo_update_velocity(fomo_model* m, fomo_wavefield* w);             /* velocity.jl:21-46 */
+int32_t fomo_apply_habc_velocity(fomo_model* m, fomo_wavefield* w);         /* boundary.jl:139-144 */
+int32_t fomo_update_stress(fomo_model* m, fomo_wavefield* w);               /* stress.jl:18-47 */
+int32_t fomo_apply_habc_stress(fomo_model* m, fomo_wavefield* w);           /* boundary.jl:151-157 */
+int32_t fomo_apply_free_surface(fomo_model* m, fomo_wavefield* w);          /* boundary.jl:168-181 */
+int32_t fomo_record_receivers(fomo_model* m, fomo_wavefield* w, fomo_shot* s,
+                              int32_t k);                                   /* source_receiver.jl:65-77 */
+
+/* ---- batches: many shots (uneven grids allowed) in one launch per kernel ---
+ * Replaces the sequential shot loop of run_shots! (shot_manager.jl:145-180):
+ * shot b runs on wavefields[b] with models[b]; every model must have the same
+ * fd_order.  Each step of every shot is one launch per kernel for the whole
+ * batch (a work list over all shots' tiles). */
+int32_t fomo_batch_create(fomo_ctx* ctx, int32_t n, fomo_model* const* models,
+                          fomo_wavefield* const* wavefields, fomo_shot* const* shots,
+                          fomo_batch** out);
+int32_t fomo_batch_destroy(fomo_batch* b);
+/* Steps k_first .. k_first+n_steps-1 for every shot of the batch. */
+int32_t fomo_batch_run(fomo_batch* b, int32_t k_first, int32_t n_steps);
+
+/* ---- tuning ------------------------------------------------------------- */
+/* Stencil tile: 64*cols_per_lane columns (cols_per_lane in {1,2,4}) by
+ * rows_per_tile rows per wavefront; steps_per_graph = whole steps per
+ * captured graph.  Applies to runs created after the call on this context. */
+int32_t fomo_ctx_set_tiling(fomo_ctx* ctx, int32_t cols_per_lane, int32_t rows_per_tile,
+                            int32_t steps_per_graph);
+/* Queue `reps` launches of one stage kernel for (m, w, s) on the context's
+ * stream, for timing: stage 0 = velocity, 1 = stress (fused backup and
+ * injection, as in a step), 2 = HABC velocity, 3 = HABC stress,
+ * 4 = free surface + record. */
+int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t stage,
+                          int32_t k, int32_t reps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FOMO_MI355X_H */

@@ -1,0 +1,407 @@
+// ORACLE -- test infrastructure only.
+//
+// A CPU restatement of the reference's time-stepping path, used by tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.  The
+// product (fomo_julia_amd/) never links, loads or calls it.
+//
+// Arrays use the reference layout: (nx, nz) column-major, element (i, j)
+// (0-based here) at i + j*nx; HABC weights are (nz, nx): (j + i*nz).
+//
+// Functions and the reference code they follow (files under the reference's src/):
+//   stage_backup    kernels/boundary.jl:20-56     (copy_boundary_strip!, backup_boundary!)
+//   stage_inject    kernels/source_receiver.jl:18-30 (pressure);
+//                   backend/cpu.jl:171-187          (force_z / force_x, v1 semantics)
+//   stage_velocity  kernels/velocity.jl:21-46
+//   stage_habc      kernels/boundary.jl:68-132 (program order), :139-157
+//                   snapshot variant: kernels/boundary.jl:251-297 with every
+//                   neighbour read taken from the pre-HABC field
+//   stage_stress    kernels/stress.jl:18-47
+//   stage_fs        kernels/boundary.jl:168-181
+//   stage_record    kernels/source_receiver.jl:65-77
+//   time_step       simulation/time_stepper.jl:39-63
+//
+// Arithmetic modes (flag bits):
+//   ORACLE_FMA       accumulate with fused multiply-add:
+//                    acc += a*b -> fma(a, b, acc); e1*f1 + e2*f2 -> fma(e2, f2, e1*f1)
+//                    (the HIP kernels use exactly this, so FP32 parity is bit-exact)
+//                    otherwise every product and sum is rounded separately.
+//   ORACLE_SNAPSHOT  HABC neighbour reads from the pre-HABC snapshot, one
+//                    formula per cell (the CUDA kernel without its race);
+//                    otherwise the literal program order of the CPU loops.
+//
+// Pinning: see tests/test_oracle.py (polynomial-exactness of the stencils,
+// closed-form HABC cases, free surface, symmetry, 2^k linearity, energy decay,
+// the reference's shot_1.bin gather).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+enum { ORACLE_FMA = 1, ORACLE_SNAPSHOT = 2 };
+enum { SRC_PRESSURE = 0, SRC_FORCE_Z = 1, SRC_FORCE_X = 2 };
+enum { REC_VZ = 0, REC_VX = 1, REC_P = 2 };
+
+extern "C" {
+// Plain description of one run (all indices 1-based as in Julia).
+typedef struct {
+  int32_t nx, nz, pad, nbc, fd_order, free_surface;
+  double dt, dtx, dtz;
+  double qx, qz, qt_x, qt_z, qxt;
+  int32_t flags;
+  int32_t threads;
+} oracle_desc;
+}
+
+namespace {
+
+template <typename T>
+struct Model {
+  const oracle_desc* d;
+  int nx, nz, M, nbc, pad;
+  bool fs, fma_mode, snapshot;
+  T dt, dtx, dtz, qx, qz, qt_x, qt_z, qxt;
+  const T* a;
+  const T *rho_vx, *rho_vz, *lam, *mu_txx, *mu_txz;
+  const T *w_vx, *w_vz, *w_tau;
+  T* f[10];  // vx vz txx tzz txz, then the *_old of each
+};
+
+template <typename T>
+static inline T madd(bool fm, T a, T b, T c) {  // a*b + c
+  return fm ? std::fma(a, b, c) : (T)(a * b) + c;
+}
+
+// kernels/boundary.jl:20-38
+template <typename T>
+static void copy_strip(T* old, const T* cur, int nx, int nz, int nbc, bool fs) {
+  int j_top = fs ? nbc + 1 : 1;
+  auto cp = [&](int i, int j) {  // 1-based
+    size_t o = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+    old[o] = cur[o];
+  };
+  for (int i = 1; i <= nbc + 2; ++i)
+    for (int j = j_top; j <= nz; ++j) cp(i, j);
+  for (int i = nx - nbc - 1; i <= nx; ++i)
+    for (int j = j_top; j <= nz; ++j) cp(i, j);
+  for (int j = j_top; j <= nbc + 2; ++j)
+    for (int i = nbc + 3; i <= nx - nbc - 2; ++i) cp(i, j);
+  for (int j = nz - nbc - 1; j <= nz; ++j)
+    for (int i = nbc + 3; i <= nx - nbc - 2; ++i) cp(i, j);
+}
+
+template <typename T>
+static void stage_backup(Model<T>& m) {
+  for (int c = 0; c < 5; ++c) copy_strip(m.f[5 + c], m.f[c], m.nx, m.nz, m.nbc, m.fs);
+}
+
+template <typename T>
+static void stage_inject(Model<T>& m, int src_i, int src_j, int src_type, const T* w, int nw, int k) {
+  if (k > nw || k < 1) return;
+  T wav = w[k - 1];
+  size_t o = (size_t)(src_i - 1) + (size_t)(src_j - 1) * m.nx;
+  if (src_type == SRC_PRESSURE) {
+    m.f[2][o] += wav;
+    m.f[3][o] += wav;
+  } else if (src_type == SRC_FORCE_Z) {
+    T s = (T)(m.dt / m.rho_vz[o]);
+    m.f[1][o] += (T)(wav * s);
+  } else if (src_type == SRC_FORCE_X) {
+    T s = (T)(m.dt / m.rho_vx[o]);
+    m.f[0][o] += (T)(wav * s);
+  }
+}
+
+// kernels/velocity.jl:21-46
+template <typename T>
+static void stage_velocity(Model<T>& m) {
+  const int nx = m.nx, nz = m.nz, M = m.M;
+  T *vx = m.f[0], *vz = m.f[1];
+  const T *txx = m.f[2], *tzz = m.f[3], *txz = m.f[4];
+  const bool fm = m.fma_mode;
+#pragma omp parallel for schedule(static) num_threads(m.d->threads)
+  for (int j = M; j < nz - M; ++j) {
+    for (int i = M; i < nx - M; ++i) {
+      auto at = [&](const T* p, int ii, int jj) { return p[(size_t)ii + (size_t)jj * nx]; };
+      T d1 = 0, d2 = 0, d3 = 0, d4 = 0;
+      for (int l = 1; l <= M; ++l) {
+        T al = m.a[l - 1];
+        d1 = madd<T>(fm, al, at(txx, i + l - 1, j) - at(txx, i - l, j), d1);
+        d2 = madd<T>(fm, al, at(txz, i, j + l - 1) - at(txz, i, j - l), d2);
+        d3 = madd<T>(fm, al, at(txz, i + l, j) - at(txz, i - l + 1, j), d3);
+        d4 = madd<T>(fm, al, at(tzz, i, j + l) - at(tzz, i, j - l + 1), d4);
+      }
+      size_t o = (size_t)i + (size_t)j * nx;
+      T bx = m.dtx / m.rho_vx[o], bz = m.dtz / m.rho_vx[o];
+      T inc = madd<T>(fm, bz, d2, (T)(bx * d1));
+      vx[o] = vx[o] + inc;
+      bx = m.dtx / m.rho_vz[o];
+      bz = m.dtz / m.rho_vz[o];
+      inc = madd<T>(fm, bz, d4, (T)(bx * d3));
+      vz[o] = vz[o] + inc;
+    }
+  }
+}
+
+// kernels/stress.jl:18-47
+template <typename T>
+static void stage_stress(Model<T>& m) {
+  const int nx = m.nx, nz = m.nz, M = m.M;
+  const T *vx = m.f[0], *vz = m.f[1];
+  T *txx = m.f[2], *tzz = m.f[3], *txz = m.f[4];
+  const bool fm = m.fma_mode;
+#pragma omp parallel for schedule(static) num_threads(m.d->threads)
+  for (int j = M; j < nz - M; ++j) {
+    for (int i = M; i < nx - M; ++i) {
+      auto at = [&](const T* p, int ii, int jj) { return p[(size_t)ii + (size_t)jj * nx]; };
+      T d1 = 0, d2 = 0, d3 = 0, d4 = 0;
+      for (int l = 1; l <= M; ++l) {
+        T al = m.a[l - 1];
+        d1 = madd<T>(fm, al, at(vx, i + l, j) - at(vx, i - l + 1, j), d1);      // dvxdx
+        d2 = madd<T>(fm, al, at(vz, i, j + l - 1) - at(vz, i, j - l), d2);      // dvzdz
+        d3 = madd<T>(fm, al, at(vx, i, j + l) - at(vx, i, j - l + 1), d3);      // dvxdz
+        d4 = madd<T>(fm, al, at(vz, i + l - 1, j) - at(vz, i - l, j), d4);      // dvzdx
+      }
+      size_t o = (size_t)i + (size_t)j * nx;
+      T l_val = m.lam[o], m_val = m.mu_txx[o];
+      T l2m = l_val + (T)2 * m_val;
+      T t1 = d1 * m.dtx, t2 = d2 * m.dtz;
+      txx[o] = txx[o] + madd<T>(fm, l_val, t2, (T)(l2m * t1));
+      tzz[o] = tzz[o] + madd<T>(fm, l2m, t2, (T)(l_val * t1));
+      T s = madd<T>(fm, d4, m.dtx, (T)(d3 * m.dtz));
+      txz[o] = txz[o] + (T)(m.mu_txz[o] * s);
+    }
+  }
+}
+
+// One HABC blend of f(i,j) (1-based) given its neighbour values.
+template <typename T>
+struct Habc {
+  T qx, qz, qt_x, qt_z, qxt;
+  bool fm;
+  T sumx(T fn, T fo, T fon) const {  // -qx*fn - qt_x*fo - qxt*fon
+    T s = (T)(-qx * fn);
+    s = madd<T>(fm, -qt_x, fo, s);
+    return madd<T>(fm, -qxt, fon, s);
+  }
+  T sumz(T fn, T fo, T fon) const {
+    T s = (T)(-qz * fn);
+    s = madd<T>(fm, -qt_z, fo, s);
+    return madd<T>(fm, -qxt, fon, s);
+  }
+  T edge(T w, T f, T s) const { return madd<T>(fm, (T)((T)1 - w), s, (T)(w * f)); }
+  T corner(T w, T f, T sx, T sz) const {
+    T c = (T)((T)((T)1 - w) * (T)0.5);
+    return madd<T>(fm, c, (T)(sx + sz), (T)(w * f));
+  }
+};
+
+// kernels/boundary.jl:68-132, literal program order (regions L, R, B, T, LB, RB, LT, RT;
+// j outer, i inner, every read sees the current array).
+template <typename T>
+static void habc_program_order(Model<T>& m, T* f, const T* fo, const T* w) {
+  const int nx = m.nx, nz = m.nz, nbc = m.nbc;
+  Habc<T> h{m.qx, m.qz, m.qt_x, m.qt_z, m.qxt, m.fma_mode};
+  auto F = [&](int i, int j) -> T& { return f[(size_t)(i - 1) + (size_t)(j - 1) * nx]; };
+  auto O = [&](int i, int j) { return fo[(size_t)(i - 1) + (size_t)(j - 1) * nx]; };
+  auto W = [&](int i, int j) { return w[(size_t)(j - 1) + (size_t)(i - 1) * nz]; };
+  const int js = m.fs ? nbc + 1 : 2;
+  for (int j = js; j <= nz - nbc - 1; ++j)  // left
+    for (int i = 2; i <= nbc + 1; ++i)
+      F(i, j) = h.edge(W(i, j), F(i, j), h.sumx(F(i + 1, j), O(i, j), O(i + 1, j)));
+  for (int j = js; j <= nz - nbc - 1; ++j)  // right
+    for (int i = nx - nbc; i <= nx - 1; ++i)
+      F(i, j) = h.edge(W(i, j), F(i, j), h.sumx(F(i - 1, j), O(i, j), O(i - 1, j)));
+  for (int j = nz - nbc; j <= nz - 1; ++j)  // bottom
+    for (int i = nbc + 2; i <= nx - nbc - 1; ++i)
+      F(i, j) = h.edge(W(i, j), F(i, j), h.sumz(F(i, j - 1), O(i, j), O(i, j - 1)));
+  if (!m.fs)
+    for (int j = 2; j <= nbc + 1; ++j)  // top
+      for (int i = nbc + 2; i <= nx - nbc - 1; ++i)
+        F(i, j) = h.edge(W(i, j), F(i, j), h.sumz(F(i, j + 1), O(i, j), O(i, j + 1)));
+  for (int j = nz - nbc; j <= nz - 1; ++j)  // left-bottom
+    for (int i = 2; i <= nbc + 1; ++i)
+      F(i, j) = h.corner(W(i, j), F(i, j), h.sumx(F(i + 1, j), O(i, j), O(i + 1, j)),
+                         h.sumz(F(i, j - 1), O(i, j), O(i, j - 1)));
+  for (int j = nz - nbc; j <= nz - 1; ++j)  // right-bottom
+    for (int i = nx - nbc; i <= nx - 1; ++i)
+      F(i, j) = h.corner(W(i, j), F(i, j), h.sumx(F(i - 1, j), O(i, j), O(i - 1, j)),
+                         h.sumz(F(i, j - 1), O(i, j), O(i, j - 1)));
+  if (!m.fs) {
+    for (int j = 2; j <= nbc + 1; ++j)  // left-top
+      for (int i = 2; i <= nbc + 1; ++i)
+        F(i, j) = h.corner(W(i, j), F(i, j), h.sumx(F(i + 1, j), O(i, j), O(i + 1, j)),
+                           h.sumz(F(i, j + 1), O(i, j), O(i, j + 1)));
+    for (int j = 2; j <= nbc + 1; ++j)  // right-top
+      for (int i = nx - nbc; i <= nx - 1; ++i)
+        F(i, j) = h.corner(W(i, j), F(i, j), h.sumx(F(i - 1, j), O(i, j), O(i - 1, j)),
+                           h.sumz(F(i, j + 1), O(i, j), O(i, j + 1)));
+  }
+}
+
+// kernels/boundary.jl:251-297 with all reads from the pre-HABC snapshot.
+template <typename T>
+static void habc_snapshot(Model<T>& m, T* f, const T* fo, const T* w) {
+  const int nx = m.nx, nz = m.nz, nbc = m.nbc;
+  Habc<T> h{m.qx, m.qz, m.qt_x, m.qt_z, m.qxt, m.fma_mode};
+  std::vector<T> s(f, f + (size_t)nx * nz);
+  auto S = [&](int i, int j) { return s[(size_t)(i - 1) + (size_t)(j - 1) * nx]; };
+  auto O = [&](int i, int j) { return fo[(size_t)(i - 1) + (size_t)(j - 1) * nx]; };
+  auto W = [&](int i, int j) { return w[(size_t)(j - 1) + (size_t)(i - 1) * nz]; };
+  const int js = m.fs ? nbc + 1 : 2;
+  for (int j = js; j <= nz - 1; ++j)
+    for (int i = 2; i <= nx - 1; ++i) {
+      bool L = i <= nbc + 1, R = i >= nx - nbc, B = j >= nz - nbc, Tp = !m.fs && j <= nbc + 1;
+      T wt = W(i, j), v = S(i, j), out;
+      if (L && !B && !Tp) out = h.edge(wt, v, h.sumx(S(i + 1, j), O(i, j), O(i + 1, j)));
+      else if (R && !B && !Tp) out = h.edge(wt, v, h.sumx(S(i - 1, j), O(i, j), O(i - 1, j)));
+      else if (B && !L && !R) out = h.edge(wt, v, h.sumz(S(i, j - 1), O(i, j), O(i, j - 1)));
+      else if (Tp && !L && !R) out = h.edge(wt, v, h.sumz(S(i, j + 1), O(i, j), O(i, j + 1)));
+      else if (L && B) out = h.corner(wt, v, h.sumx(S(i + 1, j), O(i, j), O(i + 1, j)), h.sumz(S(i, j - 1), O(i, j), O(i, j - 1)));
+      else if (R && B) out = h.corner(wt, v, h.sumx(S(i - 1, j), O(i, j), O(i - 1, j)), h.sumz(S(i, j - 1), O(i, j), O(i, j - 1)));
+      else if (L && Tp) out = h.corner(wt, v, h.sumx(S(i + 1, j), O(i, j), O(i + 1, j)), h.sumz(S(i, j + 1), O(i, j), O(i, j + 1)));
+      else if (R && Tp) out = h.corner(wt, v, h.sumx(S(i - 1, j), O(i, j), O(i - 1, j)), h.sumz(S(i, j + 1), O(i, j), O(i, j + 1)));
+      else continue;
+      f[(size_t)(i - 1) + (size_t)(j - 1) * nx] = out;
+    }
+}
+
+template <typename T>
+static void stage_habc(Model<T>& m, int which) {  // 0 = velocity, 1 = stress
+  auto run = [&](int c, const T* w) {
+    if (m.snapshot) habc_snapshot(m, m.f[c], m.f[5 + c], w);
+    else habc_program_order(m, m.f[c], m.f[5 + c], w);
+  };
+  if (which == 0) {
+    run(0, m.w_vx);
+    run(1, m.w_vz);
+  } else {
+    run(2, m.w_tau);
+    run(3, m.w_tau);
+    run(4, m.w_tau);
+  }
+}
+
+// kernels/boundary.jl:168-181
+template <typename T>
+static void stage_fs(Model<T>& m) {
+  if (!m.fs) return;
+  int j_fs = m.pad + 1;
+  for (int j = j_fs - 5; j <= j_fs; ++j) {
+    if (j < 1 || j > m.nz) continue;
+    for (int i = 1; i <= m.nx; ++i) {
+      size_t o = (size_t)(i - 1) + (size_t)(j - 1) * m.nx;
+      m.f[3][o] = 0;
+      m.f[4][o] = 0;
+    }
+  }
+}
+
+// kernels/source_receiver.jl:65-77 (gather is [nt x n_rec] column-major)
+template <typename T>
+static void stage_record(Model<T>& m, const int* ri, const int* rj, const int* rt, int n_rec,
+                         int nt, int k, T* gather) {
+  if (k < 1 || k > nt) return;
+  for (int r = 0; r < n_rec; ++r) {
+    size_t o = (size_t)(ri[r] - 1) + (size_t)(rj[r] - 1) * m.nx;
+    T v;
+    if (rt[r] == REC_VZ) v = m.f[1][o];
+    else if (rt[r] == REC_VX) v = m.f[0][o];
+    else v = (T)(m.f[2][o] + m.f[3][o]) * (T)0.5;
+    gather[(size_t)(k - 1) + (size_t)r * nt] = v;
+  }
+}
+
+template <typename T>
+static Model<T> make_model(const oracle_desc* d, const T* a, const T* const* med, const T* const* w,
+                           T* fields) {
+  Model<T> m;
+  m.d = d;
+  m.nx = d->nx;
+  m.nz = d->nz;
+  m.M = d->fd_order / 2;
+  m.nbc = d->nbc;
+  m.pad = d->pad;
+  m.fs = d->free_surface != 0;
+  m.fma_mode = (d->flags & ORACLE_FMA) != 0;
+  m.snapshot = (d->flags & ORACLE_SNAPSHOT) != 0;
+  m.dt = (T)d->dt;
+  m.dtx = (T)d->dtx;
+  m.dtz = (T)d->dtz;
+  m.qx = (T)d->qx;
+  m.qz = (T)d->qz;
+  m.qt_x = (T)d->qt_x;
+  m.qt_z = (T)d->qt_z;
+  m.qxt = (T)d->qxt;
+  m.a = a;
+  m.rho_vx = med[0];
+  m.rho_vz = med[1];
+  m.lam = med[2];
+  m.mu_txx = med[3];
+  m.mu_txz = med[4];
+  m.w_vx = w[0];
+  m.w_vz = w[1];
+  m.w_tau = w[2];
+  size_t n = (size_t)d->nx * d->nz;
+  for (int c = 0; c < 10; ++c) m.f[c] = fields + c * n;
+  return m;
+}
+
+template <typename T>
+static int run_steps(const oracle_desc* d, const T* a, const T* const* med, const T* const* w,
+                     T* fields, int src_i, int src_j, int src_type, const T* wavelet, int nw,
+                     const int* ri, const int* rj, const int* rt, int n_rec, int nt, int k_first,
+                     int n_steps, T* gather, int stage_mask) {
+  if (d->nx < 2 * (d->nbc + 3) || d->nz < 2 * (d->nbc + 3)) return -1;
+  if (d->fd_order < 2 || d->fd_order % 2) return -2;
+  Model<T> m = make_model<T>(d, a, med, w, fields);
+  for (int k = k_first; k < k_first + n_steps; ++k) {
+    if (stage_mask & 1) stage_backup(m);
+    if (stage_mask & 2) stage_inject(m, src_i, src_j, src_type, wavelet, nw, k);
+    if (stage_mask & 4) stage_velocity(m);
+    if (stage_mask & 8) stage_habc(m, 0);
+    if (stage_mask & 16) stage_stress(m);
+    if (stage_mask & 32) stage_habc(m, 1);
+    if (stage_mask & 64) stage_fs(m);
+    if (stage_mask & 128) stage_record(m, ri, rj, rt, n_rec, nt, k, gather);
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Runs time steps k_first .. k_first+n_steps-1 (1-based, time_stepper.jl:108-109)
+// over `fields` (10 arrays of nx*nz, order vx vz txx tzz txz + *_old), writing
+// gather[(k-1) + r*nt].  stage_mask selects stages (bit order of time_step!:
+// backup, inject, velocity, habc_v, stress, habc_s, free surface, record);
+// 255 = the whole step.  med = {rho_vx, rho_vz, lam, mu_txx, mu_txz};
+// w = {w_vx, w_vz, w_tau} in the reference's (nz, nx) layout.
+int oracle_run_f32(const oracle_desc* d, const float* a, const float* const* med,
+                   const float* const* w, float* fields, int src_i, int src_j, int src_type,
+                   const float* wavelet, int nw, const int* ri, const int* rj, const int* rt,
+                   int n_rec, int nt, int k_first, int n_steps, float* gather, int stage_mask) {
+  return run_steps<float>(d, a, med, w, fields, src_i, src_j, src_type, wavelet, nw, ri, rj, rt,
+                          n_rec, nt, k_first, n_steps, gather, stage_mask);
+}
+
+int oracle_run_f64(const oracle_desc* d, const double* a, const double* const* med,
+                   const double* const* w, double* fields, int src_i, int src_j, int src_type,
+                   const double* wavelet, int nw, const int* ri, const int* rj, const int* rt,
+                   int n_rec, int nt, int k_first, int n_steps, double* gather, int stage_mask) {
+  return run_steps<double>(d, a, med, w, fields, src_i, src_j, src_type, wavelet, nw, ri, rj, rt,
+                           n_rec, nt, k_first, n_steps, gather, stage_mask);
+}
+
+int oracle_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

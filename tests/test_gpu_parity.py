@@ -1,0 +1,203 @@
+"""MI355X kernels against the CPU oracle, through the C ABI.
+
+FP32 results are compared bit for bit with the oracle's FMA mode (the kernels
+and the oracle share one arithmetic contract); the oracle's unfused mode and
+its Float64 run are compared by relative L2 (north-star tolerance 1e-5 for
+FP32 gathers)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fomo_julia_amd import elastic2d as E
+from tests.helpers import Case, rel_l2, run_jl_case
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "shot_1.bin")
+
+
+@pytest.fixture(scope="module")
+def b():
+    bk = E.backend("mi355x")
+    bk.set_tiling(2, 32, 16)
+    return bk
+
+
+def gpu_shot(b, c, cpt=2, tz=32, gsteps=16):
+    b.set_tiling(cpt, tz, gsteps)
+    W = E.Wavefield(c.M.nx, c.M.nz, b)
+    res = E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p)
+    return res, W
+
+
+def assert_same_state(W, r, what=""):
+    for f in range(10):
+        got = W.download(f)
+        want = r.field(f)
+        if not np.array_equal(got, want):
+            bad = np.argwhere(got != want)
+            i, j = bad[0]
+            raise AssertionError(f"{what} field {f}: {len(bad)} cells differ, first (i={i}, j={j}) "
+                                 f"got {got[i, j]!r} want {want[i, j]!r}")
+
+
+@pytest.mark.parametrize("fs", [True, False])
+@pytest.mark.parametrize("src_type", ["pressure", "force_z", "force_x"])
+def test_full_shot_bit_exact(b, fs, src_type):
+    c = Case(fs=fs, src_type=src_type, nt=150, water=fs)
+    print(f"case fs={fs} src={src_type} grid={c.M.nx}x{c.M.nz}", flush=True)
+    res, W = gpu_shot(b, c)
+    r = c.oracle()
+    r.run(1, c.nt)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    assert_same_state(W, r)
+
+
+@pytest.mark.parametrize("fd_order", [2, 4, 6, 10, 12, 16])
+def test_orders_bit_exact(b, fd_order):
+    c = Case(fd_order=fd_order, nt=80, nbc=10)
+    print(f"case order={fd_order}", flush=True)
+    res, W = gpu_shot(b, c)
+    r = c.oracle()
+    r.run(1, c.nt)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    assert_same_state(W, r)
+
+
+@pytest.mark.parametrize("cpt,tz,gsteps", [(1, 8, 1), (2, 17, 5), (4, 64, 32), (4, 5, 16)])
+def test_tilings_bit_exact(b, cpt, tz, gsteps):
+    c = Case(nx_in=150, nz_in=90, nbc=20, nt=70)
+    print(f"case cpt={cpt} tz={tz} gsteps={gsteps}", flush=True)
+    res, W = gpu_shot(b, c, cpt, tz, gsteps)
+    r = c.oracle()
+    r.run(1, c.nt)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    assert_same_state(W, r)
+    b.set_tiling(2, 32, 16)
+
+
+STAGES = [("backup", E.backup_boundary), ("velocity", E.update_velocity),
+          ("habc_v", E.apply_habc_velocity), ("stress", E.update_stress),
+          ("habc_s", E.apply_habc_stress), ("free_surface", E.apply_free_surface)]
+
+
+@pytest.mark.parametrize("fs", [True, False])
+@pytest.mark.parametrize("name,fn", STAGES)
+def test_each_stage_bit_exact_on_random_state(b, fs, name, fn):
+    c = Case(fs=fs, nt=10)
+    print(f"case stage={name} fs={fs}", flush=True)
+    W = E.Wavefield(c.M.nx, c.M.nz, b)
+    E.bind(b, W, c.M, c.H, c.a, c.p)
+    r = c.oracle()
+    rng = np.random.default_rng(11)
+    for f in range(10):
+        v = rng.standard_normal((c.M.nx, c.M.nz)).astype(F32)
+        W.upload(f, v)
+        r.field(f)[:] = v
+    fn(b, W)
+    b.synchronize()
+    r.run(1, 1, O.STAGES[name])
+    assert_same_state(W, r, name)
+
+
+def test_inject_and_record_stages(b):
+    c = Case(nt=12, src_type="pressure")
+    W = E.Wavefield(c.M.nx, c.M.nz, b)
+    E.bind(b, W, c.M, c.H, c.a, c.p)
+    r = c.oracle()
+    rng = np.random.default_rng(5)
+    for f in range(5):
+        v = rng.standard_normal((c.M.nx, c.M.nz)).astype(F32)
+        W.upload(f, v)
+        r.field(f)[:] = v
+    E.inject_source(b, W, c.src, 7, rec=c.rec)
+    b.synchronize()
+    r.run(7, 1, O.STAGES["inject"])
+    assert_same_state(W, r, "inject")
+    E.record_receivers(b, W, c.rec, 7, src=c.src, nt=c.nt)
+    r.run(7, 1, O.STAGES["record"])
+    assert np.array_equal(c.rec.data, r.gather_nt_by_nrec())
+
+
+def test_time_step_api_matches_run_loop(b):
+    c = Case(nt=40)
+    W = E.Wavefield(c.M.nx, c.M.nz, b)
+    E.bind(b, W, c.M, c.H, c.a, c.p)
+    for k in range(1, 41):
+        E.time_step(b, W, c.M, c.H, c.a, c.src, c.rec, k, c.p)
+    b.synchronize()
+    r = c.oracle()
+    r.run(1, 40)
+    assert_same_state(W, r, "time_step")
+
+
+def test_batched_uneven_grids_bit_exact(b):
+    cases = [Case(nx_in=72, nz_in=56, nt=90, seed=1),
+             Case(nx_in=130, nz_in=64, nt=90, seed=2, fs=False),
+             Case(nx_in=60, nz_in=90, nt=90, seed=3, src_type="force_z")]
+    import ctypes
+    from fomo_julia_amd import _lib
+    lib = _lib.load()
+    b.set_tiling(2, 16, 8)
+    hs = []
+    for c in cases:
+        dm = E._model(b, c.M, c.H, c.a, c.p)
+        W = E.Wavefield(c.M.nx, c.M.nz, b)
+        W._bind(dm)
+        ds = E._DeviceShot(dm, c.src, c.rec, c.nt)
+        hs.append((dm, W, ds))
+    n = len(hs)
+    arr = lambda xs: (ctypes.c_void_p * n)(*[x.value for x in xs])
+    bh = ctypes.c_void_p()
+    b.check(lib.fomo_batch_create(b.ctx, n, arr([h[0].h for h in hs]), arr([h[1]._h for h in hs]),
+                                  arr([h[2].h for h in hs]), ctypes.byref(bh)), "batch_create")
+    b.check(lib.fomo_batch_run(bh, 1, 50), "batch_run")
+    b.check(lib.fomo_batch_run(bh, 51, 40), "batch_run")
+    b.synchronize()
+    lib.fomo_batch_destroy(bh)
+    for c, (dm, W, ds) in zip(cases, hs):
+        r = c.oracle()
+        r.run(1, c.nt)
+        assert np.array_equal(ds.gather(), r.gather_nt_by_nrec())
+        assert_same_state(W, r, "batch")
+    b.set_tiling(2, 32, 16)
+
+
+def test_run_shots_batched_equals_sequential(b):
+    c = Case(nx_in=90, nz_in=50, nt=60)
+    xs = [F32(100.0), F32(300.0), F32(550.0)]
+    cfg = E.MultiShotConfig(xs, [F32(30.0)] * 3, c.src.wavelet)
+    W = E.Wavefield(c.M.nx, c.M.nz, b)
+    seq = E.run_shots(b, W, c.M, c.H, c.a, c.rec, cfg, c.p, batch=1)
+    seen = []
+    bat = E.run_shots(b, W, c.M, c.H, c.a, c.rec, cfg, c.p, batch=3,
+                      on_shot_complete=lambda r: seen.append(r.shot_id))
+    assert seen == [1, 2, 3]
+    for s, t in zip(seq, bat):
+        assert (s.src_i, s.src_j, s.shot_id) == (t.src_i, t.src_j, t.shot_id)
+        assert np.array_equal(s.gather, t.gather)
+
+
+def test_reference_golden_gather_and_fp64_error(b):
+    d = run_jl_case()
+    W = E.Wavefield(d["M"].nx, d["M"].nz, b)
+    res = E.run_shot(b, W, d["M"], d["H"], d["a"], d["src"], d["rec"], d["p"])
+    gold = np.fromfile(GOLDEN, dtype="<f4").reshape(100, 800).T
+    err = rel_l2(res.gather, gold)
+    print(f"run.jl gather vs reference shot_1.bin: rel L2 {err:.3e}")
+    assert err < 1e-5
+
+
+def test_fp32_gather_within_tolerance_of_unfused_and_fp64_oracle(b):
+    c = Case(nx_in=120, nz_in=80, nbc=20, nt=300)
+    res, W = gpu_shot(b, c)
+    sep = c.oracle(flags=0)
+    sep.run(1, c.nt)
+    r64 = c.oracle(dtype=np.float64)
+    r64.run(1, c.nt)
+    e_sep = rel_l2(res.gather, sep.gather_nt_by_nrec())
+    e_64 = rel_l2(res.gather, r64.gather_nt_by_nrec())
+    print(f"gather rel L2: vs unfused FP32 oracle {e_sep:.2e}, vs FP64 oracle {e_64:.2e}")
+    assert e_sep <= 1e-5 and e_64 <= 1e-5
