@@ -26,19 +26,10 @@ enum Field { F_VX = 0, F_VZ = 1, F_TXX = 2, F_TZZ = 3, F_TXZ = 4 };
 enum Ramp { RAMP_VX = 0, RAMP_VZ = 1, RAMP_TAU = 2 };
 
 // Work-partition kinds; each has a prefix-sum table over the shots of a batch.
-// HABC runs in three dependency phases per field (program order of the
-// reference's regions): phase 1 = left, right, bottom edges; phase 2 = top
-// edge (no free surface) and the bottom corners; phase 3 = top corners (no
-// free surface).  Every piece of a phase reads only cells that no other piece
-// of the same phase writes.
 enum Part {
   PART_STENCIL = 0,   // velocity / stress tiles
-  PART_HABC1_V,       // phase-1 pieces x 2 velocity fields
-  PART_HABC1_S,       // phase-1 pieces x 3 stress fields
-  PART_HABC2_V,
-  PART_HABC2_S,
-  PART_HABC3_V,
-  PART_HABC3_S,
+  PART_HABC_V,        // HABC strip pieces x 2 velocity fields
+  PART_HABC_S,        // HABC strip pieces x 3 stress fields
   PART_FSREC,         // free-surface rows + receivers
   PART_BACKUP,        // standalone strip backup (per-stage API)
   PART_SHOT,          // one item per shot
@@ -52,10 +43,7 @@ struct ShotDev {
   int nx, nz, pitch, nbc, pad, fs;
   int tz;               // rows per stencil tile
   int tiles_x, tiles_z; // stencil tiling
-  int edge_r0;          // first row (0-based) of the left/right HABC edges
-  int edge_rows;        // rows of the left/right HABC edges
-  int edge_cols;        // columns of the bottom/top HABC edges
-  int n_l, n_r, n_b, n_t;  // edge pieces per field
+  int n_l, n_r, n_b, n_t;  // HABC side/bottom/top edge pieces per field
   float dtx, dtz;
   float* f[5];          // vx vz txx tzz txz   (pointer to cell (0,0))
   float* fo[5];         // strip backups (*_old)

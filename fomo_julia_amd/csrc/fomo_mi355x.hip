@@ -30,8 +30,9 @@ struct fomo_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   int cpt = 2;         // columns per lane of the stencil tiles
-  int tz = 32;         // rows per stencil tile
+  int tz = 0;          // rows per stencil tile; 0 = sized to fill the GPU in whole rounds
   int graph_steps = 16;
+  int cus = 0;         // compute units of the device
 };
 
 struct fomo_model {
@@ -79,7 +80,7 @@ static uint64_t g_next_shot_id = 1;
 struct fomo_batch {
   fomo_ctx* ctx = nullptr;
   int n = 0, M = 0, cpt = 2, tz = 32, graph_steps = 16;
-  bool any_nofs = false;
+  int tz_req = 0;  // the context's rows_per_tile when built (0 = auto)
   std::vector<ShotDev> hs;
   ShotDev* ds = nullptr;
   int* dn = nullptr;
@@ -129,6 +130,7 @@ extern "C" int32_t fomo_ctx_create(int32_t device, fomo_ctx** out) {
   c->device = device;
   HIP_TRY(nullptr, hipSetDevice(device));
   HIP_TRY(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(nullptr, hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device));
   *out = c.release();
   return FOMO_OK;
 }
@@ -158,7 +160,7 @@ extern "C" void* fomo_ctx_stream(fomo_ctx* c) { return c ? (void*)c->stream : nu
 extern "C" int32_t fomo_ctx_set_tiling(fomo_ctx* c, int32_t cpt, int32_t tz, int32_t gsteps) {
   if (!c) return fail(nullptr, FOMO_ERR_ARG, "ctx is NULL");
   if (cpt != 1 && cpt != 2 && cpt != 4) return fail(c, FOMO_ERR_ARG, "cols_per_lane must be 1, 2 or 4");
-  if (tz < 1 || tz > 4096) return fail(c, FOMO_ERR_ARG, "rows_per_tile out of range");
+  if (tz < 0 || tz > 4096) return fail(c, FOMO_ERR_ARG, "rows_per_tile out of range (0 = auto)");
   if (gsteps < 1 || gsteps > 256) return fail(c, FOMO_ERR_ARG, "steps_per_graph out of range");
   c->cpt = cpt;
   c->tz = tz;
@@ -449,13 +451,14 @@ static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const 
   sd.tz = tz;
   sd.tiles_x = (d.nx + 64 * cpt - 1) / (64 * cpt);
   sd.tiles_z = (d.nz + tz - 1) / tz;
+  // HABC pieces (see k_habc): side-edge rows [fs ? n : n+2, nz-n-2), bottom/top
+  // edge columns [n+2, nx-n-2); the rest belongs to the corner pieces.
   const int n = d.nbc;
-  sd.edge_r0 = sd.fs ? n : 1;
-  sd.edge_rows = (d.nz - n - 1) - sd.edge_r0;
-  sd.edge_cols = d.nx - 2 * n - 2;
-  sd.n_l = (sd.edge_rows + LROWS - 1) / LROWS;
-  sd.n_r = (sd.edge_rows + 63) / 64;
-  sd.n_b = (sd.edge_cols + 63) / 64;
+  const int side_rows = (d.nz - n - 2) - (sd.fs ? n : n + 2);
+  const int edge_cols = (d.nx - n - 2) - (n + 2);
+  sd.n_l = (side_rows + 63) / 64;
+  sd.n_r = sd.n_l;
+  sd.n_b = (edge_cols + 63) / 64;
   sd.n_t = sd.fs ? 0 : sd.n_b;
   sd.dtx = d.dtx;
   sd.dtz = d.dtz;
@@ -493,20 +496,71 @@ static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const 
 }
 
 static int part_items(const ShotDev& sd, int part) {
-  const int edges = sd.n_l + sd.n_r + sd.n_b;
+  const int per_field = sd.n_l + sd.n_r + sd.n_b + sd.n_t + (sd.fs ? 2 : 4);
   switch (part) {
     case PART_STENCIL: return sd.tiles_x * sd.tiles_z;
-    case PART_HABC1_V: return 2 * edges;
-    case PART_HABC1_S: return 3 * edges;
-    case PART_HABC2_V: return 2 * (sd.n_t + 2);
-    case PART_HABC2_S: return 3 * (sd.n_t + 2);
-    case PART_HABC3_V: return sd.fs ? 0 : 2 * 2;
-    case PART_HABC3_S: return sd.fs ? 0 : 3 * 2;
+    case PART_HABC_V: return 2 * per_field;
+    case PART_HABC_S: return 3 * per_field;
     case PART_FSREC: return (6 * sd.nx + sd.n_rec + 63) / 64;
     case PART_BACKUP: return (int)(((int64_t)sd.nx * sd.nz + 63) / 64);
     case PART_SHOT: return 1;
   }
   return 0;
+}
+
+constexpr int WPB = 4;  // wavefronts per block of the stencil kernels
+
+// Resident wavefronts of the stencil kernels for (M, cpt) on the whole device.
+template <int M, int CPT>
+static int stencil_slots_mc(int cus) {
+  int bv = 0, bs = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_velocity<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
+    bv = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, k_stress<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
+    bs = 1;
+  return std::max(1, std::min(bv, bs)) * WPB * std::max(cus, 1);
+}
+template <int M>
+static int stencil_slots_m(int cpt, int cus) {
+  return cpt == 1 ? stencil_slots_mc<M, 1>(cus) : cpt == 2 ? stencil_slots_mc<M, 2>(cus)
+                                                            : stencil_slots_mc<M, 4>(cus);
+}
+static int stencil_slots(int M, int cpt, int cus) {
+  switch (M) {
+    case 1: return stencil_slots_m<1>(cpt, cus);
+    case 2: return stencil_slots_m<2>(cpt, cus);
+    case 3: return stencil_slots_m<3>(cpt, cus);
+    case 4: return stencil_slots_m<4>(cpt, cus);
+    case 5: return stencil_slots_m<5>(cpt, cus);
+    case 6: return stencil_slots_m<6>(cpt, cus);
+    case 7: return stencil_slots_m<7>(cpt, cus);
+    default: return stencil_slots_m<8>(cpt, cus);
+  }
+}
+
+// Rows per stencil tile so that the tiles of all shots fill the resident
+// wavefront slots in whole rounds (no tail round), within [8, 64] rows when
+// the grid allows.
+static int auto_tz(const std::vector<int>& tiles_x, const std::vector<int>& nz, int slots) {
+  auto items = [&](int tz) {
+    int64_t t = 0;
+    for (size_t i = 0; i < nz.size(); ++i) t += (int64_t)tiles_x[i] * ((nz[i] + tz - 1) / tz);
+    return t;
+  };
+  int best = 32;
+  double best_eff = -1;
+  for (int tz = 8; tz <= 64; ++tz) {
+    const int64_t it = items(tz);
+    const int64_t rounds = (it + slots - 1) / slots;
+    // busy fraction of the last round, discounted by the z-halo re-read cost
+    const double fill = (double)it / (double)(rounds * slots);
+    const double eff = fill * (double)tz / (double)(tz + 4);
+    if (eff > best_eff + 1e-9) {
+      best_eff = eff;
+      best = tz;
+    }
+  }
+  return best;
 }
 
 static void destroy_batch(fomo_batch* b) {
@@ -527,6 +581,7 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
   b->n = n;
   b->cpt = c->cpt;
   b->tz = c->tz;
+  b->tz_req = c->tz;
   b->graph_steps = c->graph_steps;
   b->M = ms[0]->M;
   for (int i = 0; i < n; ++i) {
@@ -534,9 +589,16 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
     if (ms[i]->ctx != c || ws[i]->m != ms[i] || ss[i]->m != ms[i])
       return fail(c, FOMO_ERR_STATE, "batch member belongs to another model or context");
     if (ms[i]->M != b->M) return fail(c, FOMO_ERR_UNSUPPORTED, "batch mixes FD orders");
-    b->hs.push_back(make_shotdev(ms[i], ws[i], ss[i], b->cpt, b->tz));
-    if (!ms[i]->d.free_surface) b->any_nofs = true;
   }
+  if (b->tz == 0) {
+    std::vector<int> tx, nzs;
+    for (int i = 0; i < n; ++i) {
+      tx.push_back((ms[i]->d.nx + 64 * b->cpt - 1) / (64 * b->cpt));
+      nzs.push_back(ms[i]->d.nz);
+    }
+    b->tz = auto_tz(tx, nzs, stencil_slots(b->M, b->cpt, c->cus));
+  }
+  for (int i = 0; i < n; ++i) b->hs.push_back(make_shotdev(ms[i], ws[i], ss[i], b->cpt, b->tz));
   b->hprefix.assign((size_t)PART_COUNT * (n + 1), 0);
   for (int p = 0; p < PART_COUNT; ++p) {
     int acc = 0;
@@ -585,7 +647,7 @@ static int single_batch(fomo_model* m, fomo_wavefield* w, fomo_shot* s, fomo_bat
   }
   const int q = (s == w->placeholder) ? 0 : 1;
   fomo_batch* b = w->single[q];
-  if (b && w->single_id[q] == s->id && b->cpt == c->cpt && b->tz == c->tz &&
+  if (b && w->single_id[q] == s->id && b->cpt == c->cpt && b->tz_req == c->tz &&
       b->graph_steps == c->graph_steps) {
     *out = b;
     return FOMO_OK;
@@ -601,7 +663,6 @@ static int single_batch(fomo_model* m, fomo_wavefield* w, fomo_shot* s, fomo_bat
 }
 
 // ------------------------------------------------------------------ launches
-constexpr int WPB = 4;  // wavefronts per block
 
 static int blocks_for(const fomo_batch* b, int part) {
   const int total = b->hprefix[(size_t)part * (b->n + 1) + b->n];
@@ -648,18 +709,10 @@ static void launch_stencil(const fomo_batch* b, bool stress, int flags, int k_of
 }
 
 static void launch_habc(const fomo_batch* b, bool stress, int k_off) {
-  const int s = stress ? 1 : 0;
-  dim3 block(64 * WPB);
-  const int p1 = stress ? PART_HABC1_S : PART_HABC1_V;
-  const int p2 = stress ? PART_HABC2_S : PART_HABC2_V;
-  const int p3 = stress ? PART_HABC3_S : PART_HABC3_V;
-  hipLaunchKernelGGL(k_habc1<WPB>, dim3(blocks_for(b, p1)), block, 0, b->ctx->stream,
-                     args_for(b, p1, k_off), s);
-  hipLaunchKernelGGL(k_habc2<WPB>, dim3(blocks_for(b, p2)), block, 0, b->ctx->stream,
-                     args_for(b, p2, k_off), s);
-  if (b->any_nofs)
-    hipLaunchKernelGGL(k_habc3<WPB>, dim3(blocks_for(b, p3)), block, 0, b->ctx->stream,
-                       args_for(b, p3, k_off), s);
+  const int part = stress ? PART_HABC_S : PART_HABC_V;
+  const int items = b->hprefix[(size_t)part * (b->n + 1) + b->n];
+  hipLaunchKernelGGL(k_habc, dim3(std::max(1, items)), dim3(64), 0, b->ctx->stream,
+                     args_for(b, part, k_off), stress ? 1 : 0);
 }
 
 static void launch_fsrec(const fomo_batch* b, int mode, int k_off) {

@@ -31,6 +31,17 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+// A pointer read from the shot table, made wave-uniform (kept in SGPRs).
+template <class T>
+__device__ __forceinline__ T* uniform_p(T* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (T*)(((uint64_t)hi << 32) | lo);
+}
 
 // Shot owning item: largest s with prefix[s] <= item (prefix[n] == total).
 __device__ __forceinline__ int find_shot(const int* prefix, int n, int item) {
@@ -50,17 +61,27 @@ struct V {
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
+// Field pointers come from the shot table as generic pointers; accessing
+// them through the global address space keeps the loads off the LDS counter
+// (flat loads would make every LDS wait drain the prefetched rows too).
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) f2v gf2v;
+typedef __attribute__((address_space(1))) f4v gf4v;
+
+__device__ __forceinline__ float ldg(const float* p) { return *(const gfloat*)p; }
+__device__ __forceinline__ void stg(float* p, float v) { *(gfloat*)p = v; }
+
 template <int N>
 __device__ __forceinline__ V<N> ldv(const float* p);
 template <>
 __device__ __forceinline__ V<1> ldv<1>(const float* p) {
   V<1> r;
-  r.v[0] = *p;
+  r.v[0] = ldg(p);
   return r;
 }
 template <>
 __device__ __forceinline__ V<2> ldv<2>(const float* p) {
-  f2v t = *(const f2v*)p;
+  f2v t = *(const gf2v*)p;
   V<2> r;
   r.v[0] = t.x;
   r.v[1] = t.y;
@@ -68,7 +89,7 @@ __device__ __forceinline__ V<2> ldv<2>(const float* p) {
 }
 template <>
 __device__ __forceinline__ V<4> ldv<4>(const float* p) {
-  f4v t = *(const f4v*)p;
+  f4v t = *(const gf4v*)p;
   V<4> r;
   r.v[0] = t.x;
   r.v[1] = t.y;
@@ -80,14 +101,14 @@ template <int N>
 __device__ __forceinline__ void stv(float* p, const V<N>& a);
 template <>
 __device__ __forceinline__ void stv<1>(float* p, const V<1>& a) {
-  *p = a.v[0];
+  stg(p, a.v[0]);
 }
 template <>
 __device__ __forceinline__ void stv<2>(float* p, const V<2>& a) {
   f2v t;
   t.x = a.v[0];
   t.y = a.v[1];
-  *(f2v*)p = t;
+  *(gf2v*)p = t;
 }
 template <>
 __device__ __forceinline__ void stv<4>(float* p, const V<4>& a) {
@@ -96,7 +117,7 @@ __device__ __forceinline__ void stv<4>(float* p, const V<4>& a) {
   t.y = a.v[1];
   t.z = a.v[2];
   t.w = a.v[3];
-  *(f4v*)p = t;
+  *(gf4v*)p = t;
 }
 
 __device__ __forceinline__ const float* row(const float* base, int j, int pitch) {
@@ -107,11 +128,13 @@ __device__ __forceinline__ float* row(float* base, int j, int pitch) {
 }
 
 // HABC strip membership of copy_boundary_strip! (kernels/boundary.jl:20-38), 0-based.
+__device__ __forceinline__ bool in_strip(int nx, int nz, int n, int fs, int i, int j) {
+  const int jtop = fs ? n : 0;
+  return j >= jtop && j < nz && i < nx &&
+         (i <= n + 1 || i >= nx - n - 2 || j <= n + 1 || j >= nz - n - 2);
+}
 __device__ __forceinline__ bool in_strip(const ShotDev& sd, int i, int j) {
-  const int n = sd.nbc;
-  const int jtop = sd.fs ? n : 0;
-  return j >= jtop && j < sd.nz && i < sd.nx &&
-         (i <= n + 1 || i >= sd.nx - n - 2 || j <= n + 1 || j >= sd.nz - n - 2);
+  return in_strip(sd.nx, sd.nz, sd.nbc, sd.fs, i, j);
 }
 
 // Step number (1-based) this launch works on for shot sd, or 0 if the shot is idle.
@@ -151,11 +174,15 @@ __global__ __launch_bounds__(64 * WPB) void k_velocity(BatchArgs ba, int flags) 
     const ShotDev& sd = ba.shots[s];
     const int k = shot_step(sd, ba.k_off);
     if (k == 0) continue;
+    // Everything the row loop needs, read once from the shot table into
+    // wave-uniform registers (the loop's stores could alias the table).
     const int t = item - ba.prefix[s];
-    const int nx = sd.nx, nz = sd.nz, P = sd.pitch;
-    const int i0 = (t % sd.tiles_x) * TW;
-    const int j0 = (t / sd.tiles_x) * sd.tz;
-    const int j1 = min(j0 + sd.tz, nz);
+    const int nx = uniform(sd.nx), nz = uniform(sd.nz), P = uniform(sd.pitch);
+    const int nbc = uniform(sd.nbc), fs = uniform(sd.fs);
+    const int tiles_x = uniform(sd.tiles_x), tz = uniform(sd.tz);
+    const int i0 = (t % tiles_x) * TW;
+    const int j0 = (t / tiles_x) * tz;
+    const int j1 = min(j0 + tz, nz);
     const int ib = i0 + lane * CPT;
     int hx = -1;
     if (lane < M) hx = lane;
@@ -163,20 +190,30 @@ __global__ __launch_bounds__(64 * WPB) void k_velocity(BatchArgs ba, int flags) 
     const bool has_halo = hx >= 0;
     const int hcol = i0 - M + hx;
 
-    float* vx = sd.f[F_VX];
-    float* vz = sd.f[F_VZ];
-    const float* txx = sd.f[F_TXX];
-    const float* tzz = sd.f[F_TZZ];
-    const float* txz = sd.f[F_TXZ];
-    const bool src_on = (flags & FLAG_INJECT) && k <= sd.nw;
-    const float wav = src_on ? sd.wavelet[k - 1] : 0.f;
-    const bool src_p = src_on && sd.stype == SRC_PRESSURE;
-    const bool src_vz = src_on && sd.stype == SRC_FORCE_Z;
-    const bool src_vx = src_on && sd.stype == SRC_FORCE_X;
-    const int si = sd.si, sj = sd.sj;
+    float* vx = uniform_p(sd.f[F_VX]);
+    float* vz = uniform_p(sd.f[F_VZ]);
+    const float* txx = uniform_p(sd.f[F_TXX]);
+    const float* tzz = uniform_p(sd.f[F_TZZ]);
+    const float* txz = uniform_p(sd.f[F_TXZ]);
+    float* ovx = uniform_p(sd.fo[F_VX]);
+    float* ovz = uniform_p(sd.fo[F_VZ]);
+    const float* b1p = uniform_p(sd.bvx_x);
+    const float* b2p = uniform_p(sd.bvx_z);
+    const float* b3p = uniform_p(sd.bvz_x);
+    const float* b4p = uniform_p(sd.bvz_z);
+    const bool bsame = (b1p == b2p) && (b3p == b4p);
+    const int nw = uniform(sd.nw), stype = uniform(sd.stype);
+    const bool src_on = (flags & FLAG_INJECT) && k <= nw;
+    const float wav = src_on ? uniform_f(uniform_p(sd.wavelet)[k - 1]) : 0.f;
+    const bool src_p = src_on && stype == SRC_PRESSURE;
+    const bool src_vz = src_on && stype == SRC_FORCE_Z;
+    const bool src_vx = src_on && stype == SRC_FORCE_X;
+    const int si = uniform(sd.si), sj = uniform(sd.sj);
+    const float src_scale = uniform_f(sd.src_scale);
     float a[M];
 #pragma unroll
-    for (int l = 0; l < M; ++l) a[l] = sd.a[l];
+    for (int l = 0; l < M; ++l) a[l] = uniform_f(sd.a[l]);
+    // ---- end of shot-table reads
 
     float wtxz[CPT][2 * M];  // txz rows j-M .. j+M-1
     float wtzz[CPT][2 * M];  // tzz rows j-M+1 .. j+M
@@ -191,29 +228,35 @@ __global__ __launch_bounds__(64 * WPB) void k_velocity(BatchArgs ba, int flags) 
       }
     }
     V<CPT> c_txx = ldv<CPT>(row(txx, j0, P) + ib);
-    float c_htxx = has_halo ? row(txx, j0, P)[hcol] : 0.f;
-    float c_htxz = has_halo ? row(txz, j0, P)[hcol] : 0.f;
+    float c_htxx = has_halo ? ldg(row(txx, j0, P) + hcol) : 0.f;
+    float c_htxz = has_halo ? ldg(row(txz, j0, P) + hcol) : 0.f;
     V<CPT> c_vx = ldv<CPT>(row(vx, j0, P) + ib);
     V<CPT> c_vz = ldv<CPT>(row(vz, j0, P) + ib);
-    V<CPT> c_b1 = ldv<CPT>(row(sd.bvx_x, j0, P) + ib);
-    V<CPT> c_b2 = ldv<CPT>(row(sd.bvx_z, j0, P) + ib);
-    V<CPT> c_b3 = ldv<CPT>(row(sd.bvz_x, j0, P) + ib);
-    V<CPT> c_b4 = ldv<CPT>(row(sd.bvz_z, j0, P) + ib);
+    V<CPT> c_b1 = ldv<CPT>(row(b1p, j0, P) + ib);
+    V<CPT> c_b3 = ldv<CPT>(row(b3p, j0, P) + ib);
+    V<CPT> c_b2 = c_b1, c_b4 = c_b3;
+    if (!bsame) {
+      c_b2 = ldv<CPT>(row(b2p, j0, P) + ib);
+      c_b4 = ldv<CPT>(row(b4p, j0, P) + ib);
+    }
 
     for (int j = j0; j < j1; ++j) {
       // Prefetch row j+1 (and the window tops) while row j is computed.
       const int jn = j + 1;
       V<CPT> n_txx = ldv<CPT>(row(txx, jn, P) + ib);
-      float n_htxx = has_halo ? row(txx, jn, P)[hcol] : 0.f;
-      float n_htxz = has_halo ? row(txz, jn, P)[hcol] : 0.f;
+      float n_htxx = has_halo ? ldg(row(txx, jn, P) + hcol) : 0.f;
+      float n_htxz = has_halo ? ldg(row(txz, jn, P) + hcol) : 0.f;
       V<CPT> n_txz = ldv<CPT>(row(txz, j + M, P) + ib);
       V<CPT> n_tzz = ldv<CPT>(row(tzz, j + M + 1, P) + ib);
       V<CPT> n_vx = ldv<CPT>(row(vx, jn, P) + ib);
       V<CPT> n_vz = ldv<CPT>(row(vz, jn, P) + ib);
-      V<CPT> n_b1 = ldv<CPT>(row(sd.bvx_x, jn, P) + ib);
-      V<CPT> n_b2 = ldv<CPT>(row(sd.bvx_z, jn, P) + ib);
-      V<CPT> n_b3 = ldv<CPT>(row(sd.bvz_x, jn, P) + ib);
-      V<CPT> n_b4 = ldv<CPT>(row(sd.bvz_z, jn, P) + ib);
+      V<CPT> n_b1 = ldv<CPT>(row(b1p, jn, P) + ib);
+      V<CPT> n_b3 = ldv<CPT>(row(b3p, jn, P) + ib);
+      V<CPT> n_b2 = n_b1, n_b4 = n_b3;
+      if (!bsame) {
+        n_b2 = ldv<CPT>(row(b2p, jn, P) + ib);
+        n_b4 = ldv<CPT>(row(b4p, jn, P) + ib);
+      }
 
       const bool src_row = src_p && j == sj;
 #pragma unroll
@@ -243,8 +286,8 @@ __global__ __launch_bounds__(64 * WPB) void k_velocity(BatchArgs ba, int flags) 
         const bool inside = (i >= M) & (i < nx - M) & (j >= M) & (j < nz - M);
         float vxs = c_vx.v[c], vzs = c_vz.v[c];
         if (i == si && j == sj) {
-          if (src_vz) vzs = vzs + wav * sd.src_scale;
-          if (src_vx) vxs = vxs + wav * sd.src_scale;
+          if (src_vz) vzs = vzs + wav * src_scale;
+          if (src_vx) vxs = vxs + wav * src_scale;
         }
         const float ix = fmaf_(c_b2.v[c], d2, c_b1.v[c] * d1);
         const float iz = fmaf_(c_b4.v[c], d4, c_b3.v[c] * d3);
@@ -256,9 +299,9 @@ __global__ __launch_bounds__(64 * WPB) void k_velocity(BatchArgs ba, int flags) 
       if (flags & FLAG_BACKUP) {
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
-          if (in_strip(sd, ib + c, j)) {
-            row(sd.fo[F_VX], j, P)[ib + c] = c_vx.v[c];
-            row(sd.fo[F_VZ], j, P)[ib + c] = c_vz.v[c];
+          if (in_strip(nx, nz, nbc, fs, ib + c, j)) {
+            stg(row(ovx, j, P) + ib + c, c_vx.v[c]);
+            stg(row(ovz, j, P) + ib + c, c_vz.v[c]);
           }
         }
       }
@@ -312,10 +355,12 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
     const int k = shot_step(sd, ba.k_off);
     if (k == 0) continue;
     const int t = item - ba.prefix[s];
-    const int nx = sd.nx, nz = sd.nz, P = sd.pitch;
-    const int i0 = (t % sd.tiles_x) * TW;
-    const int j0 = (t / sd.tiles_x) * sd.tz;
-    const int j1 = min(j0 + sd.tz, nz);
+    const int nx = uniform(sd.nx), nz = uniform(sd.nz), P = uniform(sd.pitch);
+    const int nbc = uniform(sd.nbc), fs = uniform(sd.fs);
+    const int tiles_x = uniform(sd.tiles_x), tz = uniform(sd.tz);
+    const int i0 = (t % tiles_x) * TW;
+    const int j0 = (t / tiles_x) * tz;
+    const int j1 = min(j0 + tz, nz);
     const int ib = i0 + lane * CPT;
     int hx = -1;
     if (lane < M) hx = lane;
@@ -323,18 +368,26 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
     const bool has_halo = hx >= 0;
     const int hcol = i0 - M + hx;
 
-    const float* vx = sd.f[F_VX];
-    const float* vz = sd.f[F_VZ];
-    float* txx = sd.f[F_TXX];
-    float* tzz = sd.f[F_TZZ];
-    float* txz = sd.f[F_TXZ];
-    const bool src_on = (flags & FLAG_INJECT) && k <= sd.nw && sd.stype == SRC_PRESSURE;
-    const float wav = src_on ? sd.wavelet[k - 1] : 0.f;
-    const int si = sd.si, sj = sd.sj;
-    const float dtx = sd.dtx, dtz = sd.dtz;
+    const float* vx = uniform_p(sd.f[F_VX]);
+    const float* vz = uniform_p(sd.f[F_VZ]);
+    float* txx = uniform_p(sd.f[F_TXX]);
+    float* tzz = uniform_p(sd.f[F_TZZ]);
+    float* txz = uniform_p(sd.f[F_TXZ]);
+    float* otxx = uniform_p(sd.fo[F_TXX]);
+    float* otzz = uniform_p(sd.fo[F_TZZ]);
+    float* otxz = uniform_p(sd.fo[F_TXZ]);
+    const float* lamp = uniform_p(sd.lam);
+    const float* l2mp = uniform_p(sd.l2m);
+    const float* mup = uniform_p(sd.mu_txz);
+    const int nw = uniform(sd.nw), stype = uniform(sd.stype);
+    const bool src_on = (flags & FLAG_INJECT) && k <= nw && stype == SRC_PRESSURE;
+    const float wav = src_on ? uniform_f(uniform_p(sd.wavelet)[k - 1]) : 0.f;
+    const int si = uniform(sd.si), sj = uniform(sd.sj);
+    const float dtx = uniform_f(sd.dtx), dtz = uniform_f(sd.dtz);
     float a[M];
 #pragma unroll
-    for (int l = 0; l < M; ++l) a[l] = sd.a[l];
+    for (int l = 0; l < M; ++l) a[l] = uniform_f(sd.a[l]);
+    // ---- end of shot-table reads
 
     float wvx[CPT][2 * M];  // vx rows j-M+1 .. j+M
     float wvz[CPT][2 * M];  // vz rows j-M .. j+M-1
@@ -348,24 +401,24 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
         wvz[c][q] = z.v[c];
       }
     }
-    float c_hvx = has_halo ? row(vx, j0, P)[hcol] : 0.f;
-    float c_hvz = has_halo ? row(vz, j0, P)[hcol] : 0.f;
-    V<CPT> c_lam = ldv<CPT>(row(sd.lam, j0, P) + ib);
-    V<CPT> c_l2m = ldv<CPT>(row(sd.l2m, j0, P) + ib);
-    V<CPT> c_mu = ldv<CPT>(row(sd.mu_txz, j0, P) + ib);
+    float c_hvx = has_halo ? ldg(row(vx, j0, P) + hcol) : 0.f;
+    float c_hvz = has_halo ? ldg(row(vz, j0, P) + hcol) : 0.f;
+    V<CPT> c_lam = ldv<CPT>(row(lamp, j0, P) + ib);
+    V<CPT> c_l2m = ldv<CPT>(row(l2mp, j0, P) + ib);
+    V<CPT> c_mu = ldv<CPT>(row(mup, j0, P) + ib);
     V<CPT> c_txx = ldv<CPT>(row(txx, j0, P) + ib);
     V<CPT> c_tzz = ldv<CPT>(row(tzz, j0, P) + ib);
     V<CPT> c_txz = ldv<CPT>(row(txz, j0, P) + ib);
 
     for (int j = j0; j < j1; ++j) {
       const int jn = j + 1;
-      float n_hvx = has_halo ? row(vx, jn, P)[hcol] : 0.f;
-      float n_hvz = has_halo ? row(vz, jn, P)[hcol] : 0.f;
+      float n_hvx = has_halo ? ldg(row(vx, jn, P) + hcol) : 0.f;
+      float n_hvz = has_halo ? ldg(row(vz, jn, P) + hcol) : 0.f;
       V<CPT> n_vx = ldv<CPT>(row(vx, j + M + 1, P) + ib);
       V<CPT> n_vz = ldv<CPT>(row(vz, j + M, P) + ib);
-      V<CPT> n_lam = ldv<CPT>(row(sd.lam, jn, P) + ib);
-      V<CPT> n_l2m = ldv<CPT>(row(sd.l2m, jn, P) + ib);
-      V<CPT> n_mu = ldv<CPT>(row(sd.mu_txz, jn, P) + ib);
+      V<CPT> n_lam = ldv<CPT>(row(lamp, jn, P) + ib);
+      V<CPT> n_l2m = ldv<CPT>(row(l2mp, jn, P) + ib);
+      V<CPT> n_mu = ldv<CPT>(row(mup, jn, P) + ib);
       V<CPT> n_txx = ldv<CPT>(row(txx, jn, P) + ib);
       V<CPT> n_tzz = ldv<CPT>(row(tzz, jn, P) + ib);
       V<CPT> n_txz = ldv<CPT>(row(txz, jn, P) + ib);
@@ -416,10 +469,10 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
       if (flags & FLAG_BACKUP) {
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
-          if (in_strip(sd, ib + c, j)) {
-            row(sd.fo[F_TXX], j, P)[ib + c] = c_txx.v[c];
-            row(sd.fo[F_TZZ], j, P)[ib + c] = c_tzz.v[c];
-            row(sd.fo[F_TXZ], j, P)[ib + c] = c_txz.v[c];
+          if (in_strip(nx, nz, nbc, fs, ib + c, j)) {
+            stg(row(otxx, j, P) + ib + c, c_txx.v[c]);
+            stg(row(otzz, j, P) + ib + c, c_tzz.v[c]);
+            stg(row(otxz, j, P) + ib + c, c_txz.v[c]);
           }
         }
       }
@@ -453,8 +506,20 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
 // right-top]; j outer, i inner; every read sees the current array.  That
 // order makes the left and top edges Jacobi sweeps over pre-HABC values, the
 // right edge a recurrence along +x per row, the bottom edge a recurrence
-// along +z per column, the bottom-right corner a 2D recurrence (run as
-// anti-diagonal wavefronts), and so on; the pieces below follow it exactly.
+// along +z per column, the bottom-right corner a 2D recurrence, and so on.
+//
+// One launch per pass covers every field and every strip piece.  A piece owns
+// a rectangle of cells and reads only cells that no other piece writes, so
+// the pieces are independent:
+//   L, R   64-row chunks of the side edges, excluding the rows next to the
+//          corners;
+//   B, T   64-column chunks of the bottom / top edges, excluding the columns
+//          next to the corners;
+//   BL, BR the bottom corner plus the edge row above it and the edge column
+//          beside it (the values the corner recurrence consumes);
+//   TL, TR likewise at the top (no free surface only).
+// Each piece stages its rectangle (field and backup) in LDS, runs the
+// program-order recurrences there and writes the rectangle back.
 // ---------------------------------------------------------------------------
 struct Hq {
   float qx, qz, qt_x, qt_z, qxt;
@@ -494,236 +559,241 @@ __device__ __forceinline__ HabcField habc_field(const ShotDev& sd, int stress, i
   return h;
 }
 
-constexpr int LROWS = 8;  // rows per left-edge piece
+constexpr int HS = 65;  // LDS row stride: odd, so row and column walks are bank-conflict free
 
-template <int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_habc1(BatchArgs ba, int stress) {
-  const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
+// rows [r0, r0+nr) x cols [c0, c0+nc) of g -> s[r*HS + c]   (nc <= 64)
+__device__ __forceinline__ void lds_in(float* s, const float* g, int P, int r0, int nr, int c0,
+                                       int nc, int lane) {
+#pragma unroll 4
+  for (int r = 0; r < nr; ++r)
+    if (lane < nc) s[r * HS + lane] = ldg(g + (ptrdiff_t)(r0 + r) * P + c0 + lane);
+}
+// s[(sr+r)*HS + sc + c] -> rows [r0, r0+nr) x cols [c0, c0+nc) of g   (nc <= 64)
+__device__ __forceinline__ void lds_out(const float* s, int sr, int sc, float* g, int P, int r0,
+                                        int nr, int c0, int nc, int lane) {
+#pragma unroll 4
+  for (int r = 0; r < nr; ++r)
+    if (lane < nc) stg(g + (ptrdiff_t)(r0 + r) * P + c0 + lane, s[(sr + r) * HS + sc + lane]);
+}
+
+// Items per field of the HABC partition and the first side-edge row.
+__device__ __forceinline__ int habc_side_r0(const ShotDev& sd) { return sd.fs ? sd.nbc : sd.nbc + 2; }
+
+__global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
+  __shared__ float s_f[64 * HS];
+  __shared__ float s_o[64 * HS];
+  const int lane = threadIdx.x;
   const int nsh = *ba.n_shots_p;
   const int total = ba.prefix[nsh];
-  for (int item = uniform(blockIdx.x * WPB + wid); item < total;
-       item = uniform(item + gridDim.x * WPB)) {
+  for (int item = uniform(blockIdx.x); item < total; item = uniform(item + gridDim.x)) {
     const int s = uniform(find_shot(ba.prefix, nsh, item));
     const ShotDev& sd = ba.shots[s];
     if (shot_step(sd, ba.k_off) == 0) continue;
     const int t = item - ba.prefix[s];
-    const int per = sd.n_l + sd.n_r + sd.n_b;
+    const int ncorner = sd.fs ? 2 : 4;
+    const int per = sd.n_l + sd.n_r + sd.n_b + sd.n_t + ncorner;
     const int fi = t / per;
     int p = t % per;
     const HabcField h = habc_field(sd, stress, fi);
     const Hq q{sd.qx, sd.qz, sd.qt_x, sd.qt_z, sd.qxt};
     const int n = sd.nbc, nx = sd.nx, nz = sd.nz, P = sd.pitch;
-    const int r0 = sd.edge_r0, r1 = sd.edge_r0 + sd.edge_rows;  // rows [r0, r1)
-    if (p < sd.n_l) {
-      // Left edge: i in [1, n], Jacobi on pre-HABC values of the same row.
-      const bool act = lane < n;
-      const int i = 1 + lane;
-      const int ja = r0 + p * LROWS, jb = min(ja + LROWS, r1);
-      for (int j = ja; j < jb; ++j) {
-        float* fr = row(h.f, j, P);
-        const float* gr = row(h.fo, j, P);
-        const float fv = act ? fr[i] : 0.f;
-        const float ov = act ? gr[i] : 0.f;
-        float fn = __shfl_down(fv, 1);
-        float on = __shfl_down(ov, 1);
-        if (lane == n - 1) {
-          fn = fr[n + 1];
-          on = gr[n + 1];
-        }
-        if (act) fr[i] = q.edge(habc_w(sd, h.ramp, i, j), fv, q.sumx(fn, ov, on));
-      }
-      continue;
-    }
-    p -= sd.n_l;
-    if (p < sd.n_r) {
-      // Right edge: one row per lane, recurrence along +x.
-      const int j = r0 + p * 64 + lane;
-      if (j >= r1) continue;
-      float* fr = row(h.f, j, P);
-      const float* gr = row(h.fo, j, P);
-      float prev = fr[nx - n - 2];
-      float op = gr[nx - n - 2];
-      for (int i = nx - n - 1; i <= nx - 2; ++i) {
-        const float fv = fr[i], ov = gr[i];
-        const float out = q.edge(habc_w(sd, h.ramp, i, j), fv, q.sumx(prev, ov, op));
-        fr[i] = out;
-        prev = out;
-        op = ov;
-      }
-      continue;
-    }
-    p -= sd.n_r;
-    {
-      // Bottom edge: one column per lane, recurrence along +z.
-      const int i = n + 1 + p * 64 + lane;
-      if (i > nx - n - 2) continue;
-      float prev = row(h.f, nz - n - 2, P)[i];
-      float op = row(h.fo, nz - n - 2, P)[i];
-      for (int j = nz - n - 1; j <= nz - 2; ++j) {
-        float* fp = row(h.f, j, P) + i;
-        const float fv = *fp, ov = row(h.fo, j, P)[i];
-        const float out = q.edge(habc_w(sd, h.ramp, i, j), fv, q.sumz(prev, ov, op));
-        *fp = out;
-        prev = out;
-        op = ov;
-      }
-    }
-  }
-}
+    const int ra = habc_side_r0(sd), rb = nz - n - 2;  // side-edge rows [ra, rb)
+    const int ca = n + 2, cb = nx - n - 2;             // bottom/top-edge columns [ca, cb)
+    __syncthreads();  // LDS of the previous item is no longer read
 
-// Phase 2: top edge (no free surface), left-bottom and right-bottom corners.
-template <int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_habc2(BatchArgs ba, int stress) {
-  const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
-  const int nsh = *ba.n_shots_p;
-  const int total = ba.prefix[nsh];
-  for (int item = uniform(blockIdx.x * WPB + wid); item < total;
-       item = uniform(item + gridDim.x * WPB)) {
-    const int s = uniform(find_shot(ba.prefix, nsh, item));
-    const ShotDev& sd = ba.shots[s];
-    if (shot_step(sd, ba.k_off) == 0) continue;
-    const int t = item - ba.prefix[s];
-    const int per = sd.n_t + 2;
-    const int fi = t / per;
-    const int p = t % per;
-    const HabcField h = habc_field(sd, stress, fi);
-    const Hq q{sd.qx, sd.qz, sd.qt_x, sd.qt_z, sd.qxt};
-    const int n = sd.nbc, nx = sd.nx, nz = sd.nz, P = sd.pitch;
-    if (p < sd.n_t) {
-      // Top edge: one column per lane, Jacobi in z over rows 1..n.
-      const int i = n + 1 + p * 64 + lane;
-      if (i > nx - n - 2) continue;
-      float fv = row(h.f, 1, P)[i], ov = row(h.fo, 1, P)[i];
-      for (int j = 1; j <= n; ++j) {
-        const float fn = row(h.f, j + 1, P)[i], on = row(h.fo, j + 1, P)[i];
-        row(h.f, j, P)[i] = q.edge(habc_w(sd, h.ramp, i, j), fv, q.sumz(fn, ov, on));
-        fv = fn;
-        ov = on;
-      }
-    } else if (p == sd.n_t) {
-      // Left-bottom corner: one column per lane, recurrence along +z; the
-      // right neighbour is read before it is updated (same row, i ascending).
-      const bool act = lane < n;
-      const int i = 1 + lane;
-      float up = act ? row(h.f, nz - n - 2, P)[i] : 0.f;
-      float ou = act ? row(h.fo, nz - n - 2, P)[i] : 0.f;
-      for (int j = nz - n - 1; j <= nz - 2; ++j) {
-        float* fr = row(h.f, j, P);
-        const float* gr = row(h.fo, j, P);
-        const float fv = act ? fr[i] : 0.f;
-        const float ov = act ? gr[i] : 0.f;
-        float fnb = __shfl_down(fv, 1);
-        float onb = __shfl_down(ov, 1);
-        if (lane == n - 1) {
-          fnb = fr[n + 1];
-          onb = gr[n + 1];
+    if (p < sd.n_l + sd.n_r) {
+      const bool right = p >= sd.n_l;
+      const int j0 = ra + (right ? p - sd.n_l : p) * 64;
+      const int nr = min(64, rb - j0);
+      const int c0 = right ? nx - n - 2 : 1;  // staged columns [c0, c0+n+1)
+      lds_in(s_f, h.f, P, j0, nr, c0, n + 1, lane);
+      lds_in(s_o, h.fo, P, j0, nr, c0, n + 1, lane);
+      __syncthreads();
+      if (!right) {
+        // left edge, i = 1..n: Jacobi on the row's pre-HABC values
+        for (int r = 0; r < nr; ++r) {
+          if (lane < n) {
+            const float fv = s_f[r * HS + lane], fn = s_f[r * HS + lane + 1];
+            const float ov = s_o[r * HS + lane], on = s_o[r * HS + lane + 1];
+            const float out = q.edge(habc_w(sd, h.ramp, 1 + lane, j0 + r), fv, q.sumx(fn, ov, on));
+            s_f[r * HS + lane] = out;
+          }
         }
-        if (act) {
-          const float out =
-              q.corner(habc_w(sd, h.ramp, i, j), fv, q.sumx(fnb, ov, onb), q.sumz(up, ov, ou));
-          fr[i] = out;
-          up = out;
-          ou = ov;
-        }
-      }
-    } else {
-      // Right-bottom corner: 2D recurrence (+x and +z), anti-diagonal
-      // wavefronts; lane c owns column nx-n-1+c and hands its newest value
-      // to lane c+1 one step later.
-      const int c = lane;
-      const int i = nx - n - 1 + c;
-      const bool col_ok = c < n;
-      float up = col_ok ? row(h.f, nz - n - 2, P)[i] : 0.f;
-      float ou = col_ok ? row(h.fo, nz - n - 2, P)[i] : 0.f;
-      float last = 0.f;
-      for (int st = 0; st <= 2 * n - 2; ++st) {
-        const int r = st - c;
-        const bool act = col_ok && r >= 0 && r < n;
-        const int j = nz - n - 1 + r;
-        float left = __shfl_up(last, 1);
-        if (act) {
-          float* fr = row(h.f, j, P);
-          const float* gr = row(h.fo, j, P);
-          if (c == 0) left = fr[i - 1];
-          const float fv = fr[i], ov = gr[i], ol = gr[i - 1];
-          const float out =
-              q.corner(habc_w(sd, h.ramp, i, j), fv, q.sumx(left, ov, ol), q.sumz(up, ov, ou));
-          fr[i] = out;
-          last = out;
-          up = out;
-          ou = ov;
-        }
-      }
-    }
-  }
-}
-
-// Phase 3 (no free surface): left-top and right-top corners.
-template <int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_habc3(BatchArgs ba, int stress) {
-  const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
-  const int nsh = *ba.n_shots_p;
-  const int total = ba.prefix[nsh];
-  for (int item = uniform(blockIdx.x * WPB + wid); item < total;
-       item = uniform(item + gridDim.x * WPB)) {
-    const int s = uniform(find_shot(ba.prefix, nsh, item));
-    const ShotDev& sd = ba.shots[s];
-    if (shot_step(sd, ba.k_off) == 0) continue;
-    const int t = item - ba.prefix[s];
-    const int fi = t / 2;
-    const int p = t % 2;
-    const HabcField h = habc_field(sd, stress, fi);
-    const Hq q{sd.qx, sd.qz, sd.qt_x, sd.qt_z, sd.qxt};
-    const int n = sd.nbc, nx = sd.nx, P = sd.pitch;
-    const bool act = lane < n;
-    if (p == 0) {
-      // Left-top: Jacobi over the post-left-edge values, rows 1..n ascending.
-      const int i = 1 + lane;
-      float fv = act ? row(h.f, 1, P)[i] : 0.f;
-      float ov = act ? row(h.fo, 1, P)[i] : 0.f;
-      for (int j = 1; j <= n; ++j) {
-        float* fr = row(h.f, j, P);
-        const float* gr = row(h.fo, j, P);
-        const float fd = act ? row(h.f, j + 1, P)[i] : 0.f;
-        const float od = act ? row(h.fo, j + 1, P)[i] : 0.f;
-        float fnb = __shfl_down(fv, 1);
-        float onb = __shfl_down(ov, 1);
-        if (lane == n - 1) {
-          fnb = fr[n + 1];
-          onb = gr[n + 1];
-        }
-        if (act)
-          fr[i] = q.corner(habc_w(sd, h.ramp, i, j), fv, q.sumx(fnb, ov, onb), q.sumz(fd, ov, od));
-        fv = fd;
-        ov = od;
-      }
-    } else {
-      // Right-top: one row per lane, recurrence along +x; the row below is
-      // read before it is updated (rows ascending).
-      const int j = 1 + lane;
-      float prev = act ? row(h.f, j, P)[nx - n - 2] : 0.f;
-      float op = act ? row(h.fo, j, P)[nx - n - 2] : 0.f;
-      for (int i = nx - n - 1; i <= nx - 2; ++i) {
-        const float fv = act ? row(h.f, j, P)[i] : 0.f;
-        const float ov = act ? row(h.fo, j, P)[i] : 0.f;
-        float fd = __shfl_down(fv, 1);
-        float od = __shfl_down(ov, 1);
-        if (lane == n - 1) {
-          fd = row(h.f, n + 1, P)[i];
-          od = row(h.fo, n + 1, P)[i];
-        }
-        if (act) {
-          const float out =
-              q.corner(habc_w(sd, h.ramp, i, j), fv, q.sumx(prev, ov, op), q.sumz(fd, ov, od));
-          row(h.f, j, P)[i] = out;
+      } else if (lane < nr) {
+        // right edge, one row per lane, recurrence along +x from column nx-n-2
+        float* fr = s_f + lane * HS;
+        const float* orow = s_o + lane * HS;
+        float prev = fr[0], op = orow[0];
+        for (int c = 1; c <= n; ++c) {
+          const float fv = fr[c], ov = orow[c];
+          const float out = q.edge(habc_w(sd, h.ramp, c0 + c, j0 + lane), fv, q.sumx(prev, ov, op));
+          fr[c] = out;
           prev = out;
           op = ov;
         }
       }
+      __syncthreads();
+      // left: block columns 0..n-1 = columns 1..n; right: block columns 1..n
+      lds_out(s_f, 0, right ? 1 : 0, h.f, P, j0, nr, right ? c0 + 1 : 1, n, lane);
+      continue;
     }
+    p -= sd.n_l + sd.n_r;
+    if (p < sd.n_b + sd.n_t) {
+      const bool top = p >= sd.n_b;
+      const int i0 = ca + (top ? p - sd.n_b : p) * 64;
+      const int nc = min(64, cb - i0);
+      const int r0 = top ? 1 : nz - n - 2;  // staged rows [r0, r0+n+1)
+      lds_in(s_f, h.f, P, r0, n + 1, i0, nc, lane);
+      lds_in(s_o, h.fo, P, r0, n + 1, i0, nc, lane);
+      __syncthreads();
+      if (lane < nc) {
+        const int i = i0 + lane;
+        if (!top) {
+          // bottom edge: recurrence along +z from row nz-n-2
+          float prev = s_f[lane], op = s_o[lane];
+          for (int r = 1; r <= n; ++r) {
+            const float fv = s_f[r * HS + lane], ov = s_o[r * HS + lane];
+            const float out = q.edge(habc_w(sd, h.ramp, i, r0 + r), fv, q.sumz(prev, ov, op));
+            s_f[r * HS + lane] = out;
+            prev = out;
+            op = ov;
+          }
+        } else {
+          // top edge (no free surface), rows 1..n: Jacobi in z, rows ascending
+          for (int r = 0; r < n; ++r) {
+            const float fv = s_f[r * HS + lane], fn = s_f[(r + 1) * HS + lane];
+            const float ov = s_o[r * HS + lane], on = s_o[(r + 1) * HS + lane];
+            s_f[r * HS + lane] = q.edge(habc_w(sd, h.ramp, i, r0 + r), fv, q.sumz(fn, ov, on));
+          }
+        }
+      }
+      __syncthreads();
+      // bottom: block rows 1..n = rows nz-n-1..nz-2; top: block rows 0..n-1 = rows 1..n
+      lds_out(s_f, top ? 0 : 1, 0, h.f, P, top ? r0 : r0 + 1, n, i0, nc, lane);
+      continue;
+    }
+    p -= sd.n_b + sd.n_t;
+    // Corner pieces: an (n+1) x (n+1) block.
+    const bool bottom = p < 2;
+    const bool leftc = (p % 2) == 0;
+    const int r0 = bottom ? nz - n - 2 : 1;
+    const int c0 = leftc ? 1 : nx - n - 2;
+    lds_in(s_f, h.f, P, r0, n + 1, c0, n + 1, lane);
+    lds_in(s_o, h.fo, P, r0, n + 1, c0, n + 1, lane);
+    __syncthreads();
+    const bool act = lane <= n;
+    if (bottom && leftc) {
+      // BL: block row 0 = left-edge row nz-n-2, block column n = bottom-edge
+      // column n+1.  1) left edge on row 0 (Jacobi); 2) per row, the bottom
+      // edge at column n+1 then the left-bottom corner (right neighbour read
+      // before update; the column-(n+1) neighbour is the new bottom value).
+      if (lane < n) {
+        const float fv = s_f[lane], fn = s_f[lane + 1], ov = s_o[lane], on = s_o[lane + 1];
+        s_f[lane] = q.edge(habc_w(sd, h.ramp, c0 + lane, r0), fv, q.sumx(fn, ov, on));
+      }
+      __syncthreads();
+      float up = act ? s_f[lane] : 0.f, ou = act ? s_o[lane] : 0.f;
+      for (int r = 1; r <= n; ++r) {
+        const float fv = act ? s_f[r * HS + lane] : 0.f;
+        const float ov = act ? s_o[r * HS + lane] : 0.f;
+        const float w = act ? habc_w(sd, h.ramp, c0 + lane, r0 + r) : 0.f;
+        float x = fv;
+        if (lane == n) x = q.edge(w, fv, q.sumz(up, ov, ou));
+        const float fr = __shfl_down(x, 1);
+        const float orr = __shfl_down(ov, 1);
+        const float out = (lane == n) ? x : q.corner(w, fv, q.sumx(fr, ov, orr), q.sumz(up, ov, ou));
+        if (act) s_f[r * HS + lane] = out;
+        up = out;
+        ou = ov;
+      }
+    } else if (bottom) {
+      // BR: one wavefront over the block; block row 0 is the right-edge row
+      // nz-n-2 (recurrence along +x), block column 0 the bottom-edge column
+      // nx-n-2 (recurrence along +z), the rest the right-bottom corner.
+      const int L = lane;
+      float last = act ? s_f[L] : 0.f;  // lane 0 starts with cell (0,0), which is not updated
+      for (int d = 1; d <= 2 * n; ++d) {
+        const float left = __shfl_up(last, 1);
+        const int r = d - L;
+        if (act && r >= 0 && r <= n && !(L == 0 && r == 0)) {
+          const float fv = s_f[r * HS + L], ov = s_o[r * HS + L];
+          const float w = habc_w(sd, h.ramp, c0 + L, r0 + r);
+          float out;
+          if (r == 0) out = q.edge(w, fv, q.sumx(left, ov, s_o[L - 1]));
+          else if (L == 0) out = q.edge(w, fv, q.sumz(last, ov, s_o[(r - 1) * HS]));
+          else out = q.corner(w, fv, q.sumx(left, ov, s_o[r * HS + L - 1]),
+                              q.sumz(last, ov, s_o[(r - 1) * HS + L]));
+          s_f[r * HS + L] = out;
+          last = out;
+        }
+      }
+    } else if (leftc) {
+      // TL (no free surface): block rows 0..n = rows 1..n+1, columns 0..n =
+      // columns 1..n+1.  1) left edge on every block row (Jacobi); 2) top edge
+      // on block column n (Jacobi in z); 3) left-top corner (Jacobi over 1-2).
+      for (int r = 0; r <= n; ++r) {
+        if (lane < n) {
+          const float fv = s_f[r * HS + lane], fn = s_f[r * HS + lane + 1];
+          const float ov = s_o[r * HS + lane], on = s_o[r * HS + lane + 1];
+          s_f[r * HS + lane] = q.edge(habc_w(sd, h.ramp, c0 + lane, r0 + r), fv, q.sumx(fn, ov, on));
+        }
+      }
+      __syncthreads();
+      if (lane < n) {
+        const float fv = s_f[lane * HS + n], fn = s_f[(lane + 1) * HS + n];
+        const float ov = s_o[lane * HS + n], on = s_o[(lane + 1) * HS + n];
+        s_f[lane * HS + n] = q.edge(habc_w(sd, h.ramp, c0 + n, r0 + lane), fv, q.sumz(fn, ov, on));
+      }
+      __syncthreads();
+      for (int r = 0; r < n; ++r) {
+        if (lane < n) {
+          const float fv = s_f[r * HS + lane], fr = s_f[r * HS + lane + 1];
+          const float fd = s_f[(r + 1) * HS + lane];
+          const float ov = s_o[r * HS + lane], orr = s_o[r * HS + lane + 1];
+          const float od = s_o[(r + 1) * HS + lane];
+          s_f[r * HS + lane] = q.corner(habc_w(sd, h.ramp, c0 + lane, r0 + r), fv,
+                                        q.sumx(fr, ov, orr), q.sumz(fd, ov, od));
+        }
+      }
+    } else {
+      // TR (no free surface): block rows 0..n = rows 1..n+1, column 0 =
+      // column nx-n-2.  1) right edge on every block row (+x recurrence);
+      // 2) top edge on block column 0 (Jacobi in z); 3) right-top corner, +x
+      // recurrence per row reading the row below before it is updated.
+      if (act) {
+        float* fr = s_f + lane * HS;
+        const float* orow = s_o + lane * HS;
+        float prev = fr[0], op = orow[0];
+        for (int c = 1; c <= n; ++c) {
+          const float fv = fr[c], ov = orow[c];
+          const float out = q.edge(habc_w(sd, h.ramp, c0 + c, r0 + lane), fv, q.sumx(prev, ov, op));
+          fr[c] = out;
+          prev = out;
+          op = ov;
+        }
+      }
+      __syncthreads();
+      if (lane < n) {
+        const float fv = s_f[lane * HS], fn = s_f[(lane + 1) * HS];
+        const float ov = s_o[lane * HS], on = s_o[(lane + 1) * HS];
+        s_f[lane * HS] = q.edge(habc_w(sd, h.ramp, c0, r0 + lane), fv, q.sumz(fn, ov, on));
+      }
+      __syncthreads();
+      const int rr = min(lane, n - 1);  // lanes >= n shadow row n-1 and do not write
+      float prev = s_f[rr * HS], op = s_o[rr * HS];
+      for (int c = 1; c <= n; ++c) {
+        const float fv = s_f[rr * HS + c], ov = s_o[rr * HS + c];
+        const float fd = s_f[(rr + 1) * HS + c], od = s_o[(rr + 1) * HS + c];
+        const float out = q.corner(habc_w(sd, h.ramp, c0 + c, r0 + rr), fv, q.sumx(prev, ov, op),
+                                   q.sumz(fd, ov, od));
+        wave_sync();
+        if (lane < n) s_f[rr * HS + c] = out;
+        prev = out;
+        op = ov;
+      }
+    }
+    __syncthreads();
+    lds_out(s_f, 0, 0, h.f, P, r0, n + 1, c0, n + 1, lane);
   }
 }
 

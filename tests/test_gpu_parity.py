@@ -21,11 +21,11 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "shot_1.bin")
 @pytest.fixture(scope="module")
 def b():
     bk = E.backend("mi355x")
-    bk.set_tiling(2, 32, 16)
+    bk.set_tiling(2, 0, 16)
     return bk
 
 
-def gpu_shot(b, c, cpt=2, tz=32, gsteps=16):
+def gpu_shot(b, c, cpt=2, tz=0, gsteps=16):
     b.set_tiling(cpt, tz, gsteps)
     W = E.Wavefield(c.M.nx, c.M.nz, b)
     res = E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p)
@@ -75,7 +75,7 @@ def test_tilings_bit_exact(b, cpt, tz, gsteps):
     r.run(1, c.nt)
     assert np.array_equal(res.gather, r.gather_nt_by_nrec())
     assert_same_state(W, r)
-    b.set_tiling(2, 32, 16)
+    b.set_tiling(2, 0, 16)
 
 
 STAGES = [("backup", E.backup_boundary), ("velocity", E.update_velocity),
@@ -162,7 +162,7 @@ def test_batched_uneven_grids_bit_exact(b):
         r.run(1, c.nt)
         assert np.array_equal(ds.gather(), r.gather_nt_by_nrec())
         assert_same_state(W, r, "batch")
-    b.set_tiling(2, 32, 16)
+    b.set_tiling(2, 0, 16)
 
 
 def test_run_shots_batched_equals_sequential(b):
@@ -201,3 +201,21 @@ def test_fp32_gather_within_tolerance_of_unfused_and_fp64_oracle(b):
     e_64 = rel_l2(res.gather, r64.gather_nt_by_nrec())
     print(f"gather rel L2: vs unfused FP32 oracle {e_sep:.2e}, vs FP64 oracle {e_64:.2e}")
     assert e_sep <= 1e-5 and e_64 <= 1e-5
+
+
+def test_closed_form_ramps_equal_weight_matrices(b):
+    c = Case(nt=60)
+    res_w, _ = gpu_shot(b, c)
+    c.H = E.HABCConfig(c.H.nbc, c.H.qx, c.H.qz, c.H.qt_x, c.H.qt_z, c.H.qxt)  # no matrices
+    c.rec = E.Receivers(c.rec.i, c.rec.j, np.zeros((1, len(c.rec.i)), F32), c.rec.type)
+    res_r, _ = gpu_shot(b, c)
+    assert np.array_equal(res_w.gather, res_r.gather)
+
+
+def test_non_ramp_weights_rejected(b):
+    c = Case(nt=10)
+    c.H.w_tau = c.H.w_tau.copy()
+    c.H.w_tau[5, 5] = 0.123
+    W = E.Wavefield(c.M.nx, c.M.nz, b)
+    with pytest.raises(Exception, match="edge distance"):
+        E.bind(b, W, c.M, c.H, c.a, c.p)
