@@ -30,6 +30,15 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Whole-wavefront lane shifts by one (DPP wave_shr:1 / wave_shl:1): a VALU
+// operand modifier, no LDS round trip.  The lane with no source gets 0.
+__device__ __forceinline__ float from_lower_lane(float v) {  // lane i <- lane i-1
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_upper_lane(float v) {  // lane i <- lane i+1
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float uniform_f(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
@@ -543,6 +552,15 @@ __device__ __forceinline__ float habc_w(const ShotDev& sd, const float* ramp, in
   int d = min(min(i, sd.nx - 1 - i), min(j, sd.nz - 1 - j));
   return ramp[min(d, sd.nbc + 1)];
 }
+// Same, with the grid scalars in registers and the ramp in LDS.
+struct HabcGeo {
+  int nx, nz, nbc;
+  const float* w;  // LDS copy of the field's ramp
+  __device__ __forceinline__ float operator()(int i, int j) const {
+    const int d = min(min(i, nx - 1 - i), min(j, nz - 1 - j));
+    return w[min(d, nbc + 1)];
+  }
+};
 
 struct HabcField {
   float* f;
@@ -561,19 +579,43 @@ __device__ __forceinline__ HabcField habc_field(const ShotDev& sd, int stress, i
 
 constexpr int HS = 65;  // LDS row stride: odd, so row and column walks are bank-conflict free
 
-// rows [r0, r0+nr) x cols [c0, c0+nc) of g -> s[r*HS + c]   (nc <= 64)
-__device__ __forceinline__ void lds_in(float* s, const float* g, int P, int r0, int nr, int c0,
-                                       int nc, int lane) {
-#pragma unroll 4
-  for (int r = 0; r < nr; ++r)
-    if (lane < nc) s[r * HS + lane] = ldg(g + (ptrdiff_t)(r0 + r) * P + c0 + lane);
+// rows [r0, r0+nr) x cols [c0, c0+nc) of the field g and its backup go -> s, so
+// (s[r*HS + c], so[r*HS + c]), nc <= 64.  Loads of both arrays go out 32 rows
+// at a time before any LDS store: a chunk costs one memory latency.
+constexpr int STAGE_CHUNK = 32;
+__device__ __forceinline__ void lds_in2(float* s, float* so, const float* g, const float* go,
+                                        int P, int r0, int nr, int c0, int nc, int lane) {
+  const bool on = lane < nc;
+  for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
+    float v[STAGE_CHUNK], w[STAGE_CHUNK];
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r) {
+      const bool ok = on && rb + r < nr;
+      const ptrdiff_t o = (ptrdiff_t)(r0 + rb + r) * P + c0 + lane;
+      v[r] = ok ? ldg(g + o) : 0.f;
+      w[r] = ok ? ldg(go + o) : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r)
+      if (on && rb + r < nr) {
+        s[(rb + r) * HS + lane] = v[r];
+        so[(rb + r) * HS + lane] = w[r];
+      }
+  }
 }
 // s[(sr+r)*HS + sc + c] -> rows [r0, r0+nr) x cols [c0, c0+nc) of g   (nc <= 64)
 __device__ __forceinline__ void lds_out(const float* s, int sr, int sc, float* g, int P, int r0,
                                         int nr, int c0, int nc, int lane) {
-#pragma unroll 4
-  for (int r = 0; r < nr; ++r)
-    if (lane < nc) stg(g + (ptrdiff_t)(r0 + r) * P + c0 + lane, s[(sr + r) * HS + sc + lane]);
+  const bool on = lane < nc;
+  for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
+    float v[STAGE_CHUNK];
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r)
+      v[r] = (on && rb + r < nr) ? s[(sr + rb + r) * HS + sc + lane] : 0.f;
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r)
+      if (on && rb + r < nr) stg(g + (ptrdiff_t)(r0 + rb + r) * P + c0 + lane, v[r]);
+  }
 }
 
 // Items per field of the HABC partition and the first side-edge row.
@@ -582,6 +624,7 @@ __device__ __forceinline__ int habc_side_r0(const ShotDev& sd) { return sd.fs ? 
 __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
   __shared__ float s_f[64 * HS];
   __shared__ float s_o[64 * HS];
+  __shared__ float s_w[MAX_NBC + 2];
   const int lane = threadIdx.x;
   const int nsh = *ba.n_shots_p;
   const int total = ba.prefix[nsh];
@@ -594,28 +637,38 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
     const int per = sd.n_l + sd.n_r + sd.n_b + sd.n_t + ncorner;
     const int fi = t / per;
     int p = t % per;
-    const HabcField h = habc_field(sd, stress, fi);
-    const Hq q{sd.qx, sd.qz, sd.qt_x, sd.qt_z, sd.qxt};
-    const int n = sd.nbc, nx = sd.nx, nz = sd.nz, P = sd.pitch;
+    // piece order: corners first (the longest recurrences start first), then
+    // left/right edge chunks, then bottom/top edge chunks
+    const bool is_corner = p < ncorner;
+    const int pc = p;
+    p -= ncorner;
+    HabcField h = habc_field(sd, stress, fi);
+    h.f = uniform_p(h.f);
+    h.fo = uniform_p(h.fo);
+    const Hq q{uniform_f(sd.qx), uniform_f(sd.qz), uniform_f(sd.qt_x), uniform_f(sd.qt_z),
+               uniform_f(sd.qxt)};
+    const int n = uniform(sd.nbc), nx = uniform(sd.nx), nz = uniform(sd.nz), P = uniform(sd.pitch);
     const int ra = habc_side_r0(sd), rb = nz - n - 2;  // side-edge rows [ra, rb)
     const int ca = n + 2, cb = nx - n - 2;             // bottom/top-edge columns [ca, cb)
     __syncthreads();  // LDS of the previous item is no longer read
+    if (lane < n + 2) s_w[lane] = ldg(h.ramp + lane);
+    const HabcGeo W{nx, nz, n, s_w};
 
-    if (p < sd.n_l + sd.n_r) {
+    if (!is_corner && p < sd.n_l + sd.n_r) {
       const bool right = p >= sd.n_l;
       const int j0 = ra + (right ? p - sd.n_l : p) * 64;
       const int nr = min(64, rb - j0);
       const int c0 = right ? nx - n - 2 : 1;  // staged columns [c0, c0+n+1)
-      lds_in(s_f, h.f, P, j0, nr, c0, n + 1, lane);
-      lds_in(s_o, h.fo, P, j0, nr, c0, n + 1, lane);
+      lds_in2(s_f, s_o, h.f, h.fo, P, j0, nr, c0, n + 1, lane);
       __syncthreads();
       if (!right) {
-        // left edge, i = 1..n: Jacobi on the row's pre-HABC values
+        // left edge, i = 1..n: Jacobi on the row's pre-HABC values (rows independent)
+#pragma unroll 4
         for (int r = 0; r < nr; ++r) {
           if (lane < n) {
             const float fv = s_f[r * HS + lane], fn = s_f[r * HS + lane + 1];
             const float ov = s_o[r * HS + lane], on = s_o[r * HS + lane + 1];
-            const float out = q.edge(habc_w(sd, h.ramp, 1 + lane, j0 + r), fv, q.sumx(fn, ov, on));
+            const float out = q.edge(W(1 + lane, j0 + r), fv, q.sumx(fn, ov, on));
             s_f[r * HS + lane] = out;
           }
         }
@@ -624,12 +677,17 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
         float* fr = s_f + lane * HS;
         const float* orow = s_o + lane * HS;
         float prev = fr[0], op = orow[0];
+        float fv = fr[1], ov = orow[1], w = W(c0 + 1, j0 + lane);
         for (int c = 1; c <= n; ++c) {
-          const float fv = fr[c], ov = orow[c];
-          const float out = q.edge(habc_w(sd, h.ramp, c0 + c, j0 + lane), fv, q.sumx(prev, ov, op));
+          const int cn = min(c + 1, n);  // next step's operands (pre-values)
+          const float nfv = fr[cn], nov = orow[cn], nw = W(c0 + cn, j0 + lane);
+          const float out = q.edge(w, fv, q.sumx(prev, ov, op));
           fr[c] = out;
           prev = out;
           op = ov;
+          fv = nfv;
+          ov = nov;
+          w = nw;
         }
       }
       __syncthreads();
@@ -638,32 +696,36 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       continue;
     }
     p -= sd.n_l + sd.n_r;
-    if (p < sd.n_b + sd.n_t) {
+    if (!is_corner) {
       const bool top = p >= sd.n_b;
       const int i0 = ca + (top ? p - sd.n_b : p) * 64;
       const int nc = min(64, cb - i0);
       const int r0 = top ? 1 : nz - n - 2;  // staged rows [r0, r0+n+1)
-      lds_in(s_f, h.f, P, r0, n + 1, i0, nc, lane);
-      lds_in(s_o, h.fo, P, r0, n + 1, i0, nc, lane);
+      lds_in2(s_f, s_o, h.f, h.fo, P, r0, n + 1, i0, nc, lane);
       __syncthreads();
       if (lane < nc) {
         const int i = i0 + lane;
         if (!top) {
           // bottom edge: recurrence along +z from row nz-n-2
           float prev = s_f[lane], op = s_o[lane];
+          float fv = s_f[HS + lane], ov = s_o[HS + lane], w = W(i, r0 + 1);
           for (int r = 1; r <= n; ++r) {
-            const float fv = s_f[r * HS + lane], ov = s_o[r * HS + lane];
-            const float out = q.edge(habc_w(sd, h.ramp, i, r0 + r), fv, q.sumz(prev, ov, op));
+            const int rn = min(r + 1, n);  // next step's operands (pre-values)
+            const float nfv = s_f[rn * HS + lane], nov = s_o[rn * HS + lane], nw = W(i, r0 + rn);
+            const float out = q.edge(w, fv, q.sumz(prev, ov, op));
             s_f[r * HS + lane] = out;
             prev = out;
             op = ov;
+            fv = nfv;
+            ov = nov;
+            w = nw;
           }
         } else {
           // top edge (no free surface), rows 1..n: Jacobi in z, rows ascending
           for (int r = 0; r < n; ++r) {
             const float fv = s_f[r * HS + lane], fn = s_f[(r + 1) * HS + lane];
             const float ov = s_o[r * HS + lane], on = s_o[(r + 1) * HS + lane];
-            s_f[r * HS + lane] = q.edge(habc_w(sd, h.ramp, i, r0 + r), fv, q.sumz(fn, ov, on));
+            s_f[r * HS + lane] = q.edge(W(i, r0 + r), fv, q.sumz(fn, ov, on));
           }
         }
       }
@@ -672,14 +734,12 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       lds_out(s_f, top ? 0 : 1, 0, h.f, P, top ? r0 : r0 + 1, n, i0, nc, lane);
       continue;
     }
-    p -= sd.n_b + sd.n_t;
     // Corner pieces: an (n+1) x (n+1) block.
-    const bool bottom = p < 2;
-    const bool leftc = (p % 2) == 0;
+    const bool bottom = pc < 2;
+    const bool leftc = (pc % 2) == 0;
     const int r0 = bottom ? nz - n - 2 : 1;
     const int c0 = leftc ? 1 : nx - n - 2;
-    lds_in(s_f, h.f, P, r0, n + 1, c0, n + 1, lane);
-    lds_in(s_o, h.fo, P, r0, n + 1, c0, n + 1, lane);
+    lds_in2(s_f, s_o, h.f, h.fo, P, r0, n + 1, c0, n + 1, lane);
     __syncthreads();
     const bool act = lane <= n;
     if (bottom && leftc) {
@@ -689,18 +749,18 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       // before update; the column-(n+1) neighbour is the new bottom value).
       if (lane < n) {
         const float fv = s_f[lane], fn = s_f[lane + 1], ov = s_o[lane], on = s_o[lane + 1];
-        s_f[lane] = q.edge(habc_w(sd, h.ramp, c0 + lane, r0), fv, q.sumx(fn, ov, on));
+        s_f[lane] = q.edge(W(c0 + lane, r0), fv, q.sumx(fn, ov, on));
       }
       __syncthreads();
       float up = act ? s_f[lane] : 0.f, ou = act ? s_o[lane] : 0.f;
       for (int r = 1; r <= n; ++r) {
         const float fv = act ? s_f[r * HS + lane] : 0.f;
         const float ov = act ? s_o[r * HS + lane] : 0.f;
-        const float w = act ? habc_w(sd, h.ramp, c0 + lane, r0 + r) : 0.f;
+        const float w = act ? W(c0 + lane, r0 + r) : 0.f;
         float x = fv;
         if (lane == n) x = q.edge(w, fv, q.sumz(up, ov, ou));
-        const float fr = __shfl_down(x, 1);
-        const float orr = __shfl_down(ov, 1);
+        const float fr = from_upper_lane(x);
+        const float orr = from_upper_lane(ov);
         const float out = (lane == n) ? x : q.corner(w, fv, q.sumx(fr, ov, orr), q.sumz(up, ov, ou));
         if (act) s_f[r * HS + lane] = out;
         up = out;
@@ -712,20 +772,41 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       // nx-n-2 (recurrence along +z), the rest the right-bottom corner.
       const int L = lane;
       float last = act ? s_f[L] : 0.f;  // lane 0 starts with cell (0,0), which is not updated
-      for (int d = 1; d <= 2 * n; ++d) {
-        const float left = __shfl_up(last, 1);
+      // operands of the cell this lane handles at step d (prefetched one step ahead)
+      auto fetch = [&](int d, float& fv, float& ov, float& ol, float& ou, float& w) {
         const int r = d - L;
+        const bool ok = act && r >= 0 && r <= n;
+        const int rr = ok ? r : 0;
+        fv = s_f[rr * HS + L];
+        ov = s_o[rr * HS + L];
+        ol = s_o[rr * HS + max(L - 1, 0)];
+        ou = s_o[max(rr - 1, 0) * HS + L];
+        w = W(c0 + L, r0 + rr);
+      };
+      float fv, ov, ol, ou, w;
+      fetch(1, fv, ov, ol, ou, w);
+      for (int d = 1; d <= 2 * n; ++d) {
+        float nfv, nov, nol, nou, nw;
+        fetch(d + 1, nfv, nov, nol, nou, nw);
+        const float left = from_lower_lane(last);
+        const int r = d - L;
+        // One formula for the three cell kinds: the bottom-edge column has no
+        // x-term, the right-edge row no z-term, and their weight factor is
+        // (1-w) instead of (1-w)/2; adding an exact 0 keeps each result
+        // identical to its own formula.
+        const float sx = (L == 0) ? 0.f : q.sumx(left, ov, ol);
+        const float sz = (r == 0) ? 0.f : q.sumz(last, ov, ou);
+        const float cw = (L == 0 || r == 0) ? (1.0f - w) : (1.0f - w) * 0.5f;
+        const float out = fmaf_(cw, sx + sz, w * fv);
         if (act && r >= 0 && r <= n && !(L == 0 && r == 0)) {
-          const float fv = s_f[r * HS + L], ov = s_o[r * HS + L];
-          const float w = habc_w(sd, h.ramp, c0 + L, r0 + r);
-          float out;
-          if (r == 0) out = q.edge(w, fv, q.sumx(left, ov, s_o[L - 1]));
-          else if (L == 0) out = q.edge(w, fv, q.sumz(last, ov, s_o[(r - 1) * HS]));
-          else out = q.corner(w, fv, q.sumx(left, ov, s_o[r * HS + L - 1]),
-                              q.sumz(last, ov, s_o[(r - 1) * HS + L]));
           s_f[r * HS + L] = out;
           last = out;
         }
+        fv = nfv;
+        ov = nov;
+        ol = nol;
+        ou = nou;
+        w = nw;
       }
     } else if (leftc) {
       // TL (no free surface): block rows 0..n = rows 1..n+1, columns 0..n =
@@ -735,14 +816,14 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
         if (lane < n) {
           const float fv = s_f[r * HS + lane], fn = s_f[r * HS + lane + 1];
           const float ov = s_o[r * HS + lane], on = s_o[r * HS + lane + 1];
-          s_f[r * HS + lane] = q.edge(habc_w(sd, h.ramp, c0 + lane, r0 + r), fv, q.sumx(fn, ov, on));
+          s_f[r * HS + lane] = q.edge(W(c0 + lane, r0 + r), fv, q.sumx(fn, ov, on));
         }
       }
       __syncthreads();
       if (lane < n) {
         const float fv = s_f[lane * HS + n], fn = s_f[(lane + 1) * HS + n];
         const float ov = s_o[lane * HS + n], on = s_o[(lane + 1) * HS + n];
-        s_f[lane * HS + n] = q.edge(habc_w(sd, h.ramp, c0 + n, r0 + lane), fv, q.sumz(fn, ov, on));
+        s_f[lane * HS + n] = q.edge(W(c0 + n, r0 + lane), fv, q.sumz(fn, ov, on));
       }
       __syncthreads();
       for (int r = 0; r < n; ++r) {
@@ -751,7 +832,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
           const float fd = s_f[(r + 1) * HS + lane];
           const float ov = s_o[r * HS + lane], orr = s_o[r * HS + lane + 1];
           const float od = s_o[(r + 1) * HS + lane];
-          s_f[r * HS + lane] = q.corner(habc_w(sd, h.ramp, c0 + lane, r0 + r), fv,
+          s_f[r * HS + lane] = q.corner(W(c0 + lane, r0 + r), fv,
                                         q.sumx(fr, ov, orr), q.sumz(fd, ov, od));
         }
       }
@@ -766,7 +847,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
         float prev = fr[0], op = orow[0];
         for (int c = 1; c <= n; ++c) {
           const float fv = fr[c], ov = orow[c];
-          const float out = q.edge(habc_w(sd, h.ramp, c0 + c, r0 + lane), fv, q.sumx(prev, ov, op));
+          const float out = q.edge(W(c0 + c, r0 + lane), fv, q.sumx(prev, ov, op));
           fr[c] = out;
           prev = out;
           op = ov;
@@ -776,7 +857,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       if (lane < n) {
         const float fv = s_f[lane * HS], fn = s_f[(lane + 1) * HS];
         const float ov = s_o[lane * HS], on = s_o[(lane + 1) * HS];
-        s_f[lane * HS] = q.edge(habc_w(sd, h.ramp, c0, r0 + lane), fv, q.sumz(fn, ov, on));
+        s_f[lane * HS] = q.edge(W(c0, r0 + lane), fv, q.sumz(fn, ov, on));
       }
       __syncthreads();
       const int rr = min(lane, n - 1);  // lanes >= n shadow row n-1 and do not write
@@ -784,7 +865,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       for (int c = 1; c <= n; ++c) {
         const float fv = s_f[rr * HS + c], ov = s_o[rr * HS + c];
         const float fd = s_f[(rr + 1) * HS + c], od = s_o[(rr + 1) * HS + c];
-        const float out = q.corner(habc_w(sd, h.ramp, c0 + c, r0 + rr), fv, q.sumx(prev, ov, op),
+        const float out = q.corner(W(c0 + c, r0 + rr), fv, q.sumx(prev, ov, op),
                                    q.sumz(fd, ov, od));
         wave_sync();
         if (lane < n) s_f[rr * HS + c] = out;
