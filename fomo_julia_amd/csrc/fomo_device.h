@@ -15,7 +15,7 @@
 namespace fomo {
 
 constexpr int WAVE = 64;
-constexpr int GUARD_ROWS = 10;  // >= max half-order (8) + 2
+constexpr int GUARD_ROWS = 18;  // >= 2 * max half-order (8) + 2: the fused z-march reads 2M rows out
 constexpr int SLACK = 256;      // floats before/after a field block
 constexpr int MAX_M = 8;        // fd_order <= 16
 constexpr int MAX_NBC = 63;     // one wavefront spans an HABC strip's width
@@ -27,14 +27,23 @@ enum Ramp { RAMP_VX = 0, RAMP_VZ = 1, RAMP_TAU = 2 };
 
 // Work-partition kinds; each has a prefix-sum table over the shots of a batch.
 enum Part {
-  PART_STENCIL = 0,   // velocity / stress tiles
+  PART_STENCIL = 0,   // velocity / stress tiles (in-place two-pass kernels)
   PART_HABC_V,        // HABC strip pieces x 2 velocity fields
   PART_HABC_S,        // HABC strip pieces x 3 stress fields
   PART_FSREC,         // free-surface rows + receivers
   PART_BACKUP,        // standalone strip backup (per-stage API)
   PART_SHOT,          // one item per shot
+  PART_FUSED,         // fused velocity+stress tiles (ping-pong step)
+  PART_BAND,          // stress band tiles next to the HABC strips (ping-pong step)
   PART_COUNT
 };
+
+// Stress cells whose stencil reads velocity cells the HABC changes: the
+// fused kernel leaves them to the band kernel, which runs after the velocity
+// HABC.  Bands (0-based, half-open): left columns [0, b), right columns
+// [nx-b, nx), bottom rows [nz-b, nz), top rows [0, b) without a free surface,
+// with b = nbc + M + 1.
+constexpr int BAND_ROWS = 64;  // rows per band-kernel item
 
 // Everything a kernel needs to know about one shot.  Lives in device memory
 // (a table indexed by shot), so a captured graph stays valid when the host
@@ -43,6 +52,10 @@ struct ShotDev {
   int nx, nz, pitch, nbc, pad, fs;
   int tz;               // rows per stencil tile
   int tiles_x, tiles_z; // stencil tiling
+  int ftw;              // fused kernel: output columns per tile (64*CPT - 2*halo)
+  int ftiles_x, ftiles_z;
+  int band;             // band width b = nbc + M + 1
+  int nb_side, nb_mid;  // band items: per side band, per bottom/top band
   int n_l, n_r, n_b, n_t;  // HABC side/bottom/top edge pieces per field
   float dtx, dtz;
   float* f[5];          // vx vz txx tzz txz   (pointer to cell (0,0))

@@ -6,7 +6,8 @@
 //                   coefficient arrays the kernels read (dt/rho per staggered
 //                   point and direction, lambda, lambda+2mu, mu_txz), the HABC
 //                   weight ramps, FD coefficients and scalars
-//   fomo_wavefield  the 10 fields of Wavefield (5 current + 5 strip backups)
+//   fomo_wavefield  two state buffers of the 5 fields (the fused step reads one
+//                   and writes the other) + the 5 strip backups of Wavefield
 //   fomo_shot       source cell + wavelet, receivers, gather [nt][n_rec], step counter
 //   fomo_batch      a device table of ShotDev rows + per-kernel work lists;
 //                   single-shot calls use a one-row batch cached on the wavefield
@@ -33,6 +34,7 @@ struct fomo_ctx {
   int tz = 0;          // rows per stencil tile; 0 = sized to fill the GPU in whole rounds
   int graph_steps = 16;
   int cus = 0;         // compute units of the device
+  int step_mode = 1;   // 1 = fused ping-pong step, 0 = two-pass in-place step
 };
 
 struct fomo_model {
@@ -55,7 +57,12 @@ struct fomo_batch;
 struct fomo_wavefield {
   fomo_model* m = nullptr;
   float* dbuf = nullptr;
-  float* f[10] = {};
+  float* buf[2][5] = {};  // state buffers
+  float* old[5] = {};     // strip backups (*_old) of the in-place kernels
+  int cur = 0;            // buffer holding the current state
+  // after a fused step the backup is the previous buffer's strip cells; old[]
+  // is refreshed from it only when the in-place path or a download needs it
+  bool backup_is_prev = false;
   int64_t bytes = 0;
   fomo_shot* placeholder = nullptr;  // idle shot for stage calls that take none
   // one-shot batches cached per wavefield: [0] placeholder, [1] last real shot
@@ -77,17 +84,23 @@ struct fomo_shot {
 
 static uint64_t g_next_shot_id = 1;
 
+// Shot tables of a batch: in-place kernels on buffer 0 / 1 (f = buffer,
+// fo = strip backups) and the fused step from buffer 0 / 1 (fo = read
+// buffer, f = written buffer).
+enum Table { TBL_IN0 = 0, TBL_IN1, TBL_FUSED0, TBL_FUSED1, TBL_COUNT };
+
 struct fomo_batch {
   fomo_ctx* ctx = nullptr;
-  int n = 0, M = 0, cpt = 2, tz = 32, graph_steps = 16;
+  int n = 0, M = 0, cpt = 2, tz = 32, graph_steps = 16, step_mode = 1;
   int tz_req = 0;  // the context's rows_per_tile when built (0 = auto)
-  std::vector<ShotDev> hs;
+  std::vector<fomo_wavefield*> ws;
+  std::vector<ShotDev> hs;  // [TBL_COUNT][n]
   ShotDev* ds = nullptr;
   int* dn = nullptr;
   int* dprefix = nullptr;  // [PART_COUNT][n+1]
   std::vector<int> hprefix;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // by the parity the chunk starts from
+  hipGraph_t graph[2] = {nullptr, nullptr};
 };
 
 static thread_local std::string g_err;
@@ -109,7 +122,8 @@ static int fail(fomo_ctx* c, int code, const std::string& msg) {
 static int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // ------------------------------------------------------------------ layout
-static int pitch_for(int nx) { return round_up(nx + 8, 256); }
+// The fused kernel's last tile reaches up to ftw + halo + M < 272 columns past nx.
+static int pitch_for(int nx) { return round_up(nx + 272, 256); }
 static size_t field_stride(int nz, int pitch) {
   return (size_t)(nz + 2 * GUARD_ROWS) * pitch + 2 * SLACK;
 }
@@ -157,11 +171,19 @@ extern "C" int32_t fomo_ctx_synchronize(fomo_ctx* c) {
 
 extern "C" void* fomo_ctx_stream(fomo_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
+extern "C" int32_t fomo_ctx_set_step_mode(fomo_ctx* c, int32_t mode) {
+  if (!c) return fail(nullptr, FOMO_ERR_ARG, "ctx is NULL");
+  if (mode != 0 && mode != 1) return fail(c, FOMO_ERR_ARG, "step mode must be 0 or 1");
+  c->step_mode = mode;
+  return FOMO_OK;
+}
+
 extern "C" int32_t fomo_ctx_set_tiling(fomo_ctx* c, int32_t cpt, int32_t tz, int32_t gsteps) {
   if (!c) return fail(nullptr, FOMO_ERR_ARG, "ctx is NULL");
   if (cpt != 1 && cpt != 2 && cpt != 4) return fail(c, FOMO_ERR_ARG, "cols_per_lane must be 1, 2 or 4");
   if (tz < 0 || tz > 4096) return fail(c, FOMO_ERR_ARG, "rows_per_tile out of range (0 = auto)");
-  if (gsteps < 1 || gsteps > 256) return fail(c, FOMO_ERR_ARG, "steps_per_graph out of range");
+  if (gsteps < 2 || gsteps > 256 || gsteps % 2)
+    return fail(c, FOMO_ERR_ARG, "steps_per_graph must be even, 2..256");
   c->cpt = cpt;
   c->tz = tz;
   c->graph_steps = gsteps;
@@ -287,17 +309,22 @@ extern "C" int64_t fomo_model_device_bytes(const fomo_model* m) { return m ? m->
 
 // ------------------------------------------------------------------ wavefield
 static void drop_single(fomo_wavefield* w);
+static int refresh_backup(fomo_wavefield* w);
 
 extern "C" int32_t fomo_wavefield_create(fomo_model* m, fomo_wavefield** out) {
   if (!m || !out) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   fomo_ctx* c = m->ctx;
   auto w = std::make_unique<fomo_wavefield>();
   w->m = m;
-  const size_t bytes = 10 * m->fstride * sizeof(float);
+  const size_t bytes = 15 * m->fstride * sizeof(float);
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipMalloc(&w->dbuf, bytes));
   HIP_TRY(c, hipMemset(w->dbuf, 0, bytes));
-  for (int k = 0; k < 10; ++k) w->f[k] = block_origin(w->dbuf, m->fstride, k, m->pitch);
+  for (int k = 0; k < 5; ++k) {
+    w->buf[0][k] = block_origin(w->dbuf, m->fstride, k, m->pitch);
+    w->buf[1][k] = block_origin(w->dbuf, m->fstride, 5 + k, m->pitch);
+    w->old[k] = block_origin(w->dbuf, m->fstride, 10 + k, m->pitch);
+  }
   w->bytes = (int64_t)bytes;
   *out = w.release();
   return FOMO_OK;
@@ -318,18 +345,28 @@ extern "C" int32_t fomo_wavefield_reset(fomo_wavefield* w) {
   fomo_ctx* c = w->m->ctx;
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipMemsetAsync(w->dbuf, 0, (size_t)w->bytes, c->stream));
+  w->cur = 0;
+  w->backup_is_prev = false;
   return FOMO_OK;
 }
 
 extern "C" int64_t fomo_wavefield_device_bytes(const fomo_wavefield* w) { return w ? w->bytes : 0; }
 
+static float* field_ptr(fomo_wavefield* w, int field) {
+  return field < 5 ? w->buf[w->cur][field] : w->old[field - 5];
+}
+
 extern "C" int32_t fomo_wavefield_download(fomo_wavefield* w, int32_t field, float* host) {
   if (!w || !host) return fail(w ? w->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   fomo_ctx* c = w->m->ctx;
   if (field < 0 || field > 9) return fail(c, FOMO_ERR_ARG, "field id out of range");
+  if (field >= 5) {
+    int rc = refresh_backup(w);
+    if (rc) return rc;
+  }
   const int nx = w->m->d.nx, nz = w->m->d.nz, P = w->m->pitch;
   HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpy2DAsync(host, nx * sizeof(float), w->f[field], P * sizeof(float),
+  HIP_TRY(c, hipMemcpy2DAsync(host, nx * sizeof(float), field_ptr(w, field), P * sizeof(float),
                               nx * sizeof(float), nz, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return FOMO_OK;
@@ -339,9 +376,13 @@ extern "C" int32_t fomo_wavefield_upload(fomo_wavefield* w, int32_t field, const
   if (!w || !host) return fail(w ? w->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   fomo_ctx* c = w->m->ctx;
   if (field < 0 || field > 9) return fail(c, FOMO_ERR_ARG, "field id out of range");
+  if (field >= 5) {
+    int rc = refresh_backup(w);
+    if (rc) return rc;
+  }
   const int nx = w->m->d.nx, nz = w->m->d.nz, P = w->m->pitch;
   HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpy2DAsync(w->f[field], P * sizeof(float), host, nx * sizeof(float),
+  HIP_TRY(c, hipMemcpy2DAsync(field_ptr(w, field), P * sizeof(float), host, nx * sizeof(float),
                               nx * sizeof(float), nz, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return FOMO_OK;
@@ -438,7 +479,7 @@ extern "C" int32_t fomo_shot_gather(fomo_shot* s, float* out) {
 
 // ------------------------------------------------------------------ batches
 static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const fomo_shot* s,
-                            int cpt, int tz) {
+                            int cpt, int tz, int table) {
   ShotDev sd;
   std::memset(&sd, 0, sizeof(sd));
   const fomo_model_desc& d = m->d;
@@ -463,9 +504,23 @@ static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const 
   sd.dtx = d.dtx;
   sd.dtz = d.dtz;
   for (int k = 0; k < 5; ++k) {
-    sd.f[k] = w->f[k];
-    sd.fo[k] = w->f[5 + k];
+    switch (table) {
+      case TBL_IN0: sd.f[k] = w->buf[0][k]; sd.fo[k] = w->old[k]; break;
+      case TBL_IN1: sd.f[k] = w->buf[1][k]; sd.fo[k] = w->old[k]; break;
+      case TBL_FUSED0: sd.f[k] = w->buf[1][k]; sd.fo[k] = w->buf[0][k]; break;
+      default: sd.f[k] = w->buf[0][k]; sd.fo[k] = w->buf[1][k]; break;
+    }
   }
+  // fused kernel tiling: lanes cover 64*cpt columns, the outer halo lanes
+  // (M rounded up to cpt) are computed but not written
+  const int M = m->M;
+  const int hl = (M + cpt - 1) / cpt * cpt;
+  sd.ftw = 64 * cpt - 2 * hl;
+  sd.ftiles_x = (d.nx + sd.ftw - 1) / sd.ftw;
+  sd.ftiles_z = (d.nz + tz - 1) / tz;
+  sd.band = d.nbc + M + 1;
+  sd.nb_side = ((sd.band + 63) / 64) * ((d.nz + BAND_ROWS - 1) / BAND_ROWS);
+  sd.nb_mid = ((d.nx - 2 * sd.band + 63) / 64) * ((sd.band + BAND_ROWS - 1) / BAND_ROWS);
   sd.bvx_x = m->coef[0];
   sd.bvx_z = m->coef[1];
   sd.bvz_x = m->coef[2];
@@ -504,6 +559,8 @@ static int part_items(const ShotDev& sd, int part) {
     case PART_FSREC: return (6 * sd.nx + sd.n_rec + 63) / 64;
     case PART_BACKUP: return (int)(((int64_t)sd.nx * sd.nz + 63) / 64);
     case PART_SHOT: return 1;
+    case PART_FUSED: return sd.ftiles_x * sd.ftiles_z;
+    case PART_BAND: return 2 * sd.nb_side + (sd.fs ? 1 : 2) * sd.nb_mid;
   }
   return 0;
 }
@@ -512,29 +569,35 @@ constexpr int WPB = 4;  // wavefronts per block of the stencil kernels
 
 // Resident wavefronts of the stencil kernels for (M, cpt) on the whole device.
 template <int M, int CPT>
-static int stencil_slots_mc(int cus) {
+static int stencil_slots_mc(int cus, int mode) {
   int bv = 0, bs = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_velocity<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
-    bv = 1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, k_stress<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
-    bs = 1;
+  if (mode == 1) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
+      bv = 1;
+    bs = bv;
+  } else {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_velocity<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
+      bv = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, k_stress<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
+      bs = 1;
+  }
   return std::max(1, std::min(bv, bs)) * WPB * std::max(cus, 1);
 }
 template <int M>
-static int stencil_slots_m(int cpt, int cus) {
-  return cpt == 1 ? stencil_slots_mc<M, 1>(cus) : cpt == 2 ? stencil_slots_mc<M, 2>(cus)
-                                                            : stencil_slots_mc<M, 4>(cus);
+static int stencil_slots_m(int cpt, int cus, int mode) {
+  return cpt == 1 ? stencil_slots_mc<M, 1>(cus, mode)
+                  : cpt == 2 ? stencil_slots_mc<M, 2>(cus, mode) : stencil_slots_mc<M, 4>(cus, mode);
 }
-static int stencil_slots(int M, int cpt, int cus) {
+static int stencil_slots(int M, int cpt, int cus, int mode) {
   switch (M) {
-    case 1: return stencil_slots_m<1>(cpt, cus);
-    case 2: return stencil_slots_m<2>(cpt, cus);
-    case 3: return stencil_slots_m<3>(cpt, cus);
-    case 4: return stencil_slots_m<4>(cpt, cus);
-    case 5: return stencil_slots_m<5>(cpt, cus);
-    case 6: return stencil_slots_m<6>(cpt, cus);
-    case 7: return stencil_slots_m<7>(cpt, cus);
-    default: return stencil_slots_m<8>(cpt, cus);
+    case 1: return stencil_slots_m<1>(cpt, cus, mode);
+    case 2: return stencil_slots_m<2>(cpt, cus, mode);
+    case 3: return stencil_slots_m<3>(cpt, cus, mode);
+    case 4: return stencil_slots_m<4>(cpt, cus, mode);
+    case 5: return stencil_slots_m<5>(cpt, cus, mode);
+    case 6: return stencil_slots_m<6>(cpt, cus, mode);
+    case 7: return stencil_slots_m<7>(cpt, cus, mode);
+    default: return stencil_slots_m<8>(cpt, cus, mode);
   }
 }
 
@@ -565,8 +628,10 @@ static int auto_tz(const std::vector<int>& tiles_x, const std::vector<int>& nz, 
 
 static void destroy_batch(fomo_batch* b) {
   if (!b) return;
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (int q = 0; q < 2; ++q) {
+    if (b->gexec[q]) (void)hipGraphExecDestroy(b->gexec[q]);
+    if (b->graph[q]) (void)hipGraphDestroy(b->graph[q]);
+  }
   (void)hipFree(b->ds);
   (void)hipFree(b->dn);
   (void)hipFree(b->dprefix);
@@ -583,22 +648,31 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
   b->tz = c->tz;
   b->tz_req = c->tz;
   b->graph_steps = c->graph_steps;
+  b->step_mode = c->step_mode;
   b->M = ms[0]->M;
   for (int i = 0; i < n; ++i) {
     if (!ms[i] || !ws[i] || !ss[i]) return fail(c, FOMO_ERR_ARG, "NULL batch member");
     if (ms[i]->ctx != c || ws[i]->m != ms[i] || ss[i]->m != ms[i])
       return fail(c, FOMO_ERR_STATE, "batch member belongs to another model or context");
     if (ms[i]->M != b->M) return fail(c, FOMO_ERR_UNSUPPORTED, "batch mixes FD orders");
+    for (int j = 0; j < i; ++j)
+      if (ws[j] == ws[i]) return fail(c, FOMO_ERR_ARG, "a wavefield appears twice in the batch");
+    b->ws.push_back(ws[i]);
   }
   if (b->tz == 0) {
     std::vector<int> tx, nzs;
     for (int i = 0; i < n; ++i) {
-      tx.push_back((ms[i]->d.nx + 64 * b->cpt - 1) / (64 * b->cpt));
+      const int M = ms[i]->M, hl = (M + b->cpt - 1) / b->cpt * b->cpt;
+      const int w = b->step_mode == 1 ? 64 * b->cpt - 2 * hl : 64 * b->cpt;
+      tx.push_back((ms[i]->d.nx + w - 1) / w);
       nzs.push_back(ms[i]->d.nz);
     }
-    b->tz = auto_tz(tx, nzs, stencil_slots(b->M, b->cpt, c->cus));
+    b->tz = auto_tz(tx, nzs, stencil_slots(b->M, b->cpt, c->cus, b->step_mode));
   }
-  for (int i = 0; i < n; ++i) b->hs.push_back(make_shotdev(ms[i], ws[i], ss[i], b->cpt, b->tz));
+  b->hs.resize((size_t)TBL_COUNT * n);
+  for (int t = 0; t < TBL_COUNT; ++t)
+    for (int i = 0; i < n; ++i)
+      b->hs[(size_t)t * n + i] = make_shotdev(ms[i], ws[i], ss[i], b->cpt, b->tz, t);
   b->hprefix.assign((size_t)PART_COUNT * (n + 1), 0);
   for (int p = 0; p < PART_COUNT; ++p) {
     int acc = 0;
@@ -611,8 +685,8 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
     b->hprefix[(size_t)p * (n + 1) + n] = acc;
   }
   HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMalloc(&b->ds, n * sizeof(ShotDev)));
-  HIP_TRY(c, hipMemcpy(b->ds, b->hs.data(), n * sizeof(ShotDev), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMalloc(&b->ds, b->hs.size() * sizeof(ShotDev)));
+  HIP_TRY(c, hipMemcpy(b->ds, b->hs.data(), b->hs.size() * sizeof(ShotDev), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMalloc(&b->dn, sizeof(int)));
   HIP_TRY(c, hipMemcpy(b->dn, &n, sizeof(int), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMalloc(&b->dprefix, b->hprefix.size() * sizeof(int)));
@@ -648,7 +722,7 @@ static int single_batch(fomo_model* m, fomo_wavefield* w, fomo_shot* s, fomo_bat
   const int q = (s == w->placeholder) ? 0 : 1;
   fomo_batch* b = w->single[q];
   if (b && w->single_id[q] == s->id && b->cpt == c->cpt && b->tz_req == c->tz &&
-      b->graph_steps == c->graph_steps) {
+      b->graph_steps == c->graph_steps && b->step_mode == c->step_mode) {
     *out = b;
     return FOMO_OK;
   }
@@ -663,71 +737,99 @@ static int single_batch(fomo_model* m, fomo_wavefield* w, fomo_shot* s, fomo_bat
 }
 
 // ------------------------------------------------------------------ launches
-
-static int blocks_for(const fomo_batch* b, int part) {
+static int blocks_for(const fomo_batch* b, int part, int wpb = WPB) {
   const int total = b->hprefix[(size_t)part * (b->n + 1) + b->n];
-  return std::max(1, (total + WPB - 1) / WPB);
+  return std::max(1, (total + wpb - 1) / wpb);
 }
 
-static BatchArgs args_for(const fomo_batch* b, int part, int k_off) {
+static BatchArgs args_for(const fomo_batch* b, int table, int part, int k_off) {
   BatchArgs a;
-  a.shots = b->ds;
+  a.shots = b->ds + (size_t)table * b->n;
   a.prefix = b->dprefix + (size_t)part * (b->n + 1);
   a.n_shots_p = b->dn;
   a.k_off = k_off;
   return a;
 }
 
+// kind: 0 velocity, 1 stress (in-place two-pass), 2 fused, 3 band
 template <int M, int CPT>
-static void launch_stencil_mc(const fomo_batch* b, bool stress, int flags, int k_off) {
-  dim3 grid(blocks_for(b, PART_STENCIL)), block(64 * WPB);
-  BatchArgs a = args_for(b, PART_STENCIL, k_off);
-  if (stress) hipLaunchKernelGGL((k_stress<M, CPT, WPB>), grid, block, 0, b->ctx->stream, a, flags);
-  else hipLaunchKernelGGL((k_velocity<M, CPT, WPB>), grid, block, 0, b->ctx->stream, a, flags);
+static void launch_stencil_mc(const fomo_batch* b, int table, int kind, int flags, int k_off) {
+  dim3 block(64 * WPB);
+  hipStream_t st = b->ctx->stream;
+  switch (kind) {
+    case 0:
+      hipLaunchKernelGGL((k_velocity<M, CPT, WPB>), dim3(blocks_for(b, PART_STENCIL)), block, 0, st,
+                         args_for(b, table, PART_STENCIL, k_off), flags);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_stress<M, CPT, WPB>), dim3(blocks_for(b, PART_STENCIL)), block, 0, st,
+                         args_for(b, table, PART_STENCIL, k_off), flags);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_fused<M, CPT, WPB>), dim3(blocks_for(b, PART_FUSED)), block, 0, st,
+                         args_for(b, table, PART_FUSED, k_off), flags);
+      break;
+    default:
+      hipLaunchKernelGGL((k_band<M, WPB>), dim3(blocks_for(b, PART_BAND)), block, 0, st,
+                         args_for(b, table, PART_BAND, k_off), flags);
+      break;
+  }
 }
 
 template <int M>
-static void launch_stencil_m(const fomo_batch* b, bool stress, int flags, int k_off) {
+static void launch_stencil_m(const fomo_batch* b, int table, int kind, int flags, int k_off) {
   switch (b->cpt) {
-    case 1: launch_stencil_mc<M, 1>(b, stress, flags, k_off); break;
-    case 2: launch_stencil_mc<M, 2>(b, stress, flags, k_off); break;
-    default: launch_stencil_mc<M, 4>(b, stress, flags, k_off); break;
+    case 1: launch_stencil_mc<M, 1>(b, table, kind, flags, k_off); break;
+    case 2: launch_stencil_mc<M, 2>(b, table, kind, flags, k_off); break;
+    default: launch_stencil_mc<M, 4>(b, table, kind, flags, k_off); break;
   }
 }
 
-static void launch_stencil(const fomo_batch* b, bool stress, int flags, int k_off) {
+static void launch_stencil(const fomo_batch* b, int table, int kind, int flags, int k_off) {
   switch (b->M) {
-    case 1: launch_stencil_m<1>(b, stress, flags, k_off); break;
-    case 2: launch_stencil_m<2>(b, stress, flags, k_off); break;
-    case 3: launch_stencil_m<3>(b, stress, flags, k_off); break;
-    case 4: launch_stencil_m<4>(b, stress, flags, k_off); break;
-    case 5: launch_stencil_m<5>(b, stress, flags, k_off); break;
-    case 6: launch_stencil_m<6>(b, stress, flags, k_off); break;
-    case 7: launch_stencil_m<7>(b, stress, flags, k_off); break;
-    default: launch_stencil_m<8>(b, stress, flags, k_off); break;
+    case 1: launch_stencil_m<1>(b, table, kind, flags, k_off); break;
+    case 2: launch_stencil_m<2>(b, table, kind, flags, k_off); break;
+    case 3: launch_stencil_m<3>(b, table, kind, flags, k_off); break;
+    case 4: launch_stencil_m<4>(b, table, kind, flags, k_off); break;
+    case 5: launch_stencil_m<5>(b, table, kind, flags, k_off); break;
+    case 6: launch_stencil_m<6>(b, table, kind, flags, k_off); break;
+    case 7: launch_stencil_m<7>(b, table, kind, flags, k_off); break;
+    default: launch_stencil_m<8>(b, table, kind, flags, k_off); break;
   }
 }
 
-static void launch_habc(const fomo_batch* b, bool stress, int k_off) {
+static void launch_habc(const fomo_batch* b, int table, bool stress, int k_off) {
   const int part = stress ? PART_HABC_S : PART_HABC_V;
   const int items = b->hprefix[(size_t)part * (b->n + 1) + b->n];
   hipLaunchKernelGGL(k_habc, dim3(std::max(1, items)), dim3(64), 0, b->ctx->stream,
-                     args_for(b, part, k_off), stress ? 1 : 0);
+                     args_for(b, table, part, k_off), stress ? 1 : 0);
 }
 
-static void launch_fsrec(const fomo_batch* b, int mode, int k_off) {
+static void launch_fsrec(const fomo_batch* b, int table, int mode, int k_off) {
   hipLaunchKernelGGL(k_fsrec<WPB>, dim3(blocks_for(b, PART_FSREC)), dim3(64 * WPB), 0,
-                     b->ctx->stream, args_for(b, PART_FSREC, k_off), mode);
+                     b->ctx->stream, args_for(b, table, PART_FSREC, k_off), mode);
 }
 
-// One fused step (time_stepper.jl:39-63): backup and injection ride in the
-// stencil kernels; record follows the free surface.
-static void launch_step(const fomo_batch* b, int k_off) {
-  launch_stencil(b, false, FLAG_BACKUP | FLAG_INJECT, k_off);
-  launch_habc(b, false, k_off);
-  launch_stencil(b, true, FLAG_BACKUP | FLAG_INJECT, k_off);
-  launch_habc(b, true, k_off);
-  launch_fsrec(b, MODE_FS | MODE_RECORD, k_off);
+// One step (time_stepper.jl:39-63) of every shot, starting from state buffer
+// `parity`.  Fused: velocity+stress in one pass (backup = the read buffer),
+// velocity HABC, band stress, stress HABC, free surface + receivers.
+// Two-pass: in place, with backup and injection inside the stencil kernels.
+static void launch_step(const fomo_batch* b, int parity, int k_off) {
+  if (b->step_mode == 1) {
+    const int t = TBL_FUSED0 + parity;
+    launch_stencil(b, t, 2, FLAG_INJECT, k_off);
+    launch_habc(b, t, false, k_off);
+    launch_stencil(b, t, 3, FLAG_INJECT, k_off);
+    launch_habc(b, t, true, k_off);
+    launch_fsrec(b, t, MODE_FS | MODE_RECORD, k_off);
+  } else {
+    const int t = TBL_IN0 + parity;
+    launch_stencil(b, t, 0, FLAG_BACKUP | FLAG_INJECT, k_off);
+    launch_habc(b, t, false, k_off);
+    launch_stencil(b, t, 1, FLAG_BACKUP | FLAG_INJECT, k_off);
+    launch_habc(b, t, true, k_off);
+    launch_fsrec(b, t, MODE_FS | MODE_RECORD, k_off);
+  }
 }
 
 static void launch_set_step(const fomo_batch* b, int k) {
@@ -745,26 +847,57 @@ static int check_launch(fomo_ctx* c, const char* what) {
   return FOMO_OK;
 }
 
+// After fused steps the strip backup lives in the previous state buffer; copy
+// it into old[] (k_backup from that buffer) before anything reads old[].
+static int refresh_backup(fomo_wavefield* w) {
+  if (!w->backup_is_prev) return FOMO_OK;
+  fomo_batch* b = nullptr;
+  int rc = single_batch(w->m, w, nullptr, &b);
+  if (rc) return rc;
+  HIP_TRY(w->m->ctx, hipSetDevice(w->m->ctx->device));
+  launch_set_step(b, 1);
+  hipLaunchKernelGGL(k_backup<WPB>, dim3(blocks_for(b, PART_BACKUP)), dim3(64 * WPB), 0,
+                     w->m->ctx->stream, args_for(b, TBL_IN0 + (1 - w->cur), PART_BACKUP, 0));
+  w->backup_is_prev = false;
+  return check_launch(w->m->ctx, "refresh_backup");
+}
+
 static int run_batch(fomo_batch* b, int k_first, int n_steps) {
   fomo_ctx* c = b->ctx;
   if (n_steps < 0) return fail(c, FOMO_ERR_ARG, "n_steps < 0");
   if (n_steps == 0) return FOMO_OK;
+  const int parity = b->ws[0]->cur;
+  for (fomo_wavefield* w : b->ws)
+    if (w->cur != parity)
+      return fail(c, FOMO_ERR_STATE, "batch wavefields are at different buffer parities");
+  if (b->step_mode == 0)
+    for (fomo_wavefield* w : b->ws) {
+      int rc = refresh_backup(w);
+      if (rc) return rc;
+    }
   HIP_TRY(c, hipSetDevice(c->device));
   launch_set_step(b, k_first);
-  const int G = b->graph_steps;
+  const int G = b->graph_steps;  // even: a chunk ends on the parity it started from
   const int chunks = n_steps / G, rem = n_steps % G;
-  if (chunks > 0 && !b->gexec) {
+  const int flip = b->step_mode == 1 ? 1 : 0;
+  if (chunks > 0 && !b->gexec[parity]) {
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int g = 0; g < G; ++g) launch_step(b, g);
+    for (int g = 0; g < G; ++g) launch_step(b, (parity + flip * g) & 1, g);
     launch_advance(b, G);
     hipGraph_t graph = nullptr;
     HIP_TRY(c, hipStreamEndCapture(c->stream, &graph));
-    b->graph = graph;
-    HIP_TRY(c, hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0));
+    b->graph[parity] = graph;
+    HIP_TRY(c, hipGraphInstantiate(&b->gexec[parity], graph, nullptr, nullptr, 0));
   }
-  for (int i = 0; i < chunks; ++i) HIP_TRY(c, hipGraphLaunch(b->gexec, c->stream));
-  for (int g = 0; g < rem; ++g) launch_step(b, g);
+  for (int i = 0; i < chunks; ++i) HIP_TRY(c, hipGraphLaunch(b->gexec[parity], c->stream));
+  for (int g = 0; g < rem; ++g) launch_step(b, (parity + flip * g) & 1, g);
   if (rem) launch_advance(b, rem);
+  if (flip) {
+    for (fomo_wavefield* w : b->ws) {
+      w->cur = (parity + n_steps) & 1;
+      w->backup_is_prev = true;
+    }
+  }
   return check_launch(c, "time loop");
 }
 
@@ -782,35 +915,47 @@ extern "C" int32_t fomo_time_step(fomo_model* m, fomo_wavefield* w, fomo_shot* s
   fomo_batch* b = nullptr;
   int rc = single_batch(m, w, s, &b);
   if (rc) return rc;
+  if (b->step_mode == 0) {
+    rc = refresh_backup(w);
+    if (rc) return rc;
+  }
   HIP_TRY(m->ctx, hipSetDevice(m->ctx->device));
   launch_set_step(b, k);
-  launch_step(b, 0);
+  launch_step(b, w->cur, 0);
+  if (b->step_mode == 1) {
+    w->cur ^= 1;
+    w->backup_is_prev = true;
+  }
   return check_launch(m->ctx, "time_step");
 }
 
-// Per-stage entry points: the unfused kernels, one stage each.
+// Per-stage entry points: the unfused in-place kernels, one stage each, on
+// the current buffer with old[] as the strip backup.
 static int stage(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int k, int which) {
   if (!m || !w) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  int rc = refresh_backup(w);
+  if (rc) return rc;
   fomo_batch* b = nullptr;
-  int rc = single_batch(m, w, s, &b);
+  rc = single_batch(m, w, s, &b);
   if (rc) return rc;
   fomo_ctx* c = m->ctx;
+  const int t = TBL_IN0 + w->cur;
   HIP_TRY(c, hipSetDevice(c->device));
   launch_set_step(b, s ? k : 1);
   switch (which) {
     case 0:
       hipLaunchKernelGGL(k_backup<WPB>, dim3(blocks_for(b, PART_BACKUP)), dim3(64 * WPB), 0,
-                         c->stream, args_for(b, PART_BACKUP, 0));
+                         c->stream, args_for(b, t, PART_BACKUP, 0));
       break;
     case 1:
-      hipLaunchKernelGGL(k_inject, dim3(1), dim3(64), 0, c->stream, args_for(b, PART_SHOT, 0));
+      hipLaunchKernelGGL(k_inject, dim3(1), dim3(64), 0, c->stream, args_for(b, t, PART_SHOT, 0));
       break;
-    case 2: launch_stencil(b, false, 0, 0); break;
-    case 3: launch_habc(b, false, 0); break;
-    case 4: launch_stencil(b, true, 0, 0); break;
-    case 5: launch_habc(b, true, 0); break;
-    case 6: launch_fsrec(b, MODE_FS, 0); break;
-    case 7: launch_fsrec(b, MODE_RECORD, 0); break;
+    case 2: launch_stencil(b, t, 0, 0, 0); break;
+    case 3: launch_habc(b, t, false, 0); break;
+    case 4: launch_stencil(b, t, 1, 0, 0); break;
+    case 5: launch_habc(b, t, true, 0); break;
+    case 6: launch_fsrec(b, t, MODE_FS, 0); break;
+    case 7: launch_fsrec(b, t, MODE_RECORD, 0); break;
   }
   return check_launch(c, "stage");
 }
@@ -830,22 +975,32 @@ extern "C" int32_t fomo_record_receivers(fomo_model* m, fomo_wavefield* w, fomo_
   return stage(m, w, s, k, 7);
 }
 
+// Timing hook: `reps` launches of one kernel with the step's configuration.
+// Stages 0-4 as declared in the header; 5 = fused velocity+stress kernel,
+// 6 = band stress kernel, 7 = HABC of velocity, 8 = HABC of stress, the last
+// three in the fused step's buffer roles (they read one state buffer and
+// write the other, so repeated launches leave the state unchanged).
 extern "C" int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t which,
                                      int32_t k, int32_t reps) {
   if (!m || !w || !s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
-  if (which < 0 || which > 4 || reps < 0) return fail(m->ctx, FOMO_ERR_ARG, "bad stage or reps");
+  if (which < 0 || which > 8 || reps < 0) return fail(m->ctx, FOMO_ERR_ARG, "bad stage or reps");
   fomo_batch* b = nullptr;
   int rc = single_batch(m, w, s, &b);
   if (rc) return rc;
   HIP_TRY(m->ctx, hipSetDevice(m->ctx->device));
   launch_set_step(b, k);
+  const int ti = TBL_IN0 + w->cur, tf = TBL_FUSED0 + w->cur;
   for (int r = 0; r < reps; ++r) {
     switch (which) {
-      case 0: launch_stencil(b, false, FLAG_BACKUP | FLAG_INJECT, 0); break;
-      case 1: launch_stencil(b, true, FLAG_BACKUP | FLAG_INJECT, 0); break;
-      case 2: launch_habc(b, false, 0); break;
-      case 3: launch_habc(b, true, 0); break;
-      case 4: launch_fsrec(b, MODE_FS | MODE_RECORD, 0); break;
+      case 0: launch_stencil(b, ti, 0, FLAG_BACKUP | FLAG_INJECT, 0); break;
+      case 1: launch_stencil(b, ti, 1, FLAG_BACKUP | FLAG_INJECT, 0); break;
+      case 2: launch_habc(b, ti, false, 0); break;
+      case 3: launch_habc(b, ti, true, 0); break;
+      case 4: launch_fsrec(b, ti, MODE_FS | MODE_RECORD, 0); break;
+      case 5: launch_stencil(b, tf, 2, FLAG_INJECT, 0); break;
+      case 6: launch_stencil(b, tf, 3, FLAG_INJECT, 0); break;
+      case 7: launch_habc(b, tf, false, 0); break;
+      case 8: launch_habc(b, tf, true, 0); break;
     }
   }
   return check_launch(m->ctx, "launch_stage");

@@ -510,6 +510,423 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
 }
 
 // ---------------------------------------------------------------------------
+// Fused step on ping-pong buffers: velocity (velocity.jl:21-46) and stress
+// (stress.jl:18-47) of one time step in a single z-march, reading the previous
+// state A (fo[]) and writing the new state B (f[]).  A is never written, so it
+// also serves as the HABC strip backup (the backup of boundary.jl:45-56 is the
+// state at the start of the step).
+//
+// A wavefront covers 64*CPT lane columns and writes the inner ftw of them
+// (H = M rounded up to CPT lanes on each side are computed but not written):
+// new velocities are needed M columns and M rows beyond the stress cells.
+// Iteration jv computes the new velocity row jv and the new stress row
+// j = jv - M.  Register windows hold old txz (2M rows), old tzz (2M+1), old
+// txx (M+1) and the new vx (2M) / vz (2M+1); LDS carries the rows needed
+// for x-derivatives.  Stress cells in the bands next to the HABC strips are
+// left to k_band (they need HABC-corrected velocities).
+// Arithmetic is that of k_velocity / k_stress exactly.
+// ---------------------------------------------------------------------------
+template <int M, int CPT, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
+  constexpr int HL = ((M + CPT - 1) / CPT) * CPT;  // computed-not-written lane columns per side
+  constexpr int LC = 64 * CPT;
+  constexpr int LW = LC + 2 * M;
+  __shared__ float lds[WPB][4][LW];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  float* s_txx = lds[wid][0];
+  float* s_txz = lds[wid][1];
+  float* s_vx = lds[wid][2];
+  float* s_vz = lds[wid][3];
+  const int nsh = *ba.n_shots_p;
+  const int total = ba.prefix[nsh];
+  for (int item = uniform(blockIdx.x * WPB + wid); item < total;
+       item = uniform(item + gridDim.x * WPB)) {
+    const int s = uniform(find_shot(ba.prefix, nsh, item));
+    const ShotDev& sd = ba.shots[s];
+    const int k = shot_step(sd, ba.k_off);
+    if (k == 0) continue;
+    const int t = item - ba.prefix[s];
+    const int nx = uniform(sd.nx), nz = uniform(sd.nz), P = uniform(sd.pitch);
+    const int fs = uniform(sd.fs), band = uniform(sd.band);
+    const int ftiles_x = uniform(sd.ftiles_x), tz = uniform(sd.tz), ftw = uniform(sd.ftw);
+    const int o0 = (t % ftiles_x) * ftw;      // first written column
+    const int j0 = (t / ftiles_x) * tz;
+    const int j1 = min(j0 + tz, nz);
+    const int ib = o0 - HL + lane * CPT;      // this lane's first column
+    int hx = -1;
+    if (lane < M) hx = lane;
+    else if (lane >= 64 - M) hx = LC + M + (lane - (64 - M));
+    const bool has_halo = hx >= 0;
+    const int hcol = o0 - HL - M + hx;
+
+    const float* avx = uniform_p(sd.fo[F_VX]);
+    const float* avz = uniform_p(sd.fo[F_VZ]);
+    const float* atxx = uniform_p(sd.fo[F_TXX]);
+    const float* atzz = uniform_p(sd.fo[F_TZZ]);
+    const float* atxz = uniform_p(sd.fo[F_TXZ]);
+    float* bvx = uniform_p(sd.f[F_VX]);
+    float* bvz = uniform_p(sd.f[F_VZ]);
+    float* btxx = uniform_p(sd.f[F_TXX]);
+    float* btzz = uniform_p(sd.f[F_TZZ]);
+    float* btxz = uniform_p(sd.f[F_TXZ]);
+    const float* b1p = uniform_p(sd.bvx_x);
+    const float* b2p = uniform_p(sd.bvx_z);
+    const float* b3p = uniform_p(sd.bvz_x);
+    const float* b4p = uniform_p(sd.bvz_z);
+    const bool bsame = (b1p == b2p) && (b3p == b4p);
+    const float* lamp = uniform_p(sd.lam);
+    const float* l2mp = uniform_p(sd.l2m);
+    const float* mup = uniform_p(sd.mu_txz);
+    const int nw = uniform(sd.nw), stype = uniform(sd.stype);
+    const bool src_on = (flags & FLAG_INJECT) && k <= nw;
+    const float wav = src_on ? uniform_f(uniform_p(sd.wavelet)[k - 1]) : 0.f;
+    const bool src_p = src_on && stype == SRC_PRESSURE;
+    const bool src_vz = src_on && stype == SRC_FORCE_Z;
+    const bool src_vx = src_on && stype == SRC_FORCE_X;
+    const int si = uniform(sd.si), sj = uniform(sd.sj);
+    const float src_scale = uniform_f(sd.src_scale);
+    const float dtx = uniform_f(sd.dtx), dtz = uniform_f(sd.dtz);
+    float a[M];
+#pragma unroll
+    for (int l = 0; l < M; ++l) a[l] = uniform_f(sd.a[l]);
+    // ---- end of shot-table reads
+    // pressure source as seen by every reader of old txx / tzz
+    auto srcp = [&](int i, int j) { return (src_p && i == si && j == sj) ? wav : 0.f; };
+
+    float wtxz[CPT][2 * M];      // old txz rows jv-M .. jv+M-1
+    float wtzz[CPT][2 * M + 1];  // old tzz rows jv-M .. jv+M
+    float wtxx[CPT][M + 1];      // old txx rows jv-M .. jv
+    float wvx[CPT][2 * M];       // new vx rows jv-2M+1 .. jv
+    float wvz[CPT][2 * M + 1];   // new vz rows jv-2M .. jv
+    const int jb = j0 - M;       // first velocity row
+#pragma unroll
+    for (int q = 0; q < 2 * M; ++q) {
+      V<CPT> x = ldv<CPT>(row(atxz, jb - M + q, P) + ib);
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) wtxz[c][q] = x.v[c];
+    }
+#pragma unroll
+    for (int q = 0; q < 2 * M + 1; ++q) {
+      V<CPT> z = ldv<CPT>(row(atzz, jb - M + q, P) + ib);
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) wtzz[c][q] = z.v[c] + srcp(ib + c, jb - M + q);
+    }
+#pragma unroll
+    for (int q = 0; q < M + 1; ++q) {
+      V<CPT> x = ldv<CPT>(row(atxx, jb - M + q, P) + ib);
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) wtxx[c][q] = x.v[c] + srcp(ib + c, jb - M + q);
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+      for (int q = 0; q < 2 * M; ++q) wvx[c][q] = 0.f;
+#pragma unroll
+      for (int q = 0; q < 2 * M + 1; ++q) wvz[c][q] = 0.f;
+    }
+    float c_htxx = has_halo ? ldg(row(atxx, jb, P) + hcol) + srcp(hcol, jb) : 0.f;
+    float c_htxz = has_halo ? ldg(row(atxz, jb, P) + hcol) : 0.f;
+    V<CPT> c_vx = ldv<CPT>(row(avx, jb, P) + ib);
+    V<CPT> c_vz = ldv<CPT>(row(avz, jb, P) + ib);
+    V<CPT> c_b1 = ldv<CPT>(row(b1p, jb, P) + ib);
+    V<CPT> c_b3 = ldv<CPT>(row(b3p, jb, P) + ib);
+    V<CPT> c_b2 = c_b1, c_b4 = c_b3;
+    if (!bsame) {
+      c_b2 = ldv<CPT>(row(b2p, jb, P) + ib);
+      c_b4 = ldv<CPT>(row(b4p, jb, P) + ib);
+    }
+    V<CPT> c_lam = ldv<CPT>(row(lamp, jb - M, P) + ib);
+    V<CPT> c_l2m = ldv<CPT>(row(l2mp, jb - M, P) + ib);
+    V<CPT> c_mu = ldv<CPT>(row(mup, jb - M, P) + ib);
+
+    const int jend = j1 + M;
+    for (int jv = jb; jv < jend; ++jv) {
+      // prefetch for jv+1 (velocity inputs) and stress row jv+1-M (coefficients)
+      const int jn = jv + 1;
+      float n_htxx = has_halo ? ldg(row(atxx, jn, P) + hcol) : 0.f;
+      float n_htxz = has_halo ? ldg(row(atxz, jn, P) + hcol) : 0.f;
+      V<CPT> n_txx = ldv<CPT>(row(atxx, jn, P) + ib);
+      V<CPT> n_txz = ldv<CPT>(row(atxz, jv + M, P) + ib);
+      V<CPT> n_tzz = ldv<CPT>(row(atzz, jv + M + 1, P) + ib);
+      V<CPT> n_vx = ldv<CPT>(row(avx, jn, P) + ib);
+      V<CPT> n_vz = ldv<CPT>(row(avz, jn, P) + ib);
+      V<CPT> n_b1 = ldv<CPT>(row(b1p, jn, P) + ib);
+      V<CPT> n_b3 = ldv<CPT>(row(b3p, jn, P) + ib);
+      V<CPT> n_b2 = n_b1, n_b4 = n_b3;
+      if (!bsame) {
+        n_b2 = ldv<CPT>(row(b2p, jn, P) + ib);
+        n_b4 = ldv<CPT>(row(b4p, jn, P) + ib);
+      }
+      V<CPT> n_lam = ldv<CPT>(row(lamp, jn - M, P) + ib);
+      V<CPT> n_l2m = ldv<CPT>(row(l2mp, jn - M, P) + ib);
+      V<CPT> n_mu = ldv<CPT>(row(mup, jn - M, P) + ib);
+
+      // ---- new velocity, row jv, all lane columns
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+        s_txx[M + lane * CPT + c] = wtxx[c][M];
+        s_txz[M + lane * CPT + c] = wtxz[c][M];
+      }
+      if (has_halo) {
+        s_txx[hx] = c_htxx;
+        s_txz[hx] = c_htxz;
+      }
+      wave_sync();
+      V<CPT> nvx, nvz;
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+        const int x = M + lane * CPT + c;
+        float d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f;
+#pragma unroll
+        for (int l = 1; l <= M; ++l) {
+          d1 = fmaf_(a[l - 1], s_txx[x + l - 1] - s_txx[x - l], d1);
+          d2 = fmaf_(a[l - 1], wtxz[c][M + l - 1] - wtxz[c][M - l], d2);
+          d3 = fmaf_(a[l - 1], s_txz[x + l] - s_txz[x - l + 1], d3);
+          d4 = fmaf_(a[l - 1], wtzz[c][M + l] - wtzz[c][M - l + 1], d4);
+        }
+        const int i = ib + c;
+        const bool inside = (i >= M) & (i < nx - M) & (jv >= M) & (jv < nz - M);
+        float vxs = c_vx.v[c], vzs = c_vz.v[c];
+        if (i == si && jv == sj) {
+          if (src_vz) vzs = vzs + wav * src_scale;
+          if (src_vx) vxs = vxs + wav * src_scale;
+        }
+        const float ix = fmaf_(c_b2.v[c], d2, c_b1.v[c] * d1);
+        const float iz = fmaf_(c_b4.v[c], d4, c_b3.v[c] * d3);
+        nvx.v[c] = inside ? vxs + ix : vxs;
+        nvz.v[c] = inside ? vzs + iz : vzs;
+      }
+      const bool own_lane = lane * CPT >= HL && lane * CPT < LC - HL;
+      if (jv >= j0 && jv < j1 && own_lane) {
+#pragma unroll
+        for (int c = 0; c < CPT; ++c)
+          if (ib + c < nx) {
+            stg(row(bvx, jv, P) + ib + c, nvx.v[c]);
+            stg(row(bvz, jv, P) + ib + c, nvz.v[c]);
+          }
+      }
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+        for (int q = 0; q < 2 * M - 1; ++q) wvx[c][q] = wvx[c][q + 1];
+        wvx[c][2 * M - 1] = nvx.v[c];
+#pragma unroll
+        for (int q = 0; q < 2 * M; ++q) wvz[c][q] = wvz[c][q + 1];
+        wvz[c][2 * M] = nvz.v[c];
+      }
+
+      // ---- new stress, row j = jv - M
+      const int j = jv - M;
+      if (j >= j0) {
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+          s_vx[lane * CPT + c] = wvx[c][M - 1];
+          s_vz[lane * CPT + c] = wvz[c][M];
+        }
+        wave_sync();
+        const bool band_row = j >= nz - band || (!fs && j < band);
+        V<CPT> oxx, ozz, oxz;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+          const int x = lane * CPT + c;
+          float d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f;
+          if (own_lane) {
+#pragma unroll
+            for (int l = 1; l <= M; ++l) {
+              d1 = fmaf_(a[l - 1], s_vx[x + l] - s_vx[x - l + 1], d1);        // dvxdx
+              d2 = fmaf_(a[l - 1], wvz[c][M + l - 1] - wvz[c][M - l], d2);    // dvzdz
+              d3 = fmaf_(a[l - 1], wvx[c][M + l - 1] - wvx[c][M - l], d3);    // dvxdz
+              d4 = fmaf_(a[l - 1], s_vz[x + l - 1] - s_vz[x - l], d4);        // dvzdx
+            }
+          }
+          const int i = ib + c;
+          const bool inside = (i >= M) & (i < nx - M) & (j >= M) & (j < nz - M);
+          const float xx = wtxx[c][0], zz = wtzz[c][0], xz = wtxz[c][0];
+          const float t1 = d1 * dtx, t2 = d2 * dtz;
+          const float lv = c_lam.v[c], l2 = c_l2m.v[c];
+          const float ixx = fmaf_(lv, t2, l2 * t1);
+          const float izz = fmaf_(l2, t2, lv * t1);
+          const float sxz = fmaf_(d4, dtx, d3 * dtz);
+          const float ixz = c_mu.v[c] * sxz;
+          oxx.v[c] = inside ? xx + ixx : xx;
+          ozz.v[c] = inside ? zz + izz : zz;
+          oxz.v[c] = inside ? xz + ixz : xz;
+        }
+        if (own_lane && !band_row) {
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) {
+            const int i = ib + c;
+            if (i >= band && i < nx - band) {
+              stg(row(btxx, j, P) + i, oxx.v[c]);
+              stg(row(btzz, j, P) + i, ozz.v[c]);
+              stg(row(btxz, j, P) + i, oxz.v[c]);
+            }
+          }
+        }
+      }
+      wave_sync();
+
+      // ---- slide the old-state windows to jv+1
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+        for (int q = 0; q < 2 * M - 1; ++q) wtxz[c][q] = wtxz[c][q + 1];
+        wtxz[c][2 * M - 1] = n_txz.v[c];
+#pragma unroll
+        for (int q = 0; q < 2 * M; ++q) wtzz[c][q] = wtzz[c][q + 1];
+        wtzz[c][2 * M] = n_tzz.v[c] + srcp(ib + c, jv + M + 1);
+#pragma unroll
+        for (int q = 0; q < M; ++q) wtxx[c][q] = wtxx[c][q + 1];
+        wtxx[c][M] = n_txx.v[c] + srcp(ib + c, jn);
+      }
+      c_htxx = n_htxx + srcp(hcol, jn);
+      c_htxz = n_htxz;
+      c_vx = n_vx;
+      c_vz = n_vz;
+      c_b1 = n_b1;
+      c_b2 = n_b2;
+      c_b3 = n_b3;
+      c_b4 = n_b4;
+      c_lam = n_lam;
+      c_l2m = n_l2m;
+      c_mu = n_mu;
+    }
+  }
+}
+
+// Stress in the bands next to the HABC strips, after the velocity HABC:
+// new velocities from B (f[]), old stress from A (fo[]), output to B.  One
+// wavefront per (band, 64-column chunk, BAND_ROWS-row segment); the same
+// arithmetic as k_stress.
+template <int M, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
+  constexpr int LW = 64 + 2 * M;
+  __shared__ float lds[WPB][2][LW];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  float* s_vx = lds[wid][0];
+  float* s_vz = lds[wid][1];
+  const int nsh = *ba.n_shots_p;
+  const int total = ba.prefix[nsh];
+  for (int item = uniform(blockIdx.x * WPB + wid); item < total;
+       item = uniform(item + gridDim.x * WPB)) {
+    const int s = uniform(find_shot(ba.prefix, nsh, item));
+    const ShotDev& sd = ba.shots[s];
+    const int k = shot_step(sd, ba.k_off);
+    if (k == 0) continue;
+    int t = item - ba.prefix[s];
+    const int nx = uniform(sd.nx), nz = uniform(sd.nz), P = uniform(sd.pitch);
+    const int band = uniform(sd.band), nb_side = uniform(sd.nb_side), nb_mid = uniform(sd.nb_mid);
+    // band rectangle [c0, c1) x [r0, r1) of this item
+    int c0, c1, r0, r1;
+    {
+      const int side_chunks = (band + 63) / 64;
+      const int mid_chunks = (nx - 2 * band + 63) / 64;
+      if (t < 2 * nb_side) {
+        const bool right = t >= nb_side;
+        const int u = right ? t - nb_side : t;
+        const int cc = u % side_chunks, rs = u / side_chunks;
+        c0 = (right ? nx - band : 0) + cc * 64;
+        c1 = min(c0 + 64, right ? nx : band);
+        r0 = rs * BAND_ROWS;
+        r1 = min(r0 + BAND_ROWS, nz);
+      } else {
+        t -= 2 * nb_side;
+        const bool top = t >= nb_mid;
+        const int u = top ? t - nb_mid : t;
+        const int cc = u % mid_chunks, rs = u / mid_chunks;
+        c0 = band + cc * 64;
+        c1 = min(c0 + 64, nx - band);
+        r0 = (top ? 0 : nz - band) + rs * BAND_ROWS;
+        r1 = min(r0 + BAND_ROWS, top ? band : nz);
+      }
+    }
+    const int ib = c0 + lane;
+    int hx = -1;
+    if (lane < M) hx = lane;
+    else if (lane >= 64 - M) hx = 64 + M + (lane - (64 - M));
+    const bool has_halo = hx >= 0;
+    const int hcol = c0 - M + hx;
+    const float* vx = uniform_p(sd.f[F_VX]);
+    const float* vz = uniform_p(sd.f[F_VZ]);
+    const float* atxx = uniform_p(sd.fo[F_TXX]);
+    const float* atzz = uniform_p(sd.fo[F_TZZ]);
+    const float* atxz = uniform_p(sd.fo[F_TXZ]);
+    float* btxx = uniform_p(sd.f[F_TXX]);
+    float* btzz = uniform_p(sd.f[F_TZZ]);
+    float* btxz = uniform_p(sd.f[F_TXZ]);
+    const float* lamp = uniform_p(sd.lam);
+    const float* l2mp = uniform_p(sd.l2m);
+    const float* mup = uniform_p(sd.mu_txz);
+    const int nw = uniform(sd.nw), stype = uniform(sd.stype);
+    const bool src_on = (flags & FLAG_INJECT) && k <= nw && stype == SRC_PRESSURE;
+    const float wav = src_on ? uniform_f(uniform_p(sd.wavelet)[k - 1]) : 0.f;
+    const int si = uniform(sd.si), sj = uniform(sd.sj);
+    const float dtx = uniform_f(sd.dtx), dtz = uniform_f(sd.dtz);
+    float a[M];
+#pragma unroll
+    for (int l = 0; l < M; ++l) a[l] = uniform_f(sd.a[l]);
+    // ---- end of shot-table reads
+
+    float wvx[2 * M];  // vx rows j-M+1 .. j+M
+    float wvz[2 * M];  // vz rows j-M .. j+M-1
+#pragma unroll
+    for (int q = 0; q < 2 * M; ++q) {
+      wvx[q] = ldg(row(vx, r0 - M + 1 + q, P) + ib);
+      wvz[q] = ldg(row(vz, r0 - M + q, P) + ib);
+    }
+    for (int j = r0; j < r1; ++j) {
+      const float hvx = has_halo ? ldg(row(vx, j, P) + hcol) : 0.f;
+      const float hvz = has_halo ? ldg(row(vz, j, P) + hcol) : 0.f;
+      const float lv = ldg(row(lamp, j, P) + ib), l2 = ldg(row(l2mp, j, P) + ib);
+      const float mu = ldg(row(mup, j, P) + ib);
+      float xx = ldg(row(atxx, j, P) + ib), zz = ldg(row(atzz, j, P) + ib);
+      const float xz = ldg(row(atxz, j, P) + ib);
+      s_vx[M + lane] = wvx[M - 1];
+      s_vz[M + lane] = wvz[M];
+      if (has_halo) {
+        s_vx[hx] = hvx;
+        s_vz[hx] = hvz;
+      }
+      wave_sync();
+      float d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f;
+      const int x = M + lane;
+#pragma unroll
+      for (int l = 1; l <= M; ++l) {
+        d1 = fmaf_(a[l - 1], s_vx[x + l] - s_vx[x - l + 1], d1);
+        d2 = fmaf_(a[l - 1], wvz[M + l - 1] - wvz[M - l], d2);
+        d3 = fmaf_(a[l - 1], wvx[M - 1 + l] - wvx[M - l], d3);
+        d4 = fmaf_(a[l - 1], s_vz[x + l - 1] - s_vz[x - l], d4);
+      }
+      const int i = ib;
+      const bool inside = (i >= M) & (i < nx - M) & (j >= M) & (j < nz - M);
+      if (src_on && i == si && j == sj) {
+        xx = xx + wav;
+        zz = zz + wav;
+      }
+      const float t1 = d1 * dtx, t2 = d2 * dtz;
+      const float ixx = fmaf_(lv, t2, l2 * t1);
+      const float izz = fmaf_(l2, t2, lv * t1);
+      const float sxz = fmaf_(d4, dtx, d3 * dtz);
+      if (i < c1) {
+        stg(row(btxx, j, P) + i, inside ? xx + ixx : xx);
+        stg(row(btzz, j, P) + i, inside ? zz + izz : zz);
+        stg(row(btxz, j, P) + i, inside ? xz + mu * sxz : xz);
+      }
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < 2 * M - 1; ++q) {
+        wvx[q] = wvx[q + 1];
+        wvz[q] = wvz[q + 1];
+      }
+      wvx[2 * M - 1] = ldg(row(vx, j + M + 1, P) + ib);
+      wvz[2 * M - 1] = ldg(row(vz, j + M, P) + ib);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // HABC (kernels/boundary.jl:68-132) in the literal program order of the CPU
 // loops: left, right, bottom, [top], left-bottom, right-bottom, [left-top,
 // right-top]; j outer, i inner; every read sees the current array.  That

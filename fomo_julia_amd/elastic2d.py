@@ -87,8 +87,13 @@ class MI355XBackend:
 
     def set_tiling(self, cols_per_lane: int = 2, rows_per_tile: int = 0,
                    steps_per_graph: int = 16) -> None:
+        """Stencil tile shape (rows_per_tile 0 = automatic) and steps per captured graph (even)."""
         self.check(self.lib.fomo_ctx_set_tiling(self.ctx, cols_per_lane, rows_per_tile,
                                                 steps_per_graph), "set_tiling")
+
+    def set_step_mode(self, mode: int = 1) -> None:
+        """1 = fused single-pass step on ping-pong buffers (default), 0 = two-pass in place."""
+        self.check(self.lib.fomo_ctx_set_step_mode(self.ctx, int(mode)), "set_step_mode")
 
     def stream_handle(self) -> int:
         return int(self.lib.fomo_ctx_stream(self.ctx) or 0)

@@ -165,15 +165,27 @@ int32_t fomo_batch_destroy(fomo_batch* b);
 int32_t fomo_batch_run(fomo_batch* b, int32_t k_first, int32_t n_steps);
 
 /* ---- tuning ------------------------------------------------------------- */
+/* Step implementation for runs created after the call on this context:
+ * 1 (default) = fused: velocity and stress in one pass over ping-pong state
+ *   buffers (the read buffer doubles as the strip backup), then the velocity
+ *   HABC, the stress cells beside the HABC strips, the stress HABC, free
+ *   surface and receivers;
+ * 0 = two-pass in place: velocity pass (with strip backup and source), HABC,
+ *   stress pass, HABC, free surface and receivers.
+ * Both give identical results. */
+int32_t fomo_ctx_set_step_mode(fomo_ctx* ctx, int32_t mode);
 /* Stencil tile: 64*cols_per_lane columns (cols_per_lane in {1,2,4}) by
- * rows_per_tile rows per wavefront; steps_per_graph = whole steps per
- * captured graph.  Applies to runs created after the call on this context. */
+ * rows_per_tile rows per wavefront (0 = chosen so that the tiles fill the
+ * GPU's resident wavefronts in whole rounds); steps_per_graph = whole steps
+ * per captured graph (even).  Applies to runs created after the call on
+ * this context. */
 int32_t fomo_ctx_set_tiling(fomo_ctx* ctx, int32_t cols_per_lane, int32_t rows_per_tile,
                             int32_t steps_per_graph);
-/* Queue `reps` launches of one stage kernel for (m, w, s) on the context's
- * stream, for timing: stage 0 = velocity, 1 = stress (fused backup and
- * injection, as in a step), 2 = HABC velocity, 3 = HABC stress,
- * 4 = free surface + record. */
+/* Queue `reps` launches of one kernel for (m, w, s) on the context's stream,
+ * for timing: 0 = velocity pass, 1 = stress pass (two-pass step, with strip
+ * backup and injection), 2 = HABC velocity, 3 = HABC stress (in place),
+ * 4 = free surface + record, 5 = fused velocity+stress pass, 6 = band
+ * stress, 7 = HABC velocity, 8 = HABC stress (fused-step buffer roles). */
 int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t stage,
                           int32_t k, int32_t reps);
 

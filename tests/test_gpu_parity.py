@@ -25,8 +25,9 @@ def b():
     return bk
 
 
-def gpu_shot(b, c, cpt=2, tz=0, gsteps=16):
+def gpu_shot(b, c, cpt=2, tz=0, gsteps=16, mode=1):
     b.set_tiling(cpt, tz, gsteps)
+    b.set_step_mode(mode)
     W = E.Wavefield(c.M.nx, c.M.nz, b)
     res = E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p)
     return res, W
@@ -43,34 +44,40 @@ def assert_same_state(W, r, what=""):
                                  f"got {got[i, j]!r} want {want[i, j]!r}")
 
 
+@pytest.mark.parametrize("mode", [1, 0])
 @pytest.mark.parametrize("fs", [True, False])
 @pytest.mark.parametrize("src_type", ["pressure", "force_z", "force_x"])
-def test_full_shot_bit_exact(b, fs, src_type):
+def test_full_shot_bit_exact(b, fs, src_type, mode):
     c = Case(fs=fs, src_type=src_type, nt=150, water=fs)
-    print(f"case fs={fs} src={src_type} grid={c.M.nx}x{c.M.nz}", flush=True)
-    res, W = gpu_shot(b, c)
+    print(f"case fs={fs} src={src_type} mode={mode} grid={c.M.nx}x{c.M.nz}", flush=True)
+    res, W = gpu_shot(b, c, mode=mode)
+    b.set_step_mode(1)
     r = c.oracle()
     r.run(1, c.nt)
     assert np.array_equal(res.gather, r.gather_nt_by_nrec())
     assert_same_state(W, r)
 
 
+@pytest.mark.parametrize("mode", [1, 0])
 @pytest.mark.parametrize("fd_order", [2, 4, 6, 10, 12, 16])
-def test_orders_bit_exact(b, fd_order):
+def test_orders_bit_exact(b, fd_order, mode):
     c = Case(fd_order=fd_order, nt=80, nbc=10)
-    print(f"case order={fd_order}", flush=True)
-    res, W = gpu_shot(b, c)
+    print(f"case order={fd_order} mode={mode}", flush=True)
+    res, W = gpu_shot(b, c, mode=mode)
+    b.set_step_mode(1)
     r = c.oracle()
     r.run(1, c.nt)
     assert np.array_equal(res.gather, r.gather_nt_by_nrec())
     assert_same_state(W, r)
 
 
-@pytest.mark.parametrize("cpt,tz,gsteps", [(1, 8, 1), (2, 17, 5), (4, 64, 32), (4, 5, 16)])
-def test_tilings_bit_exact(b, cpt, tz, gsteps):
-    c = Case(nx_in=150, nz_in=90, nbc=20, nt=70)
-    print(f"case cpt={cpt} tz={tz} gsteps={gsteps}", flush=True)
-    res, W = gpu_shot(b, c, cpt, tz, gsteps)
+@pytest.mark.parametrize("mode", [1, 0])
+@pytest.mark.parametrize("cpt,tz,gsteps", [(1, 8, 2), (2, 17, 6), (4, 64, 32), (4, 5, 16), (1, 0, 4)])
+def test_tilings_bit_exact(b, cpt, tz, gsteps, mode):
+    c = Case(nx_in=150, nz_in=90, nbc=20, nt=71)
+    print(f"case cpt={cpt} tz={tz} gsteps={gsteps} mode={mode}", flush=True)
+    res, W = gpu_shot(b, c, cpt, tz, gsteps, mode)
+    b.set_step_mode(1)
     r = c.oracle()
     r.run(1, c.nt)
     assert np.array_equal(res.gather, r.gather_nt_by_nrec())
@@ -121,7 +128,9 @@ def test_inject_and_record_stages(b):
     assert np.array_equal(c.rec.data, r.gather_nt_by_nrec())
 
 
-def test_time_step_api_matches_run_loop(b):
+@pytest.mark.parametrize("mode", [1, 0])
+def test_time_step_api_matches_run_loop(b, mode):
+    b.set_step_mode(mode)
     c = Case(nt=40)
     W = E.Wavefield(c.M.nx, c.M.nz, b)
     E.bind(b, W, c.M, c.H, c.a, c.p)
@@ -131,6 +140,14 @@ def test_time_step_api_matches_run_loop(b):
     r = c.oracle()
     r.run(1, 40)
     assert_same_state(W, r, "time_step")
+    # per-stage calls continue from the fused state (strip backups included)
+    E.backup_boundary(b, W)
+    E.update_velocity(b, W)
+    E.apply_habc_velocity(b, W)
+    b.synchronize()
+    r.run(41, 1, O.STAGES["backup"] | O.STAGES["velocity"] | O.STAGES["habc_v"])
+    assert_same_state(W, r, "stages after time_step")
+    b.set_step_mode(1)
 
 
 def test_batched_uneven_grids_bit_exact(b):
