@@ -10,10 +10,13 @@ stress, HABC, free surface, receivers); the default K = 11520 steps is the
 full shot.  W warm-up steps run first (graph capture, clocks), then the
 wavefield and gather are reset and exactly K steps are timed from step 1.
 
-Prints one JSON line: Mcell-updates/s over the padded grid, the stress
-kernel's HBM roofline (algorithmic 44 B per cell-update over its mean
-duration measured with HIP events on the library's stream), and the CPU
-oracle timed on a bounded sample of the same workload.
+The step is the fused single-pass form (one velocity+stress kernel over
+ping-pong buffers, then velocity HABC, band stress, stress HABC, free
+surface + receivers).  Prints one JSON line: Mcell-updates/s over the padded
+grid, the fused kernel's HBM roofline (algorithmic 60 B per cell-update over
+its mean duration measured with HIP events on the library's stream; HBM
+traffic per launch from the committed rocprofv3 counter summary), and the
+CPU oracle timed on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -29,8 +32,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_VELOCITY = 36  # read txx tzz txz, dt/rho_vx, dt/rho_vz; read+write vx vz (FP32)
-BYTES_STRESS = 44    # read vx vz lam lam+2mu mu_txz; read+write txx tzz txz (FP32)
+# Algorithmic bytes per padded cell-update (FP32, dx == dz so dt/dx/rho and
+# dt/dz/rho are one array each):
+BYTES_FUSED = 60     # read 5 old fields + dt/rho_vx, dt/rho_vz, lam, lam+2mu, mu_txz; write 5 fields
+BYTES_VELOCITY = 36  # two-pass velocity: read txx tzz txz, 2 buoyancies; read+write vx vz
+BYTES_STRESS = 44    # two-pass stress: read vx vz lam lam+2mu mu_txz; read+write txx tzz txz
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fused_seam_traffic.txt")
 METRIC = "Mcell-updates/s & %HBM roofline, 1 GPU; shots/hr + p99 latency (served)"
 
 
@@ -84,6 +91,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=50)
     ap.add_argument("--tiling", default="", help="cols_per_lane,rows_per_tile,steps_per_graph")
+    ap.add_argument("--two-pass", action="store_true", help="time the two-pass in-place step instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -104,6 +112,8 @@ def main():
     if args.tiling:
         cpt, tz, gs = (int(v) for v in args.tiling.split(","))
         b.set_tiling(cpt, tz, gs)
+    fused = not args.two_pass
+    b.set_step_mode(1 if fused else 0)
     W = E.Wavefield(M.nx, M.nz, b)
     dm = E.bind(b, W, M, H, case["a"], p)
     ds = E._shot_for(dm, src, rec, p.nt)
@@ -147,14 +157,26 @@ def main():
         return e0.elapsed_time(e1) / reps
 
     reps = args.stage_reps
-    st_ms = stage_ms(1, reps)
-    vel_ms = stage_ms(0, reps)
-    habc_v_ms = stage_ms(2, reps)
-    habc_s_ms = stage_ms(3, reps)
-    fsrec_ms = stage_ms(4, reps)
-    stress_gbs = BYTES_STRESS * cells / (st_ms / 1e3) / 1e9
-    vel_gbs = BYTES_VELOCITY * cells / (vel_ms / 1e3) / 1e9
-    step_gbs = (BYTES_VELOCITY + BYTES_STRESS) * cells / (ms_per_step / 1e3) / 1e9
+    if fused:
+        main_ms = stage_ms(5, reps)
+        kernels = {"fused": main_ms, "band_stress": stage_ms(6, reps),
+                   "habc_velocity": stage_ms(7, reps), "habc_stress": stage_ms(8, reps),
+                   "fs_record": stage_ms(4, reps)}
+        main_bytes, main_name = BYTES_FUSED, "k_fused (60 B/cell-update x padded cells per launch)"
+    else:
+        main_ms = stage_ms(1, reps)
+        kernels = {"velocity": stage_ms(0, reps), "stress": main_ms,
+                   "habc_velocity": stage_ms(2, reps), "habc_stress": stage_ms(3, reps),
+                   "fs_record": stage_ms(4, reps)}
+        main_bytes, main_name = BYTES_STRESS, "k_stress (44 B/cell-update x padded cells per launch)"
+    main_gbs = main_bytes * cells / (main_ms / 1e3) / 1e9
+    step_bytes = BYTES_FUSED if fused else BYTES_VELOCITY + BYTES_STRESS
+    step_gbs = step_bytes * cells / (ms_per_step / 1e3) / 1e9
+    traffic = None
+    if fused and os.path.exists(TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = int(line.split(":")[1])
 
     cpu = None
     if not args.no_cpu:
@@ -191,15 +213,13 @@ def main():
                    "shot_seconds": round(gpu_ms / 1e3 * 11520 / max(args.steps, 1), 3),
                    "wall_seconds_timed": round(wall, 3), "setup_seconds": round(t_setup, 2),
                    "gather_finite": finite},
-        "roofline": {"bound": "hbm", "achieved": round(stress_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(stress_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_stress (44 B/cell-update x padded cells per launch)",
-                     "kernel_ms": round(st_ms, 5)},
-        "kernels_ms": {"velocity": round(vel_ms, 5), "stress": round(st_ms, 5),
-                       "habc_velocity": round(habc_v_ms, 5), "habc_stress": round(habc_s_ms, 5),
-                       "fs_record": round(fsrec_ms, 5)},
-        "velocity_gbs": round(vel_gbs, 1),
-        "step_gbs_80B": round(step_gbs, 1),
+        "roofline": {"bound": "hbm", "achieved": round(main_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(main_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": main_name, "kernel_ms": round(main_ms, 5),
+                     "algorithmic_bytes_per_launch": main_bytes * cells},
+        "kernels_ms": {k: round(v, 5) for k, v in kernels.items()},
+        "step_mode": "fused" if fused else "two-pass",
+        "step_gbs": round(step_gbs, 1),
         "step_frac_hbm": round(step_gbs / HBM_PEAK_GBS, 4),
         "cpu_baseline": cpu,
         "reference_published": {"cpu_wall_min": 35, "rtx3060_wall_min_upper": 3,

@@ -43,7 +43,7 @@ enum Part {
 // HABC.  Bands (0-based, half-open): left columns [0, b), right columns
 // [nx-b, nx), bottom rows [nz-b, nz), top rows [0, b) without a free surface,
 // with b = nbc + M + 1.
-constexpr int BAND_ROWS = 64;  // rows per band-kernel item
+constexpr int BAND_ROWS = 16;  // rows per band-kernel item
 
 // Everything a kernel needs to know about one shot.  Lives in device memory
 // (a table indexed by shot), so a captured graph stays valid when the host

@@ -30,7 +30,7 @@ struct fomo_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  int cpt = 2;         // columns per lane of the stencil tiles
+  int cpt = 1;         // columns per lane of the stencil tiles
   int tz = 0;          // rows per stencil tile; 0 = sized to fill the GPU in whole rounds
   int graph_steps = 16;
   int cus = 0;         // compute units of the device

@@ -876,13 +876,23 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
       wvx[q] = ldg(row(vx, r0 - M + 1 + q, P) + ib);
       wvz[q] = ldg(row(vz, r0 - M + q, P) + ib);
     }
+    // row r0's pointwise operands; each iteration prefetches the next row's
+    float hvx = has_halo ? ldg(row(vx, r0, P) + hcol) : 0.f;
+    float hvz = has_halo ? ldg(row(vz, r0, P) + hcol) : 0.f;
+    float lv = ldg(row(lamp, r0, P) + ib), l2 = ldg(row(l2mp, r0, P) + ib);
+    float mu = ldg(row(mup, r0, P) + ib);
+    float oxx = ldg(row(atxx, r0, P) + ib), ozz = ldg(row(atzz, r0, P) + ib);
+    float oxz = ldg(row(atxz, r0, P) + ib);
     for (int j = r0; j < r1; ++j) {
-      const float hvx = has_halo ? ldg(row(vx, j, P) + hcol) : 0.f;
-      const float hvz = has_halo ? ldg(row(vz, j, P) + hcol) : 0.f;
-      const float lv = ldg(row(lamp, j, P) + ib), l2 = ldg(row(l2mp, j, P) + ib);
-      const float mu = ldg(row(mup, j, P) + ib);
-      float xx = ldg(row(atxx, j, P) + ib), zz = ldg(row(atzz, j, P) + ib);
-      const float xz = ldg(row(atxz, j, P) + ib);
+      const int jn = j + 1;
+      const float n_hvx = has_halo ? ldg(row(vx, jn, P) + hcol) : 0.f;
+      const float n_hvz = has_halo ? ldg(row(vz, jn, P) + hcol) : 0.f;
+      const float n_lv = ldg(row(lamp, jn, P) + ib), n_l2 = ldg(row(l2mp, jn, P) + ib);
+      const float n_mu = ldg(row(mup, jn, P) + ib);
+      const float n_oxx = ldg(row(atxx, jn, P) + ib), n_ozz = ldg(row(atzz, jn, P) + ib);
+      const float n_oxz = ldg(row(atxz, jn, P) + ib);
+      const float n_wvx = ldg(row(vx, j + M + 1, P) + ib);
+      const float n_wvz = ldg(row(vz, j + M, P) + ib);
       s_vx[M + lane] = wvx[M - 1];
       s_vz[M + lane] = wvz[M];
       if (has_halo) {
@@ -901,6 +911,7 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
       }
       const int i = ib;
       const bool inside = (i >= M) & (i < nx - M) & (j >= M) & (j < nz - M);
+      float xx = oxx, zz = ozz;
       if (src_on && i == si && j == sj) {
         xx = xx + wav;
         zz = zz + wav;
@@ -912,7 +923,7 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
       if (i < c1) {
         stg(row(btxx, j, P) + i, inside ? xx + ixx : xx);
         stg(row(btzz, j, P) + i, inside ? zz + izz : zz);
-        stg(row(btxz, j, P) + i, inside ? xz + mu * sxz : xz);
+        stg(row(btxz, j, P) + i, inside ? oxz + mu * sxz : oxz);
       }
       wave_sync();
 #pragma unroll
@@ -920,8 +931,16 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
         wvx[q] = wvx[q + 1];
         wvz[q] = wvz[q + 1];
       }
-      wvx[2 * M - 1] = ldg(row(vx, j + M + 1, P) + ib);
-      wvz[2 * M - 1] = ldg(row(vz, j + M, P) + ib);
+      wvx[2 * M - 1] = n_wvx;
+      wvz[2 * M - 1] = n_wvz;
+      hvx = n_hvx;
+      hvz = n_hvz;
+      lv = n_lv;
+      l2 = n_l2;
+      mu = n_mu;
+      oxx = n_oxx;
+      ozz = n_ozz;
+      oxz = n_oxz;
     }
   }
 }

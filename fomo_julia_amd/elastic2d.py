@@ -85,7 +85,7 @@ class MI355XBackend:
     def synchronize(self) -> None:
         self.check(self.lib.fomo_ctx_synchronize(self.ctx), "synchronize")
 
-    def set_tiling(self, cols_per_lane: int = 2, rows_per_tile: int = 0,
+    def set_tiling(self, cols_per_lane: int = 1, rows_per_tile: int = 0,
                    steps_per_graph: int = 16) -> None:
         """Stencil tile shape (rows_per_tile 0 = automatic) and steps per captured graph (even)."""
         self.check(self.lib.fomo_ctx_set_tiling(self.ctx, cols_per_lane, rows_per_tile,

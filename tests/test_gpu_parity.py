@@ -21,11 +21,11 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "shot_1.bin")
 @pytest.fixture(scope="module")
 def b():
     bk = E.backend("mi355x")
-    bk.set_tiling(2, 0, 16)
+    bk.set_tiling(1, 0, 16)
     return bk
 
 
-def gpu_shot(b, c, cpt=2, tz=0, gsteps=16, mode=1):
+def gpu_shot(b, c, cpt=1, tz=0, gsteps=16, mode=1):
     b.set_tiling(cpt, tz, gsteps)
     b.set_step_mode(mode)
     W = E.Wavefield(c.M.nx, c.M.nz, b)
@@ -82,7 +82,7 @@ def test_tilings_bit_exact(b, cpt, tz, gsteps, mode):
     r.run(1, c.nt)
     assert np.array_equal(res.gather, r.gather_nt_by_nrec())
     assert_same_state(W, r)
-    b.set_tiling(2, 0, 16)
+    b.set_tiling(1, 0, 16)
 
 
 STAGES = [("backup", E.backup_boundary), ("velocity", E.update_velocity),
@@ -179,7 +179,7 @@ def test_batched_uneven_grids_bit_exact(b):
         r.run(1, c.nt)
         assert np.array_equal(ds.gather(), r.gather_nt_by_nrec())
         assert_same_state(W, r, "batch")
-    b.set_tiling(2, 0, 16)
+    b.set_tiling(1, 0, 16)
 
 
 def test_run_shots_batched_equals_sequential(b):
