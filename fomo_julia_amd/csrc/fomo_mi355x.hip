@@ -765,9 +765,9 @@ static void launch_stencil_mc(const fomo_batch* b, int table, int kind, int flag
       hipLaunchKernelGGL((k_stress<M, CPT, WPB>), dim3(blocks_for(b, PART_STENCIL)), block, 0, st,
                          args_for(b, table, PART_STENCIL, k_off), flags);
       break;
-    case 2:
-      hipLaunchKernelGGL((k_fused<M, CPT, WPB>), dim3(blocks_for(b, PART_FUSED)), block, 0, st,
-                         args_for(b, table, PART_FUSED, k_off), flags);
+    case 2:  // grid rounded up to whole XCD rounds (see xcd_block)
+      hipLaunchKernelGGL((k_fused<M, CPT, WPB>), dim3(round_up(blocks_for(b, PART_FUSED), NUM_XCD)),
+                         block, 0, st, args_for(b, table, PART_FUSED, k_off), flags);
       break;
     default:
       hipLaunchKernelGGL((k_band<M, WPB>), dim3(blocks_for(b, PART_BAND)), block, 0, st,

@@ -40,6 +40,17 @@ __device__ __forceinline__ float from_upper_lane(float v) {  // lane i <- lane i
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Workgroups are dealt to the 8 XCDs round-robin (block b -> XCD b % 8).  For a
+// grid that is a multiple of 8 blocks, this renumbering gives each XCD one
+// contiguous run of logical blocks, so neighbouring tiles (which share halo
+// rows and columns) meet in the same L2.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int g = gridDim.x, b = blockIdx.x;
+  if (g % NUM_XCD) return b;
+  return (b % NUM_XCD) * (g / NUM_XCD) + b / NUM_XCD;
+}
 __device__ __forceinline__ float uniform_f(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
@@ -540,7 +551,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
   float* s_vz = lds[wid][3];
   const int nsh = *ba.n_shots_p;
   const int total = ba.prefix[nsh];
-  for (int item = uniform(blockIdx.x * WPB + wid); item < total;
+  for (int item = uniform(xcd_block() * WPB + wid); item < total;
        item = uniform(item + gridDim.x * WPB)) {
     const int s = uniform(find_shot(ba.prefix, nsh, item));
     const ShotDev& sd = ba.shots[s];
