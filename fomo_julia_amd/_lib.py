@@ -20,6 +20,7 @@ P_i32 = ctypes.POINTER(ctypes.c_int32)
 
 SRC_TYPES = {"pressure": 0, "force_z": 1, "force_x": 2}
 REC_TYPES = {"vz": 0, "vx": 1, "p": 2}
+STORAGE = {"f32": 0, "bf16": 1, "f16": 2, "f16c": 3}
 FIELDS = {"vx": 0, "vz": 1, "txx": 2, "tzz": 3, "txz": 4,
           "vx_old": 5, "vz_old": 6, "txx_old": 7, "tzz_old": 8, "txz_old": 9}
 
@@ -55,6 +56,7 @@ SIGNATURES = {
     "fomo_shot_create": (c_int32, [c_void_p, c_int32, c_int32, c_int32, P_f32, c_int32, c_int32,
                                    P_i32, P_i32, P_i32, c_int32, ctypes.POINTER(c_void_p)]),
     "fomo_shot_destroy": (c_int32, [c_void_p]),
+    "fomo_shot_field_scales": (c_int32, [c_void_p, P_f32]),
     "fomo_shot_reset": (c_int32, [c_void_p]),
     "fomo_shot_gather": (c_int32, [c_void_p, P_f32]),
     "fomo_time_step": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32]),

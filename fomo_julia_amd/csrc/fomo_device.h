@@ -21,6 +21,11 @@ constexpr int MAX_M = 8;        // fd_order <= 16
 constexpr int MAX_NBC = 63;     // one wavefront spans an HABC strip's width
 
 enum SrcType { SRC_PRESSURE = 0, SRC_FORCE_Z = 1, SRC_FORCE_X = 2 };
+// Storage modes of wavefield and medium arrays (arithmetic is always FP32):
+// element kinds ST_F32 / ST_BF16 / ST_F16 for both, or ST_F16C = FP16
+// coefficients with FP32 fields.
+enum StorageKind { ST_F32 = 0, ST_BF16 = 1, ST_F16 = 2, ST_F16C = 3 };
+enum Coef { C_BVX_X = 0, C_BVX_Z, C_BVZ_X, C_BVZ_Z, C_LAM, C_L2M, C_MU };
 enum RecType { REC_VZ = 0, REC_VX = 1, REC_P = 2 };
 enum Field { F_VX = 0, F_VZ = 1, F_TXX = 2, F_TZZ = 3, F_TXZ = 4 };
 enum Ramp { RAMP_VX = 0, RAMP_VZ = 1, RAMP_TAU = 2 };
@@ -58,7 +63,11 @@ struct ShotDev {
   int nb_side, nb_mid;  // band items: per side band, per bottom/top band
   int n_l, n_r, n_b, n_t;  // HABC side/bottom/top edge pieces per field
   float dtx, dtz;
-  float* f[5];          // vx vz txx tzz txz   (pointer to cell (0,0))
+  // Reduced-precision storage: a stored value is x * 2^e (exact), read back as
+  // stored * 2^-e.  fsc / fisc are 2^e / 2^-e of the five fields; csc the
+  // 2^-e of the coefficient arrays (C_*).  All 1 for FP32 storage.
+  float fsc[5], fisc[5], csc[7];
+  float* f[5];          // vx vz txx tzz txz   (pointer to cell (0,0); element type per storage)
   float* fo[5];         // strip backups (*_old)
   const float* bvx_x;   // dtx / rho_vx
   const float* bvx_z;   // dtz / rho_vx

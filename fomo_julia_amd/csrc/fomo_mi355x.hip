@@ -43,8 +43,13 @@ struct fomo_model {
   int M = 0;
   int pitch = 0;
   size_t fstride = 0;  // floats per field block (guards + slack included)
-  float* dcoef = nullptr;
-  const float* coef[7] = {};  // bvx_x bvx_z bvz_x bvz_z lam l2m mu_txz
+  int storage = FOMO_STORAGE_F32;  // storage mode (FOMO_STORAGE_*)
+  int ckind = ST_F32, fkind = ST_F32;  // element kinds of coefficients / fields
+  int es = 4, fes = 4;  // bytes per stored coefficient / field element
+  void* dcoef = nullptr;
+  const float* coef[7] = {};  // bvx_x bvx_z bvz_x bvz_z lam l2m mu_txz (element type per storage)
+  float csc[7] = {1, 1, 1, 1, 1, 1, 1};  // load scales 2^-e of the coefficient arrays
+  float zmin = 0.f, zmax = 0.f;          // P impedance sqrt((lam+2mu) rho) extremes
   float* dramp = nullptr;
   std::vector<float> rho_vx, rho_vz;  // host copies, for force-source scaling
   float a[MAX_M] = {};
@@ -56,9 +61,13 @@ struct fomo_batch;
 
 struct fomo_wavefield {
   fomo_model* m = nullptr;
-  float* dbuf = nullptr;
-  float* buf[2][5] = {};  // state buffers
-  float* old[5] = {};     // strip backups (*_old) of the in-place kernels
+  void* dbuf = nullptr;
+  float* buf[2][5] = {};  // state buffers (element type per storage)
+  float* old[5] = {};     // strip backups (*_old) of the in-place kernels; FP32 only
+  // scales of the stored state (reduced-precision storage): a zeroed state
+  // takes the scales of the first shot run on it
+  float fsc[5] = {1, 1, 1, 1, 1};
+  bool zero = true;
   int cur = 0;            // buffer holding the current state
   // after a fused step the backup is the previous buffer's strip cells; old[]
   // is refreshed from it only when the in-place path or a download needs it
@@ -79,6 +88,7 @@ struct fomo_shot {
   float* dgather = nullptr;
   int* dk = nullptr;
   float src_scale = 0.f;
+  float fsc[5] = {1, 1, 1, 1, 1};  // field store scales 2^e (FP16 storage)
   uint64_t id = 0;  // unique per shot object, keys the cached batches
 };
 
@@ -93,7 +103,9 @@ struct fomo_batch {
   fomo_ctx* ctx = nullptr;
   int n = 0, M = 0, cpt = 2, tz = 32, graph_steps = 16, step_mode = 1;
   int tz_req = 0;  // the context's rows_per_tile when built (0 = auto)
+  int storage = FOMO_STORAGE_F32;
   std::vector<fomo_wavefield*> ws;
+  std::vector<fomo_shot*> ss;
   std::vector<ShotDev> hs;  // [TBL_COUNT][n]
   ShotDev* ds = nullptr;
   int* dn = nullptr;
@@ -127,9 +139,107 @@ static int pitch_for(int nx) { return round_up(nx + 272, 256); }
 static size_t field_stride(int nz, int pitch) {
   return (size_t)(nz + 2 * GUARD_ROWS) * pitch + 2 * SLACK;
 }
-// Pointer to cell (0,0) of field block b inside a buffer of blocks.
-static float* block_origin(float* buf, size_t stride, int b, int pitch) {
-  return buf + (size_t)b * stride + SLACK + (size_t)GUARD_ROWS * pitch;
+// Pointer to cell (0,0) of field block b inside a buffer of blocks of
+// `stride` elements of es bytes.
+static float* block_origin(void* buf, size_t stride, int b, int pitch, int es) {
+  return (float*)((char*)buf + ((size_t)b * stride + SLACK + (size_t)GUARD_ROWS * pitch) * es);
+}
+
+// ------------------------------------------------------------------ storage
+// Host conversions, the same rounding as the kernels (nearest, ties to even).
+static uint16_t to_bf16(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static float from_bf16(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+static uint16_t to_f16(float x) {
+  const _Float16 h = (_Float16)x;
+  uint16_t b;
+  std::memcpy(&b, &h, 2);
+  return b;
+}
+static float from_f16(uint16_t b) {
+  _Float16 h;
+  std::memcpy(&h, &b, 2);
+  return (float)h;
+}
+// x * scale stored as element kind `kind` (ST_*) into dst[0..n)
+static void encode(int kind, const float* x, size_t n, float scale, void* dst) {
+  if (kind == ST_F32) {
+    std::memcpy(dst, x, n * 4);
+    return;
+  }
+  uint16_t* d = (uint16_t*)dst;
+  for (size_t i = 0; i < n; ++i) d[i] = kind == ST_BF16 ? to_bf16(x[i] * scale) : to_f16(x[i] * scale);
+}
+static void decode(int kind, const void* src, size_t n, float scale, float* x) {
+  if (kind == ST_F32) {
+    std::memcpy(x, src, n * 4);
+    return;
+  }
+  const uint16_t* d = (const uint16_t*)src;
+  for (size_t i = 0; i < n; ++i) x[i] = (kind == ST_BF16 ? from_bf16(d[i]) : from_f16(d[i])) * scale;
+}
+// Power-of-two scale placing `bound` in [2^target, 2^(target+1)); 1 for a zero
+// or non-finite bound.
+static float pow2_scale(double bound, int target) {
+  if (!(bound > 0) || !std::isfinite(bound)) return 1.f;
+  const int e = target - (int)std::floor(std::log2(bound));
+  return std::ldexp(1.0f, std::max(-126, std::min(127, e)));
+}
+// Reduced-precision storage supports these half-orders (fd_order 4, 8, 12).
+static bool reduced_m_ok(int M) { return M == 2 || M == 4 || M == 6; }
+
+// MurmurHash3's 32-bit finalizer (fmix32, public domain, Austin Appleby).
+static uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+// Deterministic dither in [0, 1) for cell (i, j) of coefficient array a.
+static double dither_u(uint32_t i, uint32_t j, uint32_t a) {
+  const uint32_t h = fmix32(i ^ fmix32(j ^ fmix32(a + 0x9E3779B9u)));
+  return (double)(h >> 8) * (1.0 / 16777216.0);
+}
+// Dithered rounding of x to the grid of a p-bit significand format with
+// smallest normal exponent emin: the grid point below |x| plus one spacing
+// when the remainder (in spacings) exceeds u.  The error has mean zero over
+// cells, so the medium a wave averages over keeps its coefficients.
+static double dither_round(double x, int p, int emin, double u) {
+  if (x == 0 || !std::isfinite(x)) return x;
+  int e;
+  std::frexp(x, &e);
+  const double spacing = std::ldexp(1.0, std::max(e - 1, emin) - (p - 1));
+  const double a = std::fabs(x) / spacing;
+  const double fl = std::floor(a);
+  return std::copysign((a - fl > u ? fl + 1 : fl) * spacing, x);
+}
+// Coefficient array k (nx x nz) times scale, dither-rounded to `kind`.
+static void encode_coef(int kind, const float* x, int nx, int nz, float scale, int k, uint16_t* d) {
+  for (int j = 0; j < nz; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const size_t o = (size_t)i + (size_t)j * nx;
+      const double u = dither_u((uint32_t)i, (uint32_t)j, (uint32_t)k);
+      const double v = (double)x[o] * scale;
+      const double q = kind == ST_BF16 ? dither_round(v, 8, -126, u) : dither_round(v, 11, -14, u);
+      d[o] = kind == ST_BF16 ? to_bf16((float)q) : to_f16((float)q);  // exact: q is on the grid
+    }
+}
+static int coef_kind(int storage) {
+  return storage == FOMO_STORAGE_BF16 ? ST_BF16 : storage == FOMO_STORAGE_F32 ? ST_F32 : ST_F16;
+}
+static int field_kind(int storage) {
+  return storage == FOMO_STORAGE_BF16 ? ST_BF16 : storage == FOMO_STORAGE_F16 ? ST_F16 : ST_F32;
 }
 
 // ------------------------------------------------------------------ context
@@ -228,7 +338,10 @@ extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, cons
     return fail(c, FOMO_ERR_ARG, "grid too small for nbc and fd_order");
   if (d->nx > (1 << 20) || d->nz > (1 << 20) || (int64_t)d->nx * d->nz > (int64_t)1 << 31)
     return fail(c, FOMO_ERR_ARG, "grid too large");
-  if (d->storage != FOMO_STORAGE_F32) return fail(c, FOMO_ERR_UNSUPPORTED, "storage type");
+  if (d->storage < FOMO_STORAGE_F32 || d->storage > FOMO_STORAGE_F16C)
+    return fail(c, FOMO_ERR_ARG, "unknown storage type");
+  if (d->storage != FOMO_STORAGE_F32 && !reduced_m_ok(M))
+    return fail(c, FOMO_ERR_UNSUPPORTED, "reduced-precision storage supports fd_order 4, 8 and 12");
   const bool any_w = w_vx || w_vz || w_tau;
   if (any_w && !(w_vx && w_vz && w_tau))
     return fail(c, FOMO_ERR_ARG, "pass all three HABC weight matrices or none");
@@ -237,6 +350,11 @@ extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, cons
   m->ctx = c;
   m->d = *d;
   m->M = M;
+  m->storage = d->storage;
+  m->ckind = coef_kind(d->storage);
+  m->fkind = field_kind(d->storage);
+  m->es = m->ckind == ST_F32 ? 4 : 2;
+  m->fes = m->fkind == ST_F32 ? 4 : 2;
   for (int l = 0; l < M; ++l) m->a[l] = a[l];
   const int nx = d->nx, nz = d->nz, rl = d->nbc + 2;
   std::vector<float> ramps(3 * rl);
@@ -271,26 +389,55 @@ extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, cons
   const int P = pitch_for(nx);
   const size_t fs = field_stride(nz, P);
   const int nblocks = same ? 5 : 7;  // dtx == dtz: the z-scaled arrays alias the x-scaled ones
+  const int es = m->es;
   m->pitch = P;
   m->fstride = fs;
+  // P impedance range, for the field scales of reduced-precision storage
+  double zmin = INFINITY, zmax = 0;
+  for (size_t o = 0; o < n; ++o) {
+    const double z = std::sqrt(std::max(0.0, (double)h[5][o] * (double)rho_vx[o]));
+    if (z > 0) zmin = std::min(zmin, z);
+    zmax = std::max(zmax, z);
+  }
+  m->zmin = std::isfinite(zmin) ? (float)zmin : 1.f;
+  m->zmax = zmax > 0 ? (float)zmax : 1.f;
   HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMalloc(&m->dcoef, nblocks * fs * sizeof(float)));
-  HIP_TRY(c, hipMemset(m->dcoef, 0, nblocks * fs * sizeof(float)));
+  HIP_TRY(c, hipMalloc(&m->dcoef, nblocks * fs * es));
+  HIP_TRY(c, hipMemset(m->dcoef, 0, nblocks * fs * es));
   // block order: bvx_x, bvz_x, lam, l2m, mu_txz, [bvx_z, bvz_z]
   const int blk_of[7] = {0, same ? 0 : 5, 1, same ? 1 : 6, 2, 3, 4};
+  std::vector<uint16_t> enc(m->ckind == ST_F32 ? 0 : n);
   for (int k = 0; k < 7; ++k) {
-    float* dst = block_origin(m->dcoef, fs, blk_of[k], P);
+    float* dst = block_origin(m->dcoef, fs, blk_of[k], P, es);
     m->coef[k] = dst;
-    if (same && (k == 1 || k == 3)) continue;
     const float* src = src_direct[k] ? src_direct[k] : h[k].data();
-    HIP_TRY(c, hipMemcpy2D(dst, P * sizeof(float), src, nx * sizeof(float), nx * sizeof(float), nz,
+    if (m->ckind != ST_F32) {
+      // FP16: largest magnitude into [2^14, 2^15), below the FP16 maximum
+      // 65504 after dithering; BF16 keeps FP32's range and needs no scale
+      float sc = 1.f;
+      if (m->ckind == ST_F16) {
+        double mx = 0;
+        for (size_t o = 0; o < n; ++o) mx = std::max(mx, (double)std::fabs(src[o]));
+        sc = pow2_scale(mx, 14);
+      }
+      m->csc[k] = 1.0f / sc;
+      if (same && (k == 1 || k == 3)) {  // aliases array 0 / 2
+        m->csc[k] = m->csc[k - 1];
+        continue;
+      }
+      encode_coef(m->ckind, src, nx, nz, sc, k, enc.data());
+      src = (const float*)enc.data();
+    } else if (same && (k == 1 || k == 3)) {
+      continue;
+    }
+    HIP_TRY(c, hipMemcpy2D(dst, (size_t)P * es, src, (size_t)nx * es, (size_t)nx * es, nz,
                            hipMemcpyHostToDevice));
   }
   HIP_TRY(c, hipMalloc(&m->dramp, ramps.size() * sizeof(float)));
   HIP_TRY(c, hipMemcpy(m->dramp, ramps.data(), ramps.size() * sizeof(float), hipMemcpyHostToDevice));
   m->rho_vx.assign(rho_vx, rho_vx + n);
   m->rho_vz.assign(rho_vz, rho_vz + n);
-  m->bytes = (int64_t)nblocks * fs * sizeof(float) + (int64_t)ramps.size() * sizeof(float);
+  m->bytes = (int64_t)nblocks * fs * es + (int64_t)ramps.size() * sizeof(float);
   *out = m.release();
   return FOMO_OK;
 }
@@ -310,22 +457,27 @@ extern "C" int64_t fomo_model_device_bytes(const fomo_model* m) { return m ? m->
 // ------------------------------------------------------------------ wavefield
 static void drop_single(fomo_wavefield* w);
 static int refresh_backup(fomo_wavefield* w);
+// The field scales of a zeroed wavefield (FP16 storage): a unit pressure source.
+static void default_field_scales(const fomo_model* m, float* fsc);
 
 extern "C" int32_t fomo_wavefield_create(fomo_model* m, fomo_wavefield** out) {
   if (!m || !out) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   fomo_ctx* c = m->ctx;
   auto w = std::make_unique<fomo_wavefield>();
   w->m = m;
-  const size_t bytes = 15 * m->fstride * sizeof(float);
+  // two state buffers + (FP32 only: the two-pass path's strip backups)
+  const bool f32 = m->storage == FOMO_STORAGE_F32;
+  const size_t bytes = (f32 ? 15 : 10) * m->fstride * m->fes;
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipMalloc(&w->dbuf, bytes));
   HIP_TRY(c, hipMemset(w->dbuf, 0, bytes));
   for (int k = 0; k < 5; ++k) {
-    w->buf[0][k] = block_origin(w->dbuf, m->fstride, k, m->pitch);
-    w->buf[1][k] = block_origin(w->dbuf, m->fstride, 5 + k, m->pitch);
-    w->old[k] = block_origin(w->dbuf, m->fstride, 10 + k, m->pitch);
+    w->buf[0][k] = block_origin(w->dbuf, m->fstride, k, m->pitch, m->fes);
+    w->buf[1][k] = block_origin(w->dbuf, m->fstride, 5 + k, m->pitch, m->fes);
+    w->old[k] = f32 ? block_origin(w->dbuf, m->fstride, 10 + k, m->pitch, m->fes) : nullptr;
   }
   w->bytes = (int64_t)bytes;
+  default_field_scales(m, w->fsc);
   *out = w.release();
   return FOMO_OK;
 }
@@ -347,6 +499,7 @@ extern "C" int32_t fomo_wavefield_reset(fomo_wavefield* w) {
   HIP_TRY(c, hipMemsetAsync(w->dbuf, 0, (size_t)w->bytes, c->stream));
   w->cur = 0;
   w->backup_is_prev = false;
+  w->zero = true;
   return FOMO_OK;
 }
 
@@ -360,15 +513,27 @@ extern "C" int32_t fomo_wavefield_download(fomo_wavefield* w, int32_t field, flo
   if (!w || !host) return fail(w ? w->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   fomo_ctx* c = w->m->ctx;
   if (field < 0 || field > 9) return fail(c, FOMO_ERR_ARG, "field id out of range");
+  const int st = w->m->storage;
+  if (field >= 5 && st != FOMO_STORAGE_F32)
+    return fail(c, FOMO_ERR_UNSUPPORTED, "*_old fields exist only with FP32 storage");
   if (field >= 5) {
     int rc = refresh_backup(w);
     if (rc) return rc;
   }
-  const int nx = w->m->d.nx, nz = w->m->d.nz, P = w->m->pitch;
+  const int nx = w->m->d.nx, nz = w->m->d.nz, P = w->m->pitch, es = w->m->fes;
+  const int fk = w->m->fkind;
   HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpy2DAsync(host, nx * sizeof(float), field_ptr(w, field), P * sizeof(float),
-                              nx * sizeof(float), nz, hipMemcpyDeviceToHost, c->stream));
+  if (fk == ST_F32) {
+    HIP_TRY(c, hipMemcpy2DAsync(host, nx * sizeof(float), field_ptr(w, field), P * sizeof(float),
+                                nx * sizeof(float), nz, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return FOMO_OK;
+  }
+  std::vector<uint16_t> tmp((size_t)nx * nz);
+  HIP_TRY(c, hipMemcpy2DAsync(tmp.data(), (size_t)nx * es, field_ptr(w, field), (size_t)P * es,
+                              (size_t)nx * es, nz, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  decode(fk, tmp.data(), tmp.size(), 1.0f / w->fsc[field], host);
   return FOMO_OK;
 }
 
@@ -376,15 +541,48 @@ extern "C" int32_t fomo_wavefield_upload(fomo_wavefield* w, int32_t field, const
   if (!w || !host) return fail(w ? w->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   fomo_ctx* c = w->m->ctx;
   if (field < 0 || field > 9) return fail(c, FOMO_ERR_ARG, "field id out of range");
+  const int st = w->m->storage;
+  if (field >= 5 && st != FOMO_STORAGE_F32)
+    return fail(c, FOMO_ERR_UNSUPPORTED, "*_old fields exist only with FP32 storage");
   if (field >= 5) {
     int rc = refresh_backup(w);
     if (rc) return rc;
   }
-  const int nx = w->m->d.nx, nz = w->m->d.nz, P = w->m->pitch;
+  const int nx = w->m->d.nx, nz = w->m->d.nz, P = w->m->pitch, es = w->m->fes;
+  const int fk = w->m->fkind;
   HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpy2DAsync(field_ptr(w, field), P * sizeof(float), host, nx * sizeof(float),
-                              nx * sizeof(float), nz, hipMemcpyHostToDevice, c->stream));
+  w->zero = false;
+  if (fk == ST_F32) {
+    HIP_TRY(c, hipMemcpy2DAsync(field_ptr(w, field), P * sizeof(float), host, nx * sizeof(float),
+                                nx * sizeof(float), nz, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return FOMO_OK;
+  }
+  std::vector<uint16_t> tmp((size_t)nx * nz);
+  encode(fk, host, tmp.size(), w->fsc[field], tmp.data());
+  HIP_TRY(c, hipMemcpy2DAsync(field_ptr(w, field), (size_t)P * es, tmp.data(), (size_t)nx * es,
+                              (size_t)nx * es, nz, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FOMO_OK;
+}
+
+static void default_field_scales(const fomo_model* m, float* fsc) {
+  for (int k = 0; k < 5; ++k) fsc[k] = 1.f;
+  if (m->fkind != ST_F16) return;
+  const float ss = pow2_scale(1.0, 13), sv = pow2_scale(1.0 / m->zmin, 13);
+  fsc[F_VX] = fsc[F_VZ] = sv;
+  fsc[F_TXX] = fsc[F_TZZ] = fsc[F_TXZ] = ss;
+}
+
+// A wavefield holding a nonzero state can only continue at that state's
+// scales; a zeroed one takes the shot's.
+static int adopt_scales(fomo_wavefield* w, const fomo_shot* s) {
+  if (w->m->fkind != ST_F16) return FOMO_OK;
+  if (!w->zero && std::memcmp(w->fsc, s->fsc, sizeof(w->fsc)) != 0)
+    return fail(w->m->ctx, FOMO_ERR_STATE,
+                "wavefield holds a state stored at another shot's FP16 scales: reset it first");
+  std::memcpy(w->fsc, s->fsc, sizeof(w->fsc));
+  w->zero = false;
   return FOMO_OK;
 }
 
@@ -427,6 +625,29 @@ extern "C" int32_t fomo_shot_create(fomo_model* m, int32_t src_i, int32_t src_j,
   const size_t o = (size_t)(src_i - 1) + (size_t)(src_j - 1) * nx;
   if (src_type == FOMO_SRC_FORCE_Z) s->src_scale = m->d.dt / m->rho_vz[o];
   if (src_type == FOMO_SRC_FORCE_X) s->src_scale = m->d.dt / m->rho_vx[o];
+  if (m->fkind == ST_F16) {
+    // Bound of the injected total: sum |w| (into stress) or sum |w| dt / rho
+    // (into velocity); the other fields by the plane-wave relation
+    // stress = impedance * velocity with the impedance extreme that makes it
+    // larger.  Each bound lands in [2^13, 2^14): 4x below the FP16 maximum.
+    double inj = 0;
+    for (int k = 0; k < nw; ++k) inj += std::fabs((double)wavelet[k]);
+    if (!(inj > 0)) inj = 1;
+    double sb, vb;
+    if (src_type == FOMO_SRC_PRESSURE) {
+      sb = inj;
+      vb = inj / m->zmin;
+    } else {
+      vb = inj * s->src_scale;
+      sb = vb * m->zmax;
+    }
+    const float sv = pow2_scale(vb, 13), ss = pow2_scale(sb, 13);
+    s->fsc[F_VX] = sv;
+    s->fsc[F_VZ] = sv;
+    s->fsc[F_TXX] = ss;
+    s->fsc[F_TZZ] = ss;
+    s->fsc[F_TXZ] = ss;
+  }
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipMalloc(&s->dwav, std::max(nw, 1) * sizeof(float)));
   if (nw > 0) HIP_TRY(c, hipMemcpy(s->dwav, wavelet, nw * sizeof(float), hipMemcpyHostToDevice));
@@ -438,6 +659,12 @@ extern "C" int32_t fomo_shot_create(fomo_model* m, int32_t src_i, int32_t src_j,
   HIP_TRY(c, hipMalloc(&s->dk, sizeof(int)));
   HIP_TRY(c, hipMemset(s->dk, 0, sizeof(int)));
   *out = s.release();
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_shot_field_scales(const fomo_shot* s, float* out) {
+  if (!s || !out) return fail(s ? s->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  for (int k = 0; k < 5; ++k) out[k] = s->fsc[k];
   return FOMO_OK;
 }
 
@@ -504,6 +731,11 @@ static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const 
   sd.dtx = d.dtx;
   sd.dtz = d.dtz;
   for (int k = 0; k < 5; ++k) {
+    sd.fsc[k] = s->fsc[k];
+    sd.fisc[k] = 1.0f / s->fsc[k];
+  }
+  for (int k = 0; k < 7; ++k) sd.csc[k] = m->csc[k];
+  for (int k = 0; k < 5; ++k) {
     switch (table) {
       case TBL_IN0: sd.f[k] = w->buf[0][k]; sd.fo[k] = w->old[k]; break;
       case TBL_IN1: sd.f[k] = w->buf[1][k]; sd.fo[k] = w->old[k]; break;
@@ -568,14 +800,15 @@ static int part_items(const ShotDev& sd, int part) {
 constexpr int WPB = 4;  // wavefronts per block of the stencil kernels
 
 // Resident wavefronts of the stencil kernels for (M, cpt) on the whole device.
-template <int M, int CPT>
+template <int K, int M, int CPT>
 static int stencil_slots_mc(int cus, int mode) {
   int bv = 0, bs = 0;
-  if (mode == 1) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
+  if (mode == 1 || K != ST_F32) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused<M, CPT, WPB, K>, 64 * WPB, 0) !=
+        hipSuccess)
       bv = 1;
     bs = bv;
-  } else {
+  } else if constexpr (K == ST_F32) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_velocity<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
       bv = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, k_stress<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
@@ -583,21 +816,33 @@ static int stencil_slots_mc(int cus, int mode) {
   }
   return std::max(1, std::min(bv, bs)) * WPB * std::max(cus, 1);
 }
-template <int M>
+template <int K, int M>
 static int stencil_slots_m(int cpt, int cus, int mode) {
-  return cpt == 1 ? stencil_slots_mc<M, 1>(cus, mode)
-                  : cpt == 2 ? stencil_slots_mc<M, 2>(cus, mode) : stencil_slots_mc<M, 4>(cus, mode);
+  return cpt == 1   ? stencil_slots_mc<K, M, 1>(cus, mode)
+         : cpt == 2 ? stencil_slots_mc<K, M, 2>(cus, mode)
+                    : stencil_slots_mc<K, M, 4>(cus, mode);
 }
-static int stencil_slots(int M, int cpt, int cus, int mode) {
+template <int K>
+static int stencil_slots_reduced(int M, int cpt, int cus) {
   switch (M) {
-    case 1: return stencil_slots_m<1>(cpt, cus, mode);
-    case 2: return stencil_slots_m<2>(cpt, cus, mode);
-    case 3: return stencil_slots_m<3>(cpt, cus, mode);
-    case 4: return stencil_slots_m<4>(cpt, cus, mode);
-    case 5: return stencil_slots_m<5>(cpt, cus, mode);
-    case 6: return stencil_slots_m<6>(cpt, cus, mode);
-    case 7: return stencil_slots_m<7>(cpt, cus, mode);
-    default: return stencil_slots_m<8>(cpt, cus, mode);
+    case 2: return stencil_slots_m<K, 2>(cpt, cus, 1);
+    case 4: return stencil_slots_m<K, 4>(cpt, cus, 1);
+    default: return stencil_slots_m<K, 6>(cpt, cus, 1);
+  }
+}
+static int stencil_slots(int storage, int M, int cpt, int cus, int mode) {
+  if (storage == FOMO_STORAGE_BF16) return stencil_slots_reduced<ST_BF16>(M, cpt, cus);
+  if (storage == FOMO_STORAGE_F16) return stencil_slots_reduced<ST_F16>(M, cpt, cus);
+  if (storage == FOMO_STORAGE_F16C) return stencil_slots_reduced<ST_F16C>(M, cpt, cus);
+  switch (M) {
+    case 1: return stencil_slots_m<ST_F32, 1>(cpt, cus, mode);
+    case 2: return stencil_slots_m<ST_F32, 2>(cpt, cus, mode);
+    case 3: return stencil_slots_m<ST_F32, 3>(cpt, cus, mode);
+    case 4: return stencil_slots_m<ST_F32, 4>(cpt, cus, mode);
+    case 5: return stencil_slots_m<ST_F32, 5>(cpt, cus, mode);
+    case 6: return stencil_slots_m<ST_F32, 6>(cpt, cus, mode);
+    case 7: return stencil_slots_m<ST_F32, 7>(cpt, cus, mode);
+    default: return stencil_slots_m<ST_F32, 8>(cpt, cus, mode);
   }
 }
 
@@ -649,15 +894,22 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
   b->tz_req = c->tz;
   b->graph_steps = c->graph_steps;
   b->step_mode = c->step_mode;
+  if (!ms[0]) return fail(c, FOMO_ERR_ARG, "NULL batch member");
   b->M = ms[0]->M;
+  b->storage = ms[0]->storage;
+  if (b->storage != FOMO_STORAGE_F32 && b->step_mode != 1)
+    return fail(c, FOMO_ERR_UNSUPPORTED, "BF16/FP16 storage runs the fused step only (step mode 1)");
   for (int i = 0; i < n; ++i) {
     if (!ms[i] || !ws[i] || !ss[i]) return fail(c, FOMO_ERR_ARG, "NULL batch member");
     if (ms[i]->ctx != c || ws[i]->m != ms[i] || ss[i]->m != ms[i])
       return fail(c, FOMO_ERR_STATE, "batch member belongs to another model or context");
     if (ms[i]->M != b->M) return fail(c, FOMO_ERR_UNSUPPORTED, "batch mixes FD orders");
+    if (ms[i]->storage != b->storage)
+      return fail(c, FOMO_ERR_UNSUPPORTED, "batch mixes storage types");
     for (int j = 0; j < i; ++j)
       if (ws[j] == ws[i]) return fail(c, FOMO_ERR_ARG, "a wavefield appears twice in the batch");
     b->ws.push_back(ws[i]);
+    b->ss.push_back(ss[i]);
   }
   if (b->tz == 0) {
     std::vector<int> tx, nzs;
@@ -667,7 +919,7 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
       tx.push_back((ms[i]->d.nx + w - 1) / w);
       nzs.push_back(ms[i]->d.nz);
     }
-    b->tz = auto_tz(tx, nzs, stencil_slots(b->M, b->cpt, c->cus, b->step_mode));
+    b->tz = auto_tz(tx, nzs, stencil_slots(b->storage, b->M, b->cpt, c->cus, b->step_mode));
   }
   b->hs.resize((size_t)TBL_COUNT * n);
   for (int t = 0; t < TBL_COUNT; ++t)
@@ -751,63 +1003,86 @@ static BatchArgs args_for(const fomo_batch* b, int table, int part, int k_off) {
   return a;
 }
 
-// kind: 0 velocity, 1 stress (in-place two-pass), 2 fused, 3 band
-template <int M, int CPT>
+// kind: 0 velocity, 1 stress (in-place two-pass, FP32 only), 2 fused, 3 band
+template <int K, int M, int CPT>
 static void launch_stencil_mc(const fomo_batch* b, int table, int kind, int flags, int k_off) {
   dim3 block(64 * WPB);
   hipStream_t st = b->ctx->stream;
-  switch (kind) {
-    case 0:
+  if (kind == 2) {  // grid rounded up to whole XCD rounds (see xcd_block)
+    hipLaunchKernelGGL((k_fused<M, CPT, WPB, K>), dim3(round_up(blocks_for(b, PART_FUSED), NUM_XCD)),
+                       block, 0, st, args_for(b, table, PART_FUSED, k_off), flags);
+  } else if (kind == 3) {
+    if constexpr (CPT == 1)  // one instance per (K, M)
+      hipLaunchKernelGGL((k_band<M, WPB, K>), dim3(blocks_for(b, PART_BAND)), block, 0, st,
+                         args_for(b, table, PART_BAND, k_off), flags);
+  } else if constexpr (K == ST_F32) {
+    if (kind == 0)
       hipLaunchKernelGGL((k_velocity<M, CPT, WPB>), dim3(blocks_for(b, PART_STENCIL)), block, 0, st,
                          args_for(b, table, PART_STENCIL, k_off), flags);
-      break;
-    case 1:
+    else
       hipLaunchKernelGGL((k_stress<M, CPT, WPB>), dim3(blocks_for(b, PART_STENCIL)), block, 0, st,
                          args_for(b, table, PART_STENCIL, k_off), flags);
-      break;
-    case 2:  // grid rounded up to whole XCD rounds (see xcd_block)
-      hipLaunchKernelGGL((k_fused<M, CPT, WPB>), dim3(round_up(blocks_for(b, PART_FUSED), NUM_XCD)),
-                         block, 0, st, args_for(b, table, PART_FUSED, k_off), flags);
-      break;
-    default:
-      hipLaunchKernelGGL((k_band<M, WPB>), dim3(blocks_for(b, PART_BAND)), block, 0, st,
-                         args_for(b, table, PART_BAND, k_off), flags);
-      break;
   }
 }
 
-template <int M>
+template <int K, int M>
 static void launch_stencil_m(const fomo_batch* b, int table, int kind, int flags, int k_off) {
-  switch (b->cpt) {
-    case 1: launch_stencil_mc<M, 1>(b, table, kind, flags, k_off); break;
-    case 2: launch_stencil_mc<M, 2>(b, table, kind, flags, k_off); break;
-    default: launch_stencil_mc<M, 4>(b, table, kind, flags, k_off); break;
+  const int cpt = kind == 3 ? 1 : b->cpt;  // the band kernel has one column per lane
+  switch (cpt) {
+    case 1: launch_stencil_mc<K, M, 1>(b, table, kind, flags, k_off); break;
+    case 2: launch_stencil_mc<K, M, 2>(b, table, kind, flags, k_off); break;
+    default: launch_stencil_mc<K, M, 4>(b, table, kind, flags, k_off); break;
+  }
+}
+
+template <int K>
+static void launch_stencil_reduced(const fomo_batch* b, int table, int kind, int flags, int k_off) {
+  switch (b->M) {  // the model accepted M in {2, 4, 6} only
+    case 2: launch_stencil_m<K, 2>(b, table, kind, flags, k_off); break;
+    case 4: launch_stencil_m<K, 4>(b, table, kind, flags, k_off); break;
+    default: launch_stencil_m<K, 6>(b, table, kind, flags, k_off); break;
   }
 }
 
 static void launch_stencil(const fomo_batch* b, int table, int kind, int flags, int k_off) {
+  if (b->storage == FOMO_STORAGE_BF16) return launch_stencil_reduced<ST_BF16>(b, table, kind, flags, k_off);
+  if (b->storage == FOMO_STORAGE_F16) return launch_stencil_reduced<ST_F16>(b, table, kind, flags, k_off);
+  if (b->storage == FOMO_STORAGE_F16C)
+    return launch_stencil_reduced<ST_F16C>(b, table, kind, flags, k_off);
   switch (b->M) {
-    case 1: launch_stencil_m<1>(b, table, kind, flags, k_off); break;
-    case 2: launch_stencil_m<2>(b, table, kind, flags, k_off); break;
-    case 3: launch_stencil_m<3>(b, table, kind, flags, k_off); break;
-    case 4: launch_stencil_m<4>(b, table, kind, flags, k_off); break;
-    case 5: launch_stencil_m<5>(b, table, kind, flags, k_off); break;
-    case 6: launch_stencil_m<6>(b, table, kind, flags, k_off); break;
-    case 7: launch_stencil_m<7>(b, table, kind, flags, k_off); break;
-    default: launch_stencil_m<8>(b, table, kind, flags, k_off); break;
+    case 1: launch_stencil_m<ST_F32, 1>(b, table, kind, flags, k_off); break;
+    case 2: launch_stencil_m<ST_F32, 2>(b, table, kind, flags, k_off); break;
+    case 3: launch_stencil_m<ST_F32, 3>(b, table, kind, flags, k_off); break;
+    case 4: launch_stencil_m<ST_F32, 4>(b, table, kind, flags, k_off); break;
+    case 5: launch_stencil_m<ST_F32, 5>(b, table, kind, flags, k_off); break;
+    case 6: launch_stencil_m<ST_F32, 6>(b, table, kind, flags, k_off); break;
+    case 7: launch_stencil_m<ST_F32, 7>(b, table, kind, flags, k_off); break;
+    default: launch_stencil_m<ST_F32, 8>(b, table, kind, flags, k_off); break;
   }
 }
 
 static void launch_habc(const fomo_batch* b, int table, bool stress, int k_off) {
   const int part = stress ? PART_HABC_S : PART_HABC_V;
   const int items = b->hprefix[(size_t)part * (b->n + 1) + b->n];
-  hipLaunchKernelGGL(k_habc, dim3(std::max(1, items)), dim3(64), 0, b->ctx->stream,
-                     args_for(b, table, part, k_off), stress ? 1 : 0);
+  const BatchArgs a = args_for(b, table, part, k_off);
+  const dim3 g(std::max(1, items));
+  if (b->storage == FOMO_STORAGE_BF16)
+    hipLaunchKernelGGL(k_habc<ST_BF16>, g, dim3(64), 0, b->ctx->stream, a, stress ? 1 : 0);
+  else if (b->storage == FOMO_STORAGE_F16)
+    hipLaunchKernelGGL(k_habc<ST_F16>, g, dim3(64), 0, b->ctx->stream, a, stress ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_habc<ST_F32>, g, dim3(64), 0, b->ctx->stream, a, stress ? 1 : 0);
 }
 
 static void launch_fsrec(const fomo_batch* b, int table, int mode, int k_off) {
-  hipLaunchKernelGGL(k_fsrec<WPB>, dim3(blocks_for(b, PART_FSREC)), dim3(64 * WPB), 0,
-                     b->ctx->stream, args_for(b, table, PART_FSREC, k_off), mode);
+  const BatchArgs a = args_for(b, table, PART_FSREC, k_off);
+  const dim3 g(blocks_for(b, PART_FSREC)), blk(64 * WPB);
+  if (b->storage == FOMO_STORAGE_BF16)
+    hipLaunchKernelGGL((k_fsrec<WPB, ST_BF16>), g, blk, 0, b->ctx->stream, a, mode);
+  else if (b->storage == FOMO_STORAGE_F16)
+    hipLaunchKernelGGL((k_fsrec<WPB, ST_F16>), g, blk, 0, b->ctx->stream, a, mode);
+  else
+    hipLaunchKernelGGL((k_fsrec<WPB, ST_F32>), g, blk, 0, b->ctx->stream, a, mode);
 }
 
 // One step (time_stepper.jl:39-63) of every shot, starting from state buffer
@@ -850,7 +1125,7 @@ static int check_launch(fomo_ctx* c, const char* what) {
 // After fused steps the strip backup lives in the previous state buffer; copy
 // it into old[] (k_backup from that buffer) before anything reads old[].
 static int refresh_backup(fomo_wavefield* w) {
-  if (!w->backup_is_prev) return FOMO_OK;
+  if (!w->backup_is_prev || w->m->storage != FOMO_STORAGE_F32) return FOMO_OK;
   fomo_batch* b = nullptr;
   int rc = single_batch(w->m, w, nullptr, &b);
   if (rc) return rc;
@@ -875,6 +1150,10 @@ static int run_batch(fomo_batch* b, int k_first, int n_steps) {
       int rc = refresh_backup(w);
       if (rc) return rc;
     }
+  for (int i = 0; i < b->n; ++i) {
+    int rc = adopt_scales(b->ws[i], b->ss[i]);
+    if (rc) return rc;
+  }
   HIP_TRY(c, hipSetDevice(c->device));
   launch_set_step(b, k_first);
   const int G = b->graph_steps;  // even: a chunk ends on the parity it started from
@@ -919,6 +1198,8 @@ extern "C" int32_t fomo_time_step(fomo_model* m, fomo_wavefield* w, fomo_shot* s
     rc = refresh_backup(w);
     if (rc) return rc;
   }
+  rc = adopt_scales(w, s);
+  if (rc) return rc;
   HIP_TRY(m->ctx, hipSetDevice(m->ctx->device));
   launch_set_step(b, k);
   launch_step(b, w->cur, 0);
@@ -933,6 +1214,8 @@ extern "C" int32_t fomo_time_step(fomo_model* m, fomo_wavefield* w, fomo_shot* s
 // the current buffer with old[] as the strip backup.
 static int stage(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int k, int which) {
   if (!m || !w) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  if (m->storage != FOMO_STORAGE_F32)
+    return fail(m->ctx, FOMO_ERR_UNSUPPORTED, "per-stage calls need FP32 storage");
   int rc = refresh_backup(w);
   if (rc) return rc;
   fomo_batch* b = nullptr;
@@ -984,6 +1267,8 @@ extern "C" int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot
                                      int32_t k, int32_t reps) {
   if (!m || !w || !s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   if (which < 0 || which > 8 || reps < 0) return fail(m->ctx, FOMO_ERR_ARG, "bad stage or reps");
+  if (which < 5 && m->storage != FOMO_STORAGE_F32)
+    return fail(m->ctx, FOMO_ERR_UNSUPPORTED, "the in-place stages need FP32 storage");
   fomo_batch* b = nullptr;
   int rc = single_batch(m, w, s, &b);
   if (rc) return rc;

@@ -140,6 +140,119 @@ __device__ __forceinline__ void stv<4>(float* p, const V<4>& a) {
   *(gf4v*)p = t;
 }
 
+// ---- storage traits: FP32 is the identity (no scale multiply), BF16 rounds
+// to nearest even, FP16 uses the hardware conversion (round to nearest even).
+template <int K>
+struct Store;
+template <>
+struct Store<ST_F32> {
+  typedef float T;
+};
+template <>
+struct Store<ST_BF16> {
+  typedef uint16_t T;
+};
+template <>
+struct Store<ST_F16> {
+  typedef uint16_t T;
+};
+
+// Element kinds of the fields (F) and of the coefficient arrays (C) per mode.
+template <int K>
+struct Mode {
+  static constexpr int F = K, C = K;
+};
+template <>
+struct Mode<ST_F16C> {
+  static constexpr int F = ST_F32, C = ST_F16;
+};
+
+template <int K>
+__device__ __forceinline__ float from_bits(uint32_t b) {
+  if constexpr (K == ST_BF16) return __uint_as_float(b << 16);
+  else return (float)__builtin_bit_cast(_Float16, (uint16_t)b);
+}
+template <int K>
+__device__ __forceinline__ uint32_t to_bits(float x) {
+  if constexpr (K == ST_BF16) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+  } else {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x);
+  }
+}
+
+typedef __attribute__((address_space(1))) uint16_t gu16;
+typedef __attribute__((address_space(1))) uint32_t gu32;
+typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) u2v gu2v;
+
+// N consecutive stored values at p, as floats (x stored * s).
+template <int K, int N>
+__device__ __forceinline__ V<N> ldf(const typename Store<K>::T* p, float s) {
+  if constexpr (K == ST_F32) {
+    return ldv<N>(p);
+  } else {
+    V<N> r;
+    if constexpr (N == 1) {
+      r.v[0] = from_bits<K>(*(const gu16*)p) * s;
+    } else if constexpr (N == 2) {
+      const uint32_t w = *(const gu32*)p;
+      r.v[0] = from_bits<K>(w & 0xFFFFu) * s;
+      r.v[1] = from_bits<K>(w >> 16) * s;
+    } else {
+      const u2v w = *(const gu2v*)p;
+      r.v[0] = from_bits<K>(w.x & 0xFFFFu) * s;
+      r.v[1] = from_bits<K>(w.x >> 16) * s;
+      r.v[2] = from_bits<K>(w.y & 0xFFFFu) * s;
+      r.v[3] = from_bits<K>(w.y >> 16) * s;
+    }
+    return r;
+  }
+}
+template <int K, int N>
+__device__ __forceinline__ void stf(typename Store<K>::T* p, const V<N>& a, float s) {
+  if constexpr (K == ST_F32) {
+    stv<N>(p, a);
+  } else if constexpr (N == 1) {
+    *(gu16*)p = (uint16_t)to_bits<K>(a.v[0] * s);
+  } else if constexpr (N == 2) {
+    *(gu32*)p = to_bits<K>(a.v[0] * s) | (to_bits<K>(a.v[1] * s) << 16);
+  } else {
+    u2v w;
+    w.x = to_bits<K>(a.v[0] * s) | (to_bits<K>(a.v[1] * s) << 16);
+    w.y = to_bits<K>(a.v[2] * s) | (to_bits<K>(a.v[3] * s) << 16);
+    *(gu2v*)p = w;
+  }
+}
+template <int K>
+__device__ __forceinline__ float ld1(const typename Store<K>::T* p, float s) {
+  if constexpr (K == ST_F32) return ldg(p);
+  else return from_bits<K>(*(const gu16*)p) * s;
+}
+template <int K>
+__device__ __forceinline__ void st1(typename Store<K>::T* p, float x, float s) {
+  if constexpr (K == ST_F32) stg(p, x);
+  else *(gu16*)p = (uint16_t)to_bits<K>(x * s);
+}
+
+// x as the storage holds it (stored with scale s, read back with is = 1/s).
+template <int K>
+__device__ __forceinline__ float as_stored(float x, float s, float is) {
+  if constexpr (K == ST_F32) return x;
+  else return from_bits<K>(to_bits<K>(x * s)) * is;
+}
+
+template <class T>
+__device__ __forceinline__ const T* rowt(const T* base, int j, int pitch) {
+  return base + (ptrdiff_t)j * pitch;
+}
+template <class T>
+__device__ __forceinline__ T* rowt(T* base, int j, int pitch) {
+  return base + (ptrdiff_t)j * pitch;
+}
+
 __device__ __forceinline__ const float* row(const float* base, int j, int pitch) {
   return base + (ptrdiff_t)j * pitch;
 }
@@ -537,8 +650,11 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
 // left to k_band (they need HABC-corrected velocities).
 // Arithmetic is that of k_velocity / k_stress exactly.
 // ---------------------------------------------------------------------------
-template <int M, int CPT, int WPB>
+template <int M, int CPT, int WPB, int K>
 __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
+  constexpr int KF = Mode<K>::F, KC = Mode<K>::C;
+  typedef typename Store<KF>::T T;
+  typedef typename Store<KC>::T TC;
   constexpr int HL = ((M + CPT - 1) / CPT) * CPT;  // computed-not-written lane columns per side
   constexpr int LC = 64 * CPT;
   constexpr int LW = LC + 2 * M;
@@ -571,24 +687,24 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     const bool has_halo = hx >= 0;
     const int hcol = o0 - HL - M + hx;
 
-    const float* avx = uniform_p(sd.fo[F_VX]);
-    const float* avz = uniform_p(sd.fo[F_VZ]);
-    const float* atxx = uniform_p(sd.fo[F_TXX]);
-    const float* atzz = uniform_p(sd.fo[F_TZZ]);
-    const float* atxz = uniform_p(sd.fo[F_TXZ]);
-    float* bvx = uniform_p(sd.f[F_VX]);
-    float* bvz = uniform_p(sd.f[F_VZ]);
-    float* btxx = uniform_p(sd.f[F_TXX]);
-    float* btzz = uniform_p(sd.f[F_TZZ]);
-    float* btxz = uniform_p(sd.f[F_TXZ]);
-    const float* b1p = uniform_p(sd.bvx_x);
-    const float* b2p = uniform_p(sd.bvx_z);
-    const float* b3p = uniform_p(sd.bvz_x);
-    const float* b4p = uniform_p(sd.bvz_z);
+    const T* avx = (const T*)uniform_p(sd.fo[F_VX]);
+    const T* avz = (const T*)uniform_p(sd.fo[F_VZ]);
+    const T* atxx = (const T*)uniform_p(sd.fo[F_TXX]);
+    const T* atzz = (const T*)uniform_p(sd.fo[F_TZZ]);
+    const T* atxz = (const T*)uniform_p(sd.fo[F_TXZ]);
+    T* bvx = (T*)uniform_p(sd.f[F_VX]);
+    T* bvz = (T*)uniform_p(sd.f[F_VZ]);
+    T* btxx = (T*)uniform_p(sd.f[F_TXX]);
+    T* btzz = (T*)uniform_p(sd.f[F_TZZ]);
+    T* btxz = (T*)uniform_p(sd.f[F_TXZ]);
+    const TC* b1p = (const TC*)uniform_p(sd.bvx_x);
+    const TC* b2p = (const TC*)uniform_p(sd.bvx_z);
+    const TC* b3p = (const TC*)uniform_p(sd.bvz_x);
+    const TC* b4p = (const TC*)uniform_p(sd.bvz_z);
     const bool bsame = (b1p == b2p) && (b3p == b4p);
-    const float* lamp = uniform_p(sd.lam);
-    const float* l2mp = uniform_p(sd.l2m);
-    const float* mup = uniform_p(sd.mu_txz);
+    const TC* lamp = (const TC*)uniform_p(sd.lam);
+    const TC* l2mp = (const TC*)uniform_p(sd.l2m);
+    const TC* mup = (const TC*)uniform_p(sd.mu_txz);
     const int nw = uniform(sd.nw), stype = uniform(sd.stype);
     const bool src_on = (flags & FLAG_INJECT) && k <= nw;
     const float wav = src_on ? uniform_f(uniform_p(sd.wavelet)[k - 1]) : 0.f;
@@ -598,6 +714,17 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     const int si = uniform(sd.si), sj = uniform(sd.sj);
     const float src_scale = uniform_f(sd.src_scale);
     const float dtx = uniform_f(sd.dtx), dtz = uniform_f(sd.dtz);
+    // load (r*) and store (w*) scales of reduced-precision storage (unused for FP32)
+    const float rvx = uniform_f(sd.fisc[F_VX]), rvz = uniform_f(sd.fisc[F_VZ]);
+    const float rxx = uniform_f(sd.fisc[F_TXX]), rzz = uniform_f(sd.fisc[F_TZZ]);
+    const float rxz = uniform_f(sd.fisc[F_TXZ]);
+    const float wvxs = uniform_f(sd.fsc[F_VX]), wvzs = uniform_f(sd.fsc[F_VZ]);
+    const float wxx = uniform_f(sd.fsc[F_TXX]), wzz = uniform_f(sd.fsc[F_TZZ]);
+    const float wxz = uniform_f(sd.fsc[F_TXZ]);
+    const float r1 = uniform_f(sd.csc[C_BVX_X]), r2 = uniform_f(sd.csc[C_BVX_Z]);
+    const float r3 = uniform_f(sd.csc[C_BVZ_X]), r4 = uniform_f(sd.csc[C_BVZ_Z]);
+    const float rl = uniform_f(sd.csc[C_LAM]), rl2 = uniform_f(sd.csc[C_L2M]);
+    const float rmu = uniform_f(sd.csc[C_MU]);
     float a[M];
 #pragma unroll
     for (int l = 0; l < M; ++l) a[l] = uniform_f(sd.a[l]);
@@ -613,19 +740,19 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     const int jb = j0 - M;       // first velocity row
 #pragma unroll
     for (int q = 0; q < 2 * M; ++q) {
-      V<CPT> x = ldv<CPT>(row(atxz, jb - M + q, P) + ib);
+      V<CPT> x = ldf<KF, CPT>(rowt(atxz, jb - M + q, P) + ib, rxz);
 #pragma unroll
       for (int c = 0; c < CPT; ++c) wtxz[c][q] = x.v[c];
     }
 #pragma unroll
     for (int q = 0; q < 2 * M + 1; ++q) {
-      V<CPT> z = ldv<CPT>(row(atzz, jb - M + q, P) + ib);
+      V<CPT> z = ldf<KF, CPT>(rowt(atzz, jb - M + q, P) + ib, rzz);
 #pragma unroll
       for (int c = 0; c < CPT; ++c) wtzz[c][q] = z.v[c] + srcp(ib + c, jb - M + q);
     }
 #pragma unroll
     for (int q = 0; q < M + 1; ++q) {
-      V<CPT> x = ldv<CPT>(row(atxx, jb - M + q, P) + ib);
+      V<CPT> x = ldf<KF, CPT>(rowt(atxx, jb - M + q, P) + ib, rxx);
 #pragma unroll
       for (int c = 0; c < CPT; ++c) wtxx[c][q] = x.v[c] + srcp(ib + c, jb - M + q);
     }
@@ -636,42 +763,42 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
 #pragma unroll
       for (int q = 0; q < 2 * M + 1; ++q) wvz[c][q] = 0.f;
     }
-    float c_htxx = has_halo ? ldg(row(atxx, jb, P) + hcol) + srcp(hcol, jb) : 0.f;
-    float c_htxz = has_halo ? ldg(row(atxz, jb, P) + hcol) : 0.f;
-    V<CPT> c_vx = ldv<CPT>(row(avx, jb, P) + ib);
-    V<CPT> c_vz = ldv<CPT>(row(avz, jb, P) + ib);
-    V<CPT> c_b1 = ldv<CPT>(row(b1p, jb, P) + ib);
-    V<CPT> c_b3 = ldv<CPT>(row(b3p, jb, P) + ib);
+    float c_htxx = has_halo ? ld1<KF>(rowt(atxx, jb, P) + hcol, rxx) + srcp(hcol, jb) : 0.f;
+    float c_htxz = has_halo ? ld1<KF>(rowt(atxz, jb, P) + hcol, rxz) : 0.f;
+    V<CPT> c_vx = ldf<KF, CPT>(rowt(avx, jb, P) + ib, rvx);
+    V<CPT> c_vz = ldf<KF, CPT>(rowt(avz, jb, P) + ib, rvz);
+    V<CPT> c_b1 = ldf<KC, CPT>(rowt(b1p, jb, P) + ib, r1);
+    V<CPT> c_b3 = ldf<KC, CPT>(rowt(b3p, jb, P) + ib, r3);
     V<CPT> c_b2 = c_b1, c_b4 = c_b3;
     if (!bsame) {
-      c_b2 = ldv<CPT>(row(b2p, jb, P) + ib);
-      c_b4 = ldv<CPT>(row(b4p, jb, P) + ib);
+      c_b2 = ldf<KC, CPT>(rowt(b2p, jb, P) + ib, r2);
+      c_b4 = ldf<KC, CPT>(rowt(b4p, jb, P) + ib, r4);
     }
-    V<CPT> c_lam = ldv<CPT>(row(lamp, jb - M, P) + ib);
-    V<CPT> c_l2m = ldv<CPT>(row(l2mp, jb - M, P) + ib);
-    V<CPT> c_mu = ldv<CPT>(row(mup, jb - M, P) + ib);
+    V<CPT> c_lam = ldf<KC, CPT>(rowt(lamp, jb - M, P) + ib, rl);
+    V<CPT> c_l2m = ldf<KC, CPT>(rowt(l2mp, jb - M, P) + ib, rl2);
+    V<CPT> c_mu = ldf<KC, CPT>(rowt(mup, jb - M, P) + ib, rmu);
 
     const int jend = j1 + M;
     for (int jv = jb; jv < jend; ++jv) {
       // prefetch for jv+1 (velocity inputs) and stress row jv+1-M (coefficients)
       const int jn = jv + 1;
-      float n_htxx = has_halo ? ldg(row(atxx, jn, P) + hcol) : 0.f;
-      float n_htxz = has_halo ? ldg(row(atxz, jn, P) + hcol) : 0.f;
-      V<CPT> n_txx = ldv<CPT>(row(atxx, jn, P) + ib);
-      V<CPT> n_txz = ldv<CPT>(row(atxz, jv + M, P) + ib);
-      V<CPT> n_tzz = ldv<CPT>(row(atzz, jv + M + 1, P) + ib);
-      V<CPT> n_vx = ldv<CPT>(row(avx, jn, P) + ib);
-      V<CPT> n_vz = ldv<CPT>(row(avz, jn, P) + ib);
-      V<CPT> n_b1 = ldv<CPT>(row(b1p, jn, P) + ib);
-      V<CPT> n_b3 = ldv<CPT>(row(b3p, jn, P) + ib);
+      float n_htxx = has_halo ? ld1<KF>(rowt(atxx, jn, P) + hcol, rxx) : 0.f;
+      float n_htxz = has_halo ? ld1<KF>(rowt(atxz, jn, P) + hcol, rxz) : 0.f;
+      V<CPT> n_txx = ldf<KF, CPT>(rowt(atxx, jn, P) + ib, rxx);
+      V<CPT> n_txz = ldf<KF, CPT>(rowt(atxz, jv + M, P) + ib, rxz);
+      V<CPT> n_tzz = ldf<KF, CPT>(rowt(atzz, jv + M + 1, P) + ib, rzz);
+      V<CPT> n_vx = ldf<KF, CPT>(rowt(avx, jn, P) + ib, rvx);
+      V<CPT> n_vz = ldf<KF, CPT>(rowt(avz, jn, P) + ib, rvz);
+      V<CPT> n_b1 = ldf<KC, CPT>(rowt(b1p, jn, P) + ib, r1);
+      V<CPT> n_b3 = ldf<KC, CPT>(rowt(b3p, jn, P) + ib, r3);
       V<CPT> n_b2 = n_b1, n_b4 = n_b3;
       if (!bsame) {
-        n_b2 = ldv<CPT>(row(b2p, jn, P) + ib);
-        n_b4 = ldv<CPT>(row(b4p, jn, P) + ib);
+        n_b2 = ldf<KC, CPT>(rowt(b2p, jn, P) + ib, r2);
+        n_b4 = ldf<KC, CPT>(rowt(b4p, jn, P) + ib, r4);
       }
-      V<CPT> n_lam = ldv<CPT>(row(lamp, jn - M, P) + ib);
-      V<CPT> n_l2m = ldv<CPT>(row(l2mp, jn - M, P) + ib);
-      V<CPT> n_mu = ldv<CPT>(row(mup, jn - M, P) + ib);
+      V<CPT> n_lam = ldf<KC, CPT>(rowt(lamp, jn - M, P) + ib, rl);
+      V<CPT> n_l2m = ldf<KC, CPT>(rowt(l2mp, jn - M, P) + ib, rl2);
+      V<CPT> n_mu = ldf<KC, CPT>(rowt(mup, jn - M, P) + ib, rmu);
 
       // ---- new velocity, row jv, all lane columns
 #pragma unroll
@@ -705,16 +832,17 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
         }
         const float ix = fmaf_(c_b2.v[c], d2, c_b1.v[c] * d1);
         const float iz = fmaf_(c_b4.v[c], d4, c_b3.v[c] * d3);
-        nvx.v[c] = inside ? vxs + ix : vxs;
-        nvz.v[c] = inside ? vzs + iz : vzs;
+        // the stress half below consumes the velocity as stored
+        nvx.v[c] = as_stored<KF>(inside ? vxs + ix : vxs, wvxs, rvx);
+        nvz.v[c] = as_stored<KF>(inside ? vzs + iz : vzs, wvzs, rvz);
       }
       const bool own_lane = lane * CPT >= HL && lane * CPT < LC - HL;
       if (jv >= j0 && jv < j1 && own_lane) {
 #pragma unroll
         for (int c = 0; c < CPT; ++c)
           if (ib + c < nx) {
-            stg(row(bvx, jv, P) + ib + c, nvx.v[c]);
-            stg(row(bvz, jv, P) + ib + c, nvz.v[c]);
+            st1<KF>(rowt(bvx, jv, P) + ib + c, nvx.v[c], wvxs);
+            st1<KF>(rowt(bvz, jv, P) + ib + c, nvz.v[c], wvzs);
           }
       }
 #pragma unroll
@@ -769,9 +897,9 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
           for (int c = 0; c < CPT; ++c) {
             const int i = ib + c;
             if (i >= band && i < nx - band) {
-              stg(row(btxx, j, P) + i, oxx.v[c]);
-              stg(row(btzz, j, P) + i, ozz.v[c]);
-              stg(row(btxz, j, P) + i, oxz.v[c]);
+              st1<KF>(rowt(btxx, j, P) + i, oxx.v[c], wxx);
+              st1<KF>(rowt(btzz, j, P) + i, ozz.v[c], wzz);
+              st1<KF>(rowt(btxz, j, P) + i, oxz.v[c], wxz);
             }
           }
         }
@@ -810,8 +938,11 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
 // new velocities from B (f[]), old stress from A (fo[]), output to B.  One
 // wavefront per (band, 64-column chunk, BAND_ROWS-row segment); the same
 // arithmetic as k_stress.
-template <int M, int WPB>
+template <int M, int WPB, int K>
 __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
+  constexpr int KF = Mode<K>::F, KC = Mode<K>::C;
+  typedef typename Store<KF>::T T;
+  typedef typename Store<KC>::T TC;
   constexpr int LW = 64 + 2 * M;
   __shared__ float lds[WPB][2][LW];
   const int lane = threadIdx.x & 63;
@@ -859,22 +990,29 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
     else if (lane >= 64 - M) hx = 64 + M + (lane - (64 - M));
     const bool has_halo = hx >= 0;
     const int hcol = c0 - M + hx;
-    const float* vx = uniform_p(sd.f[F_VX]);
-    const float* vz = uniform_p(sd.f[F_VZ]);
-    const float* atxx = uniform_p(sd.fo[F_TXX]);
-    const float* atzz = uniform_p(sd.fo[F_TZZ]);
-    const float* atxz = uniform_p(sd.fo[F_TXZ]);
-    float* btxx = uniform_p(sd.f[F_TXX]);
-    float* btzz = uniform_p(sd.f[F_TZZ]);
-    float* btxz = uniform_p(sd.f[F_TXZ]);
-    const float* lamp = uniform_p(sd.lam);
-    const float* l2mp = uniform_p(sd.l2m);
-    const float* mup = uniform_p(sd.mu_txz);
+    const T* vx = (const T*)uniform_p(sd.f[F_VX]);
+    const T* vz = (const T*)uniform_p(sd.f[F_VZ]);
+    const T* atxx = (const T*)uniform_p(sd.fo[F_TXX]);
+    const T* atzz = (const T*)uniform_p(sd.fo[F_TZZ]);
+    const T* atxz = (const T*)uniform_p(sd.fo[F_TXZ]);
+    T* btxx = (T*)uniform_p(sd.f[F_TXX]);
+    T* btzz = (T*)uniform_p(sd.f[F_TZZ]);
+    T* btxz = (T*)uniform_p(sd.f[F_TXZ]);
+    const TC* lamp = (const TC*)uniform_p(sd.lam);
+    const TC* l2mp = (const TC*)uniform_p(sd.l2m);
+    const TC* mup = (const TC*)uniform_p(sd.mu_txz);
     const int nw = uniform(sd.nw), stype = uniform(sd.stype);
     const bool src_on = (flags & FLAG_INJECT) && k <= nw && stype == SRC_PRESSURE;
     const float wav = src_on ? uniform_f(uniform_p(sd.wavelet)[k - 1]) : 0.f;
     const int si = uniform(sd.si), sj = uniform(sd.sj);
     const float dtx = uniform_f(sd.dtx), dtz = uniform_f(sd.dtz);
+    const float rvx = uniform_f(sd.fisc[F_VX]), rvz = uniform_f(sd.fisc[F_VZ]);
+    const float rxx = uniform_f(sd.fisc[F_TXX]), rzz = uniform_f(sd.fisc[F_TZZ]);
+    const float rxz = uniform_f(sd.fisc[F_TXZ]);
+    const float wxx = uniform_f(sd.fsc[F_TXX]), wzz = uniform_f(sd.fsc[F_TZZ]);
+    const float wxz = uniform_f(sd.fsc[F_TXZ]);
+    const float rl = uniform_f(sd.csc[C_LAM]), rl2 = uniform_f(sd.csc[C_L2M]);
+    const float rmu = uniform_f(sd.csc[C_MU]);
     float a[M];
 #pragma unroll
     for (int l = 0; l < M; ++l) a[l] = uniform_f(sd.a[l]);
@@ -884,26 +1022,26 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
     float wvz[2 * M];  // vz rows j-M .. j+M-1
 #pragma unroll
     for (int q = 0; q < 2 * M; ++q) {
-      wvx[q] = ldg(row(vx, r0 - M + 1 + q, P) + ib);
-      wvz[q] = ldg(row(vz, r0 - M + q, P) + ib);
+      wvx[q] = ld1<KF>(rowt(vx, r0 - M + 1 + q, P) + ib, rvx);
+      wvz[q] = ld1<KF>(rowt(vz, r0 - M + q, P) + ib, rvz);
     }
     // row r0's pointwise operands; each iteration prefetches the next row's
-    float hvx = has_halo ? ldg(row(vx, r0, P) + hcol) : 0.f;
-    float hvz = has_halo ? ldg(row(vz, r0, P) + hcol) : 0.f;
-    float lv = ldg(row(lamp, r0, P) + ib), l2 = ldg(row(l2mp, r0, P) + ib);
-    float mu = ldg(row(mup, r0, P) + ib);
-    float oxx = ldg(row(atxx, r0, P) + ib), ozz = ldg(row(atzz, r0, P) + ib);
-    float oxz = ldg(row(atxz, r0, P) + ib);
+    float hvx = has_halo ? ld1<KF>(rowt(vx, r0, P) + hcol, rvx) : 0.f;
+    float hvz = has_halo ? ld1<KF>(rowt(vz, r0, P) + hcol, rvz) : 0.f;
+    float lv = ld1<KC>(rowt(lamp, r0, P) + ib, rl), l2 = ld1<KC>(rowt(l2mp, r0, P) + ib, rl2);
+    float mu = ld1<KC>(rowt(mup, r0, P) + ib, rmu);
+    float oxx = ld1<KF>(rowt(atxx, r0, P) + ib, rxx), ozz = ld1<KF>(rowt(atzz, r0, P) + ib, rzz);
+    float oxz = ld1<KF>(rowt(atxz, r0, P) + ib, rxz);
     for (int j = r0; j < r1; ++j) {
       const int jn = j + 1;
-      const float n_hvx = has_halo ? ldg(row(vx, jn, P) + hcol) : 0.f;
-      const float n_hvz = has_halo ? ldg(row(vz, jn, P) + hcol) : 0.f;
-      const float n_lv = ldg(row(lamp, jn, P) + ib), n_l2 = ldg(row(l2mp, jn, P) + ib);
-      const float n_mu = ldg(row(mup, jn, P) + ib);
-      const float n_oxx = ldg(row(atxx, jn, P) + ib), n_ozz = ldg(row(atzz, jn, P) + ib);
-      const float n_oxz = ldg(row(atxz, jn, P) + ib);
-      const float n_wvx = ldg(row(vx, j + M + 1, P) + ib);
-      const float n_wvz = ldg(row(vz, j + M, P) + ib);
+      const float n_hvx = has_halo ? ld1<KF>(rowt(vx, jn, P) + hcol, rvx) : 0.f;
+      const float n_hvz = has_halo ? ld1<KF>(rowt(vz, jn, P) + hcol, rvz) : 0.f;
+      const float n_lv = ld1<KC>(rowt(lamp, jn, P) + ib, rl), n_l2 = ld1<KC>(rowt(l2mp, jn, P) + ib, rl2);
+      const float n_mu = ld1<KC>(rowt(mup, jn, P) + ib, rmu);
+      const float n_oxx = ld1<KF>(rowt(atxx, jn, P) + ib, rxx), n_ozz = ld1<KF>(rowt(atzz, jn, P) + ib, rzz);
+      const float n_oxz = ld1<KF>(rowt(atxz, jn, P) + ib, rxz);
+      const float n_wvx = ld1<KF>(rowt(vx, j + M + 1, P) + ib, rvx);
+      const float n_wvz = ld1<KF>(rowt(vz, j + M, P) + ib, rvz);
       s_vx[M + lane] = wvx[M - 1];
       s_vz[M + lane] = wvz[M];
       if (has_halo) {
@@ -932,9 +1070,9 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
       const float izz = fmaf_(l2, t2, lv * t1);
       const float sxz = fmaf_(d4, dtx, d3 * dtz);
       if (i < c1) {
-        stg(row(btxx, j, P) + i, inside ? xx + ixx : xx);
-        stg(row(btzz, j, P) + i, inside ? zz + izz : zz);
-        stg(row(btxz, j, P) + i, inside ? oxz + mu * sxz : oxz);
+        st1<KF>(rowt(btxx, j, P) + i, inside ? xx + ixx : xx, wxx);
+        st1<KF>(rowt(btzz, j, P) + i, inside ? zz + izz : zz, wzz);
+        st1<KF>(rowt(btxz, j, P) + i, inside ? oxz + mu * sxz : oxz, wxz);
       }
       wave_sync();
 #pragma unroll
@@ -1010,9 +1148,10 @@ struct HabcGeo {
 };
 
 struct HabcField {
-  float* f;
-  const float* fo;
+  void* f;
+  const void* fo;
   const float* ramp;
+  float rs, ws;  // load / store scale of the field
 };
 
 __device__ __forceinline__ HabcField habc_field(const ShotDev& sd, int stress, int fi) {
@@ -1021,6 +1160,8 @@ __device__ __forceinline__ HabcField habc_field(const ShotDev& sd, int stress, i
   h.f = sd.f[c];
   h.fo = sd.fo[c];
   h.ramp = sd.ramp + (stress ? RAMP_TAU : fi) * sd.ramp_len;
+  h.rs = sd.fisc[c];
+  h.ws = sd.fsc[c];
   return h;
 }
 
@@ -1030,8 +1171,10 @@ constexpr int HS = 65;  // LDS row stride: odd, so row and column walks are bank
 // (s[r*HS + c], so[r*HS + c]), nc <= 64.  Loads of both arrays go out 32 rows
 // at a time before any LDS store: a chunk costs one memory latency.
 constexpr int STAGE_CHUNK = 32;
-__device__ __forceinline__ void lds_in2(float* s, float* so, const float* g, const float* go,
-                                        int P, int r0, int nr, int c0, int nc, int lane) {
+template <int K>
+__device__ __forceinline__ void lds_in2(float* s, float* so, const typename Store<K>::T* g,
+                                        const typename Store<K>::T* go, float rs, int P, int r0,
+                                        int nr, int c0, int nc, int lane) {
   const bool on = lane < nc;
   for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
     float v[STAGE_CHUNK], w[STAGE_CHUNK];
@@ -1039,8 +1182,8 @@ __device__ __forceinline__ void lds_in2(float* s, float* so, const float* g, con
     for (int r = 0; r < STAGE_CHUNK; ++r) {
       const bool ok = on && rb + r < nr;
       const ptrdiff_t o = (ptrdiff_t)(r0 + rb + r) * P + c0 + lane;
-      v[r] = ok ? ldg(g + o) : 0.f;
-      w[r] = ok ? ldg(go + o) : 0.f;
+      v[r] = ok ? ld1<K>(g + o, rs) : 0.f;
+      w[r] = ok ? ld1<K>(go + o, rs) : 0.f;
     }
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r)
@@ -1051,8 +1194,9 @@ __device__ __forceinline__ void lds_in2(float* s, float* so, const float* g, con
   }
 }
 // s[(sr+r)*HS + sc + c] -> rows [r0, r0+nr) x cols [c0, c0+nc) of g   (nc <= 64)
-__device__ __forceinline__ void lds_out(const float* s, int sr, int sc, float* g, int P, int r0,
-                                        int nr, int c0, int nc, int lane) {
+template <int K>
+__device__ __forceinline__ void lds_out(const float* s, int sr, int sc, typename Store<K>::T* g,
+                                        float ws, int P, int r0, int nr, int c0, int nc, int lane) {
   const bool on = lane < nc;
   for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
     float v[STAGE_CHUNK];
@@ -1061,14 +1205,17 @@ __device__ __forceinline__ void lds_out(const float* s, int sr, int sc, float* g
       v[r] = (on && rb + r < nr) ? s[(sr + rb + r) * HS + sc + lane] : 0.f;
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r)
-      if (on && rb + r < nr) stg(g + (ptrdiff_t)(r0 + rb + r) * P + c0 + lane, v[r]);
+      if (on && rb + r < nr) st1<K>(g + (ptrdiff_t)(r0 + rb + r) * P + c0 + lane, v[r], ws);
   }
 }
 
 // Items per field of the HABC partition and the first side-edge row.
 __device__ __forceinline__ int habc_side_r0(const ShotDev& sd) { return sd.fs ? sd.nbc : sd.nbc + 2; }
 
+template <int K>
 __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
+  constexpr int KF = Mode<K>::F;
+  typedef typename Store<KF>::T T;
   __shared__ float s_f[64 * HS];
   __shared__ float s_o[64 * HS];
   __shared__ float s_w[MAX_NBC + 2];
@@ -1090,8 +1237,9 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
     const int pc = p;
     p -= ncorner;
     HabcField h = habc_field(sd, stress, fi);
-    h.f = uniform_p(h.f);
-    h.fo = uniform_p(h.fo);
+    T* hf = (T*)uniform_p(h.f);
+    const T* hfo = (const T*)uniform_p(h.fo);
+    const float hrs = uniform_f(h.rs), hws = uniform_f(h.ws);
     const Hq q{uniform_f(sd.qx), uniform_f(sd.qz), uniform_f(sd.qt_x), uniform_f(sd.qt_z),
                uniform_f(sd.qxt)};
     const int n = uniform(sd.nbc), nx = uniform(sd.nx), nz = uniform(sd.nz), P = uniform(sd.pitch);
@@ -1106,7 +1254,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       const int j0 = ra + (right ? p - sd.n_l : p) * 64;
       const int nr = min(64, rb - j0);
       const int c0 = right ? nx - n - 2 : 1;  // staged columns [c0, c0+n+1)
-      lds_in2(s_f, s_o, h.f, h.fo, P, j0, nr, c0, n + 1, lane);
+      lds_in2<KF>(s_f, s_o, hf, hfo, hrs, P, j0, nr, c0, n + 1, lane);
       __syncthreads();
       if (!right) {
         // left edge, i = 1..n: Jacobi on the row's pre-HABC values (rows independent)
@@ -1139,7 +1287,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       }
       __syncthreads();
       // left: block columns 0..n-1 = columns 1..n; right: block columns 1..n
-      lds_out(s_f, 0, right ? 1 : 0, h.f, P, j0, nr, right ? c0 + 1 : 1, n, lane);
+      lds_out<KF>(s_f, 0, right ? 1 : 0, hf, hws, P, j0, nr, right ? c0 + 1 : 1, n, lane);
       continue;
     }
     p -= sd.n_l + sd.n_r;
@@ -1148,7 +1296,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       const int i0 = ca + (top ? p - sd.n_b : p) * 64;
       const int nc = min(64, cb - i0);
       const int r0 = top ? 1 : nz - n - 2;  // staged rows [r0, r0+n+1)
-      lds_in2(s_f, s_o, h.f, h.fo, P, r0, n + 1, i0, nc, lane);
+      lds_in2<KF>(s_f, s_o, hf, hfo, hrs, P, r0, n + 1, i0, nc, lane);
       __syncthreads();
       if (lane < nc) {
         const int i = i0 + lane;
@@ -1178,7 +1326,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       }
       __syncthreads();
       // bottom: block rows 1..n = rows nz-n-1..nz-2; top: block rows 0..n-1 = rows 1..n
-      lds_out(s_f, top ? 0 : 1, 0, h.f, P, top ? r0 : r0 + 1, n, i0, nc, lane);
+      lds_out<KF>(s_f, top ? 0 : 1, 0, hf, hws, P, top ? r0 : r0 + 1, n, i0, nc, lane);
       continue;
     }
     // Corner pieces: an (n+1) x (n+1) block.
@@ -1186,7 +1334,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
     const bool leftc = (pc % 2) == 0;
     const int r0 = bottom ? nz - n - 2 : 1;
     const int c0 = leftc ? 1 : nx - n - 2;
-    lds_in2(s_f, s_o, h.f, h.fo, P, r0, n + 1, c0, n + 1, lane);
+    lds_in2<KF>(s_f, s_o, hf, hfo, hrs, P, r0, n + 1, c0, n + 1, lane);
     __syncthreads();
     const bool act = lane <= n;
     if (bottom && leftc) {
@@ -1321,7 +1469,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       }
     }
     __syncthreads();
-    lds_out(s_f, 0, 0, h.f, P, r0, n + 1, c0, n + 1, lane);
+    lds_out<KF>(s_f, 0, 0, hf, hws, P, r0, n + 1, c0, n + 1, lane);
   }
 }
 
@@ -1331,8 +1479,10 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
 // pad-5..pad; the rest sample receivers into gather[k-1][r].  A pressure
 // receiver on a free-surface row reads tzz as the 0 it is being set to.
 // ---------------------------------------------------------------------------
-template <int WPB>
+template <int WPB, int K>
 __global__ __launch_bounds__(64 * WPB) void k_fsrec(BatchArgs ba, int mode) {
+  constexpr int KF = Mode<K>::F;
+  typedef typename Store<KF>::T T;
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
   const int nsh = *ba.n_shots_p;
@@ -1351,8 +1501,8 @@ __global__ __launch_bounds__(64 * WPB) void k_fsrec(BatchArgs ba, int mode) {
       if (!fs_on) continue;
       const int j = sd.pad - 5 + idx / nx, i = idx % nx;
       if (j >= 0 && j < sd.nz) {
-        row(sd.f[F_TZZ], j, P)[i] = 0.f;
-        row(sd.f[F_TXZ], j, P)[i] = 0.f;
+        st1<KF>(rowt((T*)sd.f[F_TZZ], j, P) + i, 0.f, 1.f);
+        st1<KF>(rowt((T*)sd.f[F_TXZ], j, P) + i, 0.f, 1.f);
       }
       continue;
     }
@@ -1360,12 +1510,13 @@ __global__ __launch_bounds__(64 * WPB) void k_fsrec(BatchArgs ba, int mode) {
     if (!(mode & MODE_RECORD) || r >= sd.n_rec) continue;
     const int ri = sd.rec[3 * r], rj = sd.rec[3 * r + 1], rt = sd.rec[3 * r + 2];
     float v;
-    if (rt == REC_VZ) v = row(sd.f[F_VZ], rj, P)[ri];
-    else if (rt == REC_VX) v = row(sd.f[F_VX], rj, P)[ri];
+    auto fld = [&](int c) { return ld1<KF>(rowt((const T*)sd.f[c], rj, P) + ri, sd.fisc[c]); };
+    if (rt == REC_VZ) v = fld(F_VZ);
+    else if (rt == REC_VX) v = fld(F_VX);
     else {
       const bool zero_tzz = fs_on && rj >= sd.pad - 5 && rj <= sd.pad;
-      const float zz = zero_tzz ? 0.f : row(sd.f[F_TZZ], rj, P)[ri];
-      v = (row(sd.f[F_TXX], rj, P)[ri] + zz) * 0.5f;
+      const float zz = zero_tzz ? 0.f : fld(F_TZZ);
+      v = (fld(F_TXX) + zz) * 0.5f;
     }
     sd.gather[(size_t)(k - 1) * sd.n_rec + r] = v;
   }

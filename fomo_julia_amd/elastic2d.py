@@ -147,6 +147,9 @@ class Medium:
     lam: np.ndarray
     mu_txx: np.ndarray
     mu_txz: np.ndarray
+    # device storage of coefficients and wavefields: "f32", "bf16" or "f16"
+    # (FP32 arithmetic in every case); not a field of the reference's Medium
+    storage: str = "f32"
 
 
 @dataclass(eq=False)
@@ -253,10 +256,13 @@ class ShotResult:
 
 # ---------------------------------------------------------------- setup (init.jl)
 def init_medium(vp, vs, rho, dx, dz, nbc: int, fd_order: int, b=None,
-                free_surface: bool = True) -> Medium:
-    """``init_medium`` (utils/init.jl:68-99)."""
+                free_surface: bool = True, storage: str = "f32") -> Medium:
+    """``init_medium`` (utils/init.jl:68-99).  ``storage`` selects how the
+    device keeps the coefficients and wavefields ("f32", "bf16", "f16")."""
     if fd_order < 2 or fd_order % 2:
         raise ValueError(f"Unsupported FD order: {fd_order}")
+    if storage not in _lib.STORAGE:
+        raise ValueError(f"storage must be one of {sorted(_lib.STORAGE)}")
     M = fd_order // 2
     pad = nbc + M
     vp = np.asarray(vp)
@@ -268,7 +274,7 @@ def init_medium(vp, vs, rho, dx, dz, nbc: int, fd_order: int, b=None,
         _pad_array(vp, pad), _pad_array(np.asarray(vs), pad), _pad_array(np.asarray(rho), pad))
     return Medium(nx, nz, F32(dx), F32(dz), x_max, z_max, M, pad, bool(free_surface),
                   _host_f32(rho_vx), _host_f32(rho_vz), _host_f32(lam), _host_f32(mu),
-                  _host_f32(mu_txz))
+                  _host_f32(mu_txz), storage)
 
 
 def init_habc(nx: int, nz: int, nbc: int, dt, dx, dz, v_ref, b=None,
@@ -302,7 +308,8 @@ class _DeviceModel:
         self.a = np.ascontiguousarray(np.asarray(a, dtype=F32))
         d = _lib.ModelDesc(M.nx, M.nz, M.pad, H.nbc, p.fd_order, int(M.is_free_surface),
                            float(M.dx), float(M.dz), float(p.dt), float(p.dtx), float(p.dtz),
-                           float(H.qx), float(H.qz), float(H.qt_x), float(H.qt_z), float(H.qxt), 0)
+                           float(H.qx), float(H.qz), float(H.qt_x), float(H.qt_z), float(H.qxt),
+                           _lib.STORAGE[getattr(M, "storage", "f32")])
         med = [_host_f32(x) for x in (M.rho_vx, M.rho_vz, M.lam, M.mu_txx, M.mu_txz)]
         for x in med:
             if x.shape != (M.nx, M.nz):
@@ -328,7 +335,7 @@ class _DeviceModel:
 
 def _model(b: MI355XBackend, M: Medium, H: HABCConfig, a, p: SimParams) -> _DeviceModel:
     key = (id(M), id(H), np.asarray(a, dtype=F32).tobytes(), float(p.dt), float(p.dtx),
-           float(p.dtz), p.fd_order)
+           float(p.dtz), p.fd_order, getattr(M, "storage", "f32"))
     dm = b._models.get(key)
     if dm is None or dm.M is not M or dm.H is not H:
         dm = _DeviceModel(b, M, H, a, p)
@@ -353,6 +360,12 @@ class _DeviceShot:
                                   ctypes.byref(h))
         dm.b.check(rc, "fomo_shot_create")
         self.h = h
+
+    def field_scales(self) -> np.ndarray:
+        """FP16 store scales of vx vz txx tzz txz (1 for FP32/BF16 storage)."""
+        out = np.zeros(5, dtype=F32)
+        self.dm.b.check(self.dm.b.lib.fomo_shot_field_scales(self.h, _f32ptr(out)), "field_scales")
+        return out
 
     def gather(self) -> np.ndarray:
         out = np.zeros((self.nt, self.n_rec), dtype=F32, order="F")
@@ -397,7 +410,8 @@ class Wavefield:
         return self._h
 
     def _dump(self):
-        return [self.download(c) for c in range(10)]
+        n = 10 if getattr(self._dm.M, "storage", "f32") == "f32" else 5
+        return [self.download(c) for c in range(n)]
 
     def _free(self):
         if self._h is not None:

@@ -56,7 +56,27 @@ extern "C" {
 #define FOMO_FIELD_TZZ_OLD 8
 #define FOMO_FIELD_TXZ_OLD 9
 
+/* Storage of the wavefields and medium coefficients; stencil arithmetic is
+ * FP32 in every case (no counterpart in the reference, which is FP32 only).
+ *   F32   everything FP32 (the reference's arithmetic, bit for bit)
+ *   BF16  coefficients and fields BF16
+ *   F16   coefficients and fields FP16
+ *   F16C  coefficients FP16, fields FP32
+ * Fields are rounded to nearest (ties to even) whenever a stage writes them.
+ * FP16 arrays carry a power-of-two scale (exact): per coefficient array from
+ * its maximum, per field from a bound of the injected wavelet, so values stay
+ * inside FP16's range.  Coefficients are quantized once, with a deterministic
+ * per-cell dither (round up when the remainder exceeds a hash of the cell), so
+ * their rounding errors average out over a wavelength instead of biasing a
+ * homogeneous medium.  Reduced storage needs fd_order 4, 8 or 12 and the fused
+ * step (step mode 1); per-stage calls, the two-pass step and the *_old fields
+ * return FOMO_ERR_UNSUPPORTED.  A wavefield's FP16 state carries the scales
+ * of the shot that produced it: continuing it with a shot of other scales
+ * returns FOMO_ERR_STATE (reset it first). */
 #define FOMO_STORAGE_F32 0
+#define FOMO_STORAGE_BF16 1
+#define FOMO_STORAGE_F16 2
+#define FOMO_STORAGE_F16C 3
 
 typedef struct fomo_ctx fomo_ctx;
 typedef struct fomo_model fomo_model;
@@ -74,7 +94,7 @@ typedef struct fomo_model_desc {
   float dx, dz;         /* Medium.dx, Medium.dz */
   float dt, dtx, dtz;   /* SimParams.dt, .dtx, .dtz (Structures.jl:144-147) */
   float qx, qz, qt_x, qt_z, qxt; /* HABCConfig (Structures.jl:96-100) */
-  int32_t storage;      /* FOMO_STORAGE_F32 */
+  int32_t storage;      /* FOMO_STORAGE_* */
 } fomo_model_desc;
 
 /* ---- context ------------------------------------------------------------ */
@@ -126,6 +146,10 @@ int32_t fomo_shot_create(fomo_model* m, int32_t src_i, int32_t src_j, int32_t sr
                          const int32_t* rec_i, const int32_t* rec_j, const int32_t* rec_type,
                          int32_t n_rec, fomo_shot** out);
 int32_t fomo_shot_destroy(fomo_shot* s);
+/* The power-of-two store scales (vx vz txx tzz txz) FP16 storage uses for this
+ * shot's wavefield: a value x is stored as x * scale.  All 1 for FP32 and BF16
+ * storage.  No counterpart in the reference. */
+int32_t fomo_shot_field_scales(const fomo_shot* s, float* out5);
 /* fill!(rec.data, 0f0) (shot_manager.jl:54). */
 int32_t fomo_shot_reset(fomo_shot* s);
 /* _get_gather (shot_manager.jl:71-72): [nt x n_rec] column-major into out. */

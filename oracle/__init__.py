@@ -32,7 +32,10 @@ class OracleDesc(ctypes.Structure):
                 ("dt", ctypes.c_double), ("dtx", ctypes.c_double), ("dtz", ctypes.c_double),
                 ("qx", ctypes.c_double), ("qz", ctypes.c_double), ("qt_x", ctypes.c_double),
                 ("qt_z", ctypes.c_double), ("qxt", ctypes.c_double),
-                ("flags", ctypes.c_int32), ("threads", ctypes.c_int32)]
+                ("flags", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("storage", ctypes.c_int32), ("fscale", ctypes.c_float * 5)]
+
+STORAGE = {"f32": 0, "bf16": 1, "f16": 2, "f16c": 3}
 
 
 def build() -> str:
@@ -59,7 +62,14 @@ def lib():
                            ctypes.c_int, ctypes.c_int, P, ctypes.c_int]
             fn.restype = ctypes.c_int
         _lib.oracle_max_threads.restype = ctypes.c_int
+        _lib.oracle_round_storage.restype = ctypes.c_float
+        _lib.oracle_round_storage.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float]
     return _lib
+
+
+def round_storage(storage: str, x: float, scale: float = 1.0) -> float:
+    """x rounded to the storage format (at the FP16 scale), as the oracle rounds fields."""
+    return lib().oracle_round_storage(STORAGE[storage], float(x), float(scale))
 
 
 def max_threads() -> int:
@@ -73,19 +83,24 @@ class OracleRun:
     numpy arrays (any order), weights are the (nz, nx) ``HABCConfig`` matrices.
     ``dtype`` np.float32 is the reference's FP32 path; np.float64 evaluates the
     same algorithm in double precision (inputs converted from the given
-    arrays, which the caller may have computed in Float64).
+    arrays, which the caller may have computed in Float64).  ``storage``
+    "bf16" / "f16" (FP32 only) emulates reduced-precision storage of the
+    coefficients and fields; ``fscale`` are the FP16 field store scales.
     """
 
     def __init__(self, *, nx, nz, pad, nbc, fd_order, free_surface, dt, dtx, dtz,
                  q, a, rho_vx, rho_vz, lam, mu_txx, mu_txz, w_vx, w_vz, w_tau,
                  src_i, src_j, src_type, wavelet, rec_i, rec_j, rec_type, nt,
-                 dtype=np.float32, flags=FMA, threads=None):
+                 dtype=np.float32, flags=FMA, threads=None, storage="f32", fscale=None):
         self.dtype = np.dtype(dtype)
         self.nx, self.nz, self.nt = int(nx), int(nz), int(nt)
         self.desc = OracleDesc(int(nx), int(nz), int(pad), int(nbc), int(fd_order),
                                int(bool(free_surface)), float(dt), float(dtx), float(dtz),
                                *[float(v) for v in q], int(flags),
-                               int(threads or max_threads()))
+                               int(threads or max_threads()), STORAGE[storage],
+                               (ctypes.c_float * 5)(*(fscale if fscale is not None else [1.0] * 5)))
+        if storage != "f32" and self.dtype != np.float32:
+            raise ValueError("storage emulation runs on the FP32 oracle only")
         cv = lambda x: np.asfortranarray(np.asarray(x, dtype=self.dtype))
         self.a = np.ascontiguousarray(np.asarray(a, dtype=self.dtype))
         self.med = [cv(x) for x in (rho_vx, rho_vz, lam, mu_txx, mu_txz)]

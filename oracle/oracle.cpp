@@ -29,11 +29,31 @@
 //                    formula per cell (the CUDA kernel without its race);
 //                    otherwise the literal program order of the CPU loops.
 //
+// Reduced-precision storage (desc.storage 1 = BF16, 2 = FP16, 3 = FP16
+// coefficients with FP32 fields; FP32 model only; the reference has no such
+// mode): the storage contract of the MI355X path, restated sequentially.
+//  - The seven per-cell coefficients (k = 0..6: dtx/rho_vx, dtz/rho_vx,
+//    dtx/rho_vz, dtz/rho_vz, lam, lam+2mu, mu_txz) are computed in FP32, scaled
+//    (FP16: by the power of two that puts the array's largest magnitude in
+//    [2^14, 2^15)) and rounded once with a dither: to the grid point below
+//    |x|, plus one spacing when the remainder in spacings exceeds
+//    u = (fmix32(i ^ fmix32(j ^ fmix32(k' + 0x9E3779B9))) >> 8) / 2^24, with
+//    fmix32 MurmurHash3's finalizer and k' = k, except k' = k - 1 for the
+//    dtz arrays when dtz == dtx (they share the dtx arrays).
+//  - After the velocity, HABC and stress stages every cell of the fields the
+//    stage writes is rounded to nearest, ties to even (FP16: at the field's
+//    scale desc.fscale[c]); stages compute in FP32 in between, so the injected
+//    source and the HABC recurrences see unrounded values within a stage.
+// Both roundings are written from the definition of the format's value grid
+// (spacing 2^(e-p+1) for a p-bit significand, the subnormal spacing below
+// the smallest normal exponent).
+//
 // Pinning: see tests/test_oracle.py (polynomial-exactness of the stencils,
 // closed-form HABC cases, free surface, symmetry, 2^k linearity, energy decay,
 // the reference's shot_1.bin gather).
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 #ifdef _OPENMP
@@ -52,6 +72,8 @@ typedef struct {
   double qx, qz, qt_x, qt_z, qxt;
   int32_t flags;
   int32_t threads;
+  int32_t storage;   // 0 = FP32 (no rounding), 1 = BF16, 2 = FP16 (see above)
+  float fscale[5];   // FP16 store scales of vx vz txx tzz txz
 } oracle_desc;
 }
 
@@ -67,7 +89,66 @@ struct Model {
   const T *rho_vx, *rho_vz, *lam, *mu_txx, *mu_txz;
   const T *w_vx, *w_vz, *w_tau;
   T* f[10];  // vx vz txx tzz txz, then the *_old of each
+  int storage = 0;
+  std::vector<T> cq[7];  // rounded coefficients (reduced-precision storage)
 };
+
+// Nearest value, ties to even, of a binary format with a p-bit significand
+// and smallest normal exponent emin (values beyond the largest finite one
+// are not expected here).
+static double round_format(double x, int p, int emin) {
+  if (x == 0 || !std::isfinite(x)) return x;
+  int e;
+  std::frexp(x, &e);  // |x| in [2^(e-1), 2^e)
+  const int ex = std::max(e - 1, emin);
+  const double spacing = std::ldexp(1.0, ex - (p - 1));
+  return std::nearbyint(x / spacing) * spacing;  // default mode: ties to even
+}
+// Field rounding of the storage mode (mode 3 keeps FP32 fields).
+static float round_storage(int storage, float x, float scale) {
+  if (storage == 1) return (float)round_format((double)x, 8, -126);          // BF16
+  if (storage == 2) return (float)(round_format((double)x * scale, 11, -14) / scale);  // FP16
+  return x;
+}
+static uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+// Coefficient k' of cell (i, j) (0-based), dither-rounded (see the header).
+static float dither_coef(int storage, float x, float scale, uint32_t i, uint32_t j, uint32_t kp) {
+  const uint32_t h = fmix32(i ^ fmix32(j ^ fmix32(kp + 0x9E3779B9u)));
+  const double u = (double)(h >> 8) / 16777216.0;
+  const int p = storage == 1 ? 8 : 11, emin = storage == 1 ? -126 : -14;
+  const double v = (double)x * scale;
+  if (v == 0) return 0.f;
+  int e;
+  std::frexp(v, &e);
+  const double spacing = std::ldexp(1.0, std::max(e - 1, emin) - (p - 1));
+  const double a = std::fabs(v) / spacing;
+  const double fl = std::floor(a);
+  const double q = std::copysign((a - fl > u ? fl + 1 : fl) * spacing, v);
+  return (float)(q / scale);
+}
+template <typename T>
+static void round_fields(Model<T>& m, int c0, int c1) {
+  if (m.storage != 1 && m.storage != 2) return;
+  const size_t n = (size_t)m.nx * m.nz;
+  for (int c = c0; c < c1; ++c) {
+    const float sc = m.d->fscale[c];
+    for (size_t o = 0; o < n; ++o) m.f[c][o] = (T)round_storage(m.storage, (float)m.f[c][o], sc);
+  }
+}
+// Power of two putting max |x| in [2^14, 2^15) (FP16 coefficient scale).
+static float coef_scale(const std::vector<float>& x) {
+  double mx = 0;
+  for (float v : x) mx = std::max(mx, (double)std::fabs(v));
+  if (!(mx > 0)) return 1.f;
+  return std::ldexp(1.0f, 14 - (int)std::floor(std::log2(mx)));
+}
 
 template <typename T>
 static inline T madd(bool fm, T a, T b, T c) {  // a*b + c
@@ -135,10 +216,18 @@ static void stage_velocity(Model<T>& m) {
       }
       size_t o = (size_t)i + (size_t)j * nx;
       T bx = m.dtx / m.rho_vx[o], bz = m.dtz / m.rho_vx[o];
+      if (m.storage) {
+        bx = m.cq[0][o];
+        bz = m.cq[1][o];
+      }
       T inc = madd<T>(fm, bz, d2, (T)(bx * d1));
       vx[o] = vx[o] + inc;
       bx = m.dtx / m.rho_vz[o];
       bz = m.dtz / m.rho_vz[o];
+      if (m.storage) {
+        bx = m.cq[2][o];
+        bz = m.cq[3][o];
+      }
       inc = madd<T>(fm, bz, d4, (T)(bx * d3));
       vz[o] = vz[o] + inc;
     }
@@ -167,11 +256,17 @@ static void stage_stress(Model<T>& m) {
       size_t o = (size_t)i + (size_t)j * nx;
       T l_val = m.lam[o], m_val = m.mu_txx[o];
       T l2m = l_val + (T)2 * m_val;
+      T mu_xz = m.mu_txz[o];
+      if (m.storage) {
+        l_val = m.cq[4][o];
+        l2m = m.cq[5][o];
+        mu_xz = m.cq[6][o];
+      }
       T t1 = d1 * m.dtx, t2 = d2 * m.dtz;
       txx[o] = txx[o] + madd<T>(fm, l_val, t2, (T)(l2m * t1));
       tzz[o] = tzz[o] + madd<T>(fm, l2m, t2, (T)(l_val * t1));
       T s = madd<T>(fm, d4, m.dtx, (T)(d3 * m.dtz));
-      txz[o] = txz[o] + (T)(m.mu_txz[o] * s);
+      txz[o] = txz[o] + (T)(mu_xz * s);
     }
   }
 }
@@ -346,6 +441,32 @@ static Model<T> make_model(const oracle_desc* d, const T* a, const T* const* med
   m.w_tau = w[2];
   size_t n = (size_t)d->nx * d->nz;
   for (int c = 0; c < 10; ++c) m.f[c] = fields + c * n;
+  m.storage = d->storage;
+  if (m.storage) {
+    std::vector<float> c(n);
+    for (int k = 0; k < 7; ++k) {
+      for (size_t o = 0; o < n; ++o) {
+        float v;
+        switch (k) {
+          case 0: v = (float)m.dtx / (float)m.rho_vx[o]; break;
+          case 1: v = (float)m.dtz / (float)m.rho_vx[o]; break;
+          case 2: v = (float)m.dtx / (float)m.rho_vz[o]; break;
+          case 3: v = (float)m.dtz / (float)m.rho_vz[o]; break;
+          case 4: v = (float)m.lam[o]; break;
+          case 5: v = (float)m.lam[o] + 2.0f * (float)m.mu_txx[o]; break;
+          default: v = (float)m.mu_txz[o]; break;
+        }
+        c[o] = v;
+      }
+      const float sc = m.storage >= 2 ? coef_scale(c) : 1.f;
+      const bool alias = (k == 1 || k == 3) && (float)m.dtx == (float)m.dtz;
+      const uint32_t kp = alias ? k - 1 : k;
+      m.cq[k].resize(n);
+      for (size_t o = 0; o < n; ++o)
+        m.cq[k][o] = (T)dither_coef(m.storage, c[o], sc, (uint32_t)(o % m.nx),
+                                    (uint32_t)(o / m.nx), kp);
+    }
+  }
   return m;
 }
 
@@ -356,14 +477,27 @@ static int run_steps(const oracle_desc* d, const T* a, const T* const* med, cons
                      int n_steps, T* gather, int stage_mask) {
   if (d->nx < 2 * (d->nbc + 3) || d->nz < 2 * (d->nbc + 3)) return -1;
   if (d->fd_order < 2 || d->fd_order % 2) return -2;
+  if (d->storage && sizeof(T) != sizeof(float)) return -3;
   Model<T> m = make_model<T>(d, a, med, w, fields);
   for (int k = k_first; k < k_first + n_steps; ++k) {
     if (stage_mask & 1) stage_backup(m);
     if (stage_mask & 2) stage_inject(m, src_i, src_j, src_type, wavelet, nw, k);
-    if (stage_mask & 4) stage_velocity(m);
-    if (stage_mask & 8) stage_habc(m, 0);
-    if (stage_mask & 16) stage_stress(m);
-    if (stage_mask & 32) stage_habc(m, 1);
+    if (stage_mask & 4) {
+      stage_velocity(m);
+      round_fields(m, 0, 2);
+    }
+    if (stage_mask & 8) {
+      stage_habc(m, 0);
+      round_fields(m, 0, 2);
+    }
+    if (stage_mask & 16) {
+      stage_stress(m);
+      round_fields(m, 2, 5);
+    }
+    if (stage_mask & 32) {
+      stage_habc(m, 1);
+      round_fields(m, 2, 5);
+    }
     if (stage_mask & 64) stage_fs(m);
     if (stage_mask & 128) stage_record(m, ri, rj, rt, n_rec, nt, k, gather);
   }
@@ -394,6 +528,11 @@ int oracle_run_f64(const oracle_desc* d, const double* a, const double* const* m
                    int n_rec, int nt, int k_first, int n_steps, double* gather, int stage_mask) {
   return run_steps<double>(d, a, med, w, fields, src_i, src_j, src_type, wavelet, nw, ri, rj, rt,
                            n_rec, nt, k_first, n_steps, gather, stage_mask);
+}
+
+// One value rounded as the storage holds it (for the rounding's own tests).
+float oracle_round_storage(int storage, float x, float scale) {
+  return round_storage(storage, x, scale);
 }
 
 int oracle_max_threads(void) {

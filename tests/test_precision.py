@@ -1,0 +1,226 @@
+"""Reduced-precision storage (BF16, FP16, FP16 coefficients with FP32 fields).
+
+The reference is FP32 only, so parity here has two parts:
+  * the kernels against the oracle's sequential restatement of the storage
+    contract (coefficients dither-quantized once, fields rounded to nearest
+    even whenever a stage writes them): bit for bit;
+  * the accuracy of each storage mode against the oracle's Float64 run of the
+    same scheme: relative L2 and peak-amplitude error per receiver component
+    (vx, vz, pressure), bounds written below.
+
+The rounding functions the oracle uses are pinned against numpy's float16 and
+torch's bfloat16 conversions (both round to nearest, ties to even)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from fomo_julia_amd import elastic2d as E
+from tests.helpers import C1Case, Case, rel_l2
+
+F32 = np.float32
+
+# Relative L2 bounds of the C1 gathers against Float64 (measured values in the
+# comments; nt = 1000).
+C1_BOUNDS = {"f32": 1e-5,    # ~1.0e-6
+             "f16c": 1e-3,   # ~8.2e-4 (vx), 5.3e-4 (vz), 5.6e-4 (p)
+             "f16": 4e-3,    # ~2.7e-3 (vx), 1.9e-3 (vz), 3.0e-3 (p)
+             "bf16": 3e-2}   # ~2.2e-2 (vx), 1.7e-2 (vz), 2.4e-2 (p)
+
+
+def _values(rng, n):
+    """Floats across FP16's normal and subnormal ranges, plus exact ties."""
+    mags = np.exp2(rng.uniform(-26, 15.9, n))
+    x = (mags * rng.choice([-1.0, 1.0], n)).astype(F32)
+    # halfway points between neighbouring FP16 values (ties go to even)
+    h = np.float16(rng.uniform(-1000, 1000, 200)).astype(np.float64)
+    nxt = np.nextafter(np.float16(h), np.float16(np.inf)).astype(np.float64)
+    ties = ((h + nxt) / 2).astype(F32)
+    return np.concatenate([x, ties, np.array([0.0, -0.0, 6.1e-5, 5.96e-8, 65504.0], F32)])
+
+
+def test_f16_rounding_matches_numpy():
+    O.build()
+    rng = np.random.default_rng(1)
+    for scale in (1.0, 2.0 ** 12, 2.0 ** -20):
+        for x in _values(rng, 3000):
+            if abs(float(x) * scale) >= 65504.0:  # beyond FP16's range: not stored
+                continue
+            want = np.float64(np.float16(np.float32(x * scale))) / scale
+            got = O.round_storage("f16", float(x), scale)
+            assert got == np.float32(want), (x, scale, got, want)
+
+
+def test_bf16_rounding_matches_torch_and_bits():
+    torch = pytest.importorskip("torch")
+    O.build()
+    rng = np.random.default_rng(2)
+    x = np.concatenate([_values(rng, 3000),
+                        (np.exp2(rng.uniform(-140, 100, 2000)) *
+                         rng.choice([-1.0, 1.0], 2000)).astype(F32)])
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    u = x.view(np.uint32).astype(np.uint64)
+    bits = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(F32)
+    assert np.array_equal(want, bits)
+    got = np.array([O.round_storage("bf16", float(v)) for v in x], F32)
+    assert np.array_equal(got, want)
+
+
+def test_emulated_storage_accuracy_c1_cpu():
+    """The oracle's storage emulation on the homogeneous 200 x 200 case: FP16
+    coefficients with FP32 fields stay within 1e-3 of Float64 because the
+    coefficients are dithered (rounding them to nearest instead gives a
+    systematic ~3e-3 error in a homogeneous medium)."""
+    O.build()
+    c = C1Case()
+    r64 = c.oracle(np.float64)
+    r64.run(1, c.nt)
+    g64 = c.split(r64.gather_nt_by_nrec())
+    r = c.oracle(storage="f16c")
+    r.run(1, c.nt)
+    g = c.split(r.gather_nt_by_nrec())
+    for t in g:
+        assert rel_l2(g[t], g64[t]) < C1_BOUNDS["f16c"], t
+
+
+# ---------------------------------------------------------------- GPU
+gpu = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def b():
+    bk = E.backend("mi355x")
+    bk.set_tiling(1, 0, 16)
+    bk.set_step_mode(1)
+    return bk
+
+
+def _shot(b, c):
+    dm = E._model(b, c.M, c.H, c.a, c.p)
+    return E._shot_for(dm, c.src, c.rec, c.nt)
+
+
+def _run_and_emulate(b, c, storage):
+    W = E.Wavefield(c.M.nx, c.M.nz, b)
+    res = E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p)
+    scales = _shot(b, c).field_scales()
+    r = c.oracle(storage=storage, fscale=[float(s) for s in scales])
+    r.run(1, c.nt)
+    return res, W, r, scales
+
+
+def _assert_same(res, W, r, what):
+    g, want = res.gather, r.gather_nt_by_nrec()
+    if not np.array_equal(g, want):
+        bad = np.argwhere(g != want)
+        k, q = bad[0]
+        raise AssertionError(f"{what}: gather differs in {len(bad)} samples, first (k={k}, r={q}) "
+                             f"got {g[k, q]!r} want {want[k, q]!r}")
+    for f in range(5):
+        got, ref = W.download(f), r.field(f)
+        if not np.array_equal(got, ref):
+            bad = np.argwhere(got != ref)
+            i, j = bad[0]
+            raise AssertionError(f"{what}: field {f} differs in {len(bad)} cells, first (i={i}, j={j}) "
+                                 f"got {got[i, j]!r} want {ref[i, j]!r}")
+
+
+@gpu
+@pytest.mark.parametrize("storage", ["bf16", "f16", "f16c"])
+@pytest.mark.parametrize("fd_order,fs,src_type", [(8, True, "pressure"), (4, False, "force_z"),
+                                                  (12, True, "force_x")])
+def test_reduced_storage_bit_exact_vs_emulation(b, storage, fd_order, fs, src_type):
+    c = Case(fd_order=fd_order, fs=fs, src_type=src_type, nt=120, water=fs, storage=storage)
+    print(f"case storage={storage} order={fd_order} fs={fs} src={src_type}", flush=True)
+    res, W, r, scales = _run_and_emulate(b, c, storage)
+    assert all(float(s) == 2.0 ** np.round(np.log2(float(s))) for s in scales)
+    assert np.abs(res.gather).max() > 0
+    _assert_same(res, W, r, storage)
+
+
+@gpu
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("cpt,tz", [(2, 17), (4, 0)])
+def test_reduced_storage_tilings(b, storage, cpt, tz):
+    c = Case(nx_in=150, nz_in=90, nbc=20, nt=71, storage=storage)
+    print(f"case storage={storage} cpt={cpt} tz={tz}", flush=True)
+    b.set_tiling(cpt, tz, 6)
+    try:
+        res, W, r, _ = _run_and_emulate(b, c, storage)
+    finally:
+        b.set_tiling(1, 0, 16)
+    _assert_same(res, W, r, f"{storage} cpt={cpt}")
+
+
+@gpu
+@pytest.mark.parametrize("storage", ["f32", "f16c", "f16", "bf16"])
+def test_c1_accuracy_vs_float64(b, storage):
+    """C1 with each storage: bit-exact against the emulation and within the
+    stated relative L2 of the Float64 oracle, per component."""
+    c = C1Case(storage=storage)
+    print(f"case C1 storage={storage}", flush=True)
+    res, W, r, _ = _run_and_emulate(b, c, storage)
+    if storage != "f32":
+        _assert_same(res, W, r, storage)
+    r64 = c.oracle(np.float64)
+    r64.run(1, c.nt)
+    g64 = c.split(r64.gather_nt_by_nrec())
+    g = c.split(res.gather)
+    for t in g:
+        e = rel_l2(g[t], g64[t])
+        peak = float(np.abs(g[t] - g64[t]).max() / np.abs(g64[t]).max())
+        print(f"  {storage} {t}: rel L2 {e:.3e}, peak {peak:.3e}", flush=True)
+        assert e < C1_BOUNDS[storage], (t, e)
+
+
+@gpu
+def test_reduced_batch_uneven_matches_single(b):
+    cases = [Case(nx_in=90, nz_in=60, nt=50, storage="bf16", seed=1),
+             Case(nx_in=130, nz_in=45, nt=50, storage="bf16", seed=2, src_type="force_z")]
+    singles = []
+    for c in cases:
+        W = E.Wavefield(c.M.nx, c.M.nz, b)
+        singles.append(E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p).gather)
+    lib = b.lib
+    import ctypes
+    hs = [(E._model(b, c.M, c.H, c.a, c.p), c) for c in cases]
+    ws, shots, models, keep = [], [], [], []
+    for dm, c in hs:
+        W = E.Wavefield(c.M.nx, c.M.nz, b)
+        W._bind(dm)
+        E.reset(b, W)
+        ds = E._DeviceShot(dm, c.src, c.rec, c.nt)
+        keep += [W, ds]
+        ws.append(W._h)
+        shots.append(ds.h)
+        models.append(dm.h)
+    arr = lambda v: (ctypes.c_void_p * len(v))(*[x.value for x in v])
+    hb = ctypes.c_void_p()
+    b.check(lib.fomo_batch_create(b.ctx, len(cases), arr(models), arr(ws), arr(shots),
+                                  ctypes.byref(hb)), "batch_create")
+    b.check(lib.fomo_batch_run(hb, 1, cases[0].nt), "batch_run")
+    b.synchronize()
+    for q, want in enumerate(singles):
+        assert np.array_equal(keep[2 * q + 1].gather(), want)
+    lib.fomo_batch_destroy(hb)
+
+
+@gpu
+def test_reduced_storage_rejections(b):
+    c = Case(nt=20, storage="bf16")
+    W = E.Wavefield(c.M.nx, c.M.nz, b)
+    E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p)
+    with pytest.raises(E._lib.FomoError):
+        E.update_velocity(b, W)
+    with pytest.raises(E._lib.FomoError):
+        W.download("vx_old")
+    odd = Case(nt=20, fd_order=6, storage="f16")
+    W2 = E.Wavefield(odd.M.nx, odd.M.nz, b)
+    with pytest.raises(E._lib.FomoError):
+        E.run_shot(b, W2, odd.M, odd.H, odd.a, odd.src, odd.rec, odd.p)
+    # an FP16 state continued with a shot whose scales differ
+    c16 = Case(nt=20, storage="f16")
+    W3 = E.Wavefield(c16.M.nx, c16.M.nz, b)
+    E.run_shot(b, W3, c16.M, c16.H, c16.a, c16.src, c16.rec, c16.p)
+    loud = E.Source(c16.src.i, c16.src.j, c16.src.wavelet * F32(1024.0), c16.src.type)
+    with pytest.raises(E._lib.FomoError):
+        E.run_time_loop(b, W3, c16.M, c16.H, c16.a, loud, c16.rec, c16.p)
