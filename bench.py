@@ -35,13 +35,15 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per padded cell-update (FP32, dx == dz so dt/dx/rho and
 # dt/dz/rho are one array each):
 BYTES_FUSED = 60     # read 5 old fields + dt/rho_vx, dt/rho_vz, lam, lam+2mu, mu_txz; write 5 fields
+# the same 15 array accesses per cell with 2-byte coefficients and/or fields
+BYTES_FUSED_STORAGE = {"f32": 60, "bf16": 30, "f16": 30, "f16c": 50}
 BYTES_VELOCITY = 36  # two-pass velocity: read txx tzz txz, 2 buoyancies; read+write vx vz
 BYTES_STRESS = 44    # two-pass stress: read vx vz lam lam+2mu mu_txz; read+write txx tzz txz
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fused_seam_traffic.txt")
 METRIC = "Mcell-updates/s & %HBM roofline, 1 GPU; shots/hr + p99 latency (served)"
 
 
-def build_seam(nt: int, nbc=50, fd_order=8, with_weights=False):
+def build_seam(nt: int, nbc=50, fd_order=8, with_weights=False, storage="f32"):
     from fomo_julia_amd import elastic2d as E
     from fomo_julia_amd import init as I
     from fomo_julia_amd import media
@@ -49,7 +51,7 @@ def build_seam(nt: int, nbc=50, fd_order=8, with_weights=False):
     dx = F32(12.5)
     vp, vs, rho = media.seam_like()
     dt = media.cfl_dt(vp, dx, 0.4)
-    M = E.init_medium(vp, vs, rho, dx, dx, nbc, fd_order, None, free_surface=True)
+    M = E.init_medium(vp, vs, rho, dx, dx, nbc, fd_order, None, free_surface=True, storage=storage)
     H = E.init_habc(M.nx, M.nz, nbc, dt, dx, dx, I.julia_mean_f32(vp), None, weights=with_weights)
     p = E.SimParams(dt, nt, dx, dx, fd_order)
     a = I.get_fd_coefficients(fd_order)
@@ -92,6 +94,8 @@ def main():
     ap.add_argument("--stage-reps", type=int, default=50)
     ap.add_argument("--tiling", default="", help="cols_per_lane,rows_per_tile,steps_per_graph")
     ap.add_argument("--two-pass", action="store_true", help="time the two-pass in-place step instead")
+    ap.add_argument("--storage", default="f32", choices=sorted(BYTES_FUSED_STORAGE),
+                    help="device storage of coefficients/fields (arithmetic stays FP32)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -103,7 +107,9 @@ def main():
 
     nt = max(args.steps, args.warmup, 1)
     t_setup = time.perf_counter()
-    case = build_seam(nt)
+    if args.two_pass and args.storage != "f32":
+        sys.exit("bench.py: the two-pass step exists for FP32 storage only")
+    case = build_seam(nt, storage=args.storage)
     M, H, p, src, rec = case["M"], case["H"], case["p"], case["src"], case["rec"]
     t_setup = time.perf_counter() - t_setup
     cells = M.nx * M.nz
@@ -162,7 +168,8 @@ def main():
         kernels = {"fused": main_ms, "band_stress": stage_ms(6, reps),
                    "habc_velocity": stage_ms(7, reps), "habc_stress": stage_ms(8, reps),
                    "fs_record": stage_ms(4, reps)}
-        main_bytes, main_name = BYTES_FUSED, "k_fused (60 B/cell-update x padded cells per launch)"
+        main_bytes = BYTES_FUSED_STORAGE[args.storage]
+        main_name = f"k_fused ({main_bytes} B/cell-update x padded cells per launch)"
     else:
         main_ms = stage_ms(1, reps)
         kernels = {"velocity": stage_ms(0, reps), "stress": main_ms,
@@ -170,10 +177,10 @@ def main():
                    "fs_record": stage_ms(4, reps)}
         main_bytes, main_name = BYTES_STRESS, "k_stress (44 B/cell-update x padded cells per launch)"
     main_gbs = main_bytes * cells / (main_ms / 1e3) / 1e9
-    step_bytes = BYTES_FUSED if fused else BYTES_VELOCITY + BYTES_STRESS
+    step_bytes = main_bytes if fused else BYTES_VELOCITY + BYTES_STRESS
     step_gbs = step_bytes * cells / (ms_per_step / 1e3) / 1e9
     traffic = None
-    if fused and os.path.exists(TRAFFIC_FILE):
+    if fused and args.storage == "f32" and os.path.exists(TRAFFIC_FILE):
         for line in open(TRAFFIC_FILE):
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = int(line.split(":")[1])
@@ -204,8 +211,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
+        "storage": args.storage,
         "data": "synthetic (seeded SEAM-like medium)",
-        "config": {"workload": "SEAM-shaped single shot, FP32: nx=4444 nz=3819 dx=dz=12.5 m, "
+        "config": {"workload": f"SEAM-shaped single shot, {args.storage.upper()} storage, FP32 "
+                               "arithmetic: nx=4444 nz=3819 dx=dz=12.5 m, "
                                "fd_order=8, nbc=50, free surface, padded 4552x3927, "
                                f"nt={args.steps}, 10 Hz Ricker pressure source at the surface "
                                "centre, 4444 vz receivers",

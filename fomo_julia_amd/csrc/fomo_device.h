@@ -89,6 +89,16 @@ struct ShotDev {
   float* gather;        // [nt][n_rec]
   int nt;
   int* kp;              // device step counter: step of k_off == 0 (1-based)
+  // Fused step, buffer addressing: byte ranges holding the five read fields
+  // (bufA), the five written fields (bufB) and the coefficient blocks (bufC),
+  // each from the start of its first block.  Field c starts at
+  // c * fstr_b + forg_b of its buffer; coefficient block q at q * cstr_b +
+  // corg_b (blocks: dt/rho_vx, dt/rho_vz, lam, lam+2mu, mu_txz, then the
+  // dtz arrays when dtz != dtx).  recF / recC: the ranges' lengths (< 2^31).
+  const void* bufA;
+  void* bufB;
+  const void* bufC;
+  uint32_t fstr_b, forg_b, cstr_b, corg_b, recF, recC;
 };
 
 struct BatchArgs {

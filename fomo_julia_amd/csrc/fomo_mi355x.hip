@@ -49,6 +49,7 @@ struct fomo_model {
   void* dcoef = nullptr;
   const float* coef[7] = {};  // bvx_x bvx_z bvz_x bvz_z lam l2m mu_txz (element type per storage)
   float csc[7] = {1, 1, 1, 1, 1, 1, 1};  // load scales 2^-e of the coefficient arrays
+  int ncoef_blocks = 5;
   float zmin = 0.f, zmax = 0.f;          // P impedance sqrt((lam+2mu) rho) extremes
   float* dramp = nullptr;
   std::vector<float> rho_vx, rho_vz;  // host copies, for force-source scaling
@@ -102,8 +103,10 @@ enum Table { TBL_IN0 = 0, TBL_IN1, TBL_FUSED0, TBL_FUSED1, TBL_COUNT };
 struct fomo_batch {
   fomo_ctx* ctx = nullptr;
   int n = 0, M = 0, cpt = 2, tz = 32, graph_steps = 16, step_mode = 1;
+  int step_mode_req = 1;  // the context's step mode when built (step_mode may fall back to 0)
   int tz_req = 0;  // the context's rows_per_tile when built (0 = auto)
   int storage = FOMO_STORAGE_F32;
+  bool same = true;  // dtx == dtz for every member
   std::vector<fomo_wavefield*> ws;
   std::vector<fomo_shot*> ss;
   std::vector<ShotDev> hs;  // [TBL_COUNT][n]
@@ -390,6 +393,7 @@ extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, cons
   const size_t fs = field_stride(nz, P);
   const int nblocks = same ? 5 : 7;  // dtx == dtz: the z-scaled arrays alias the x-scaled ones
   const int es = m->es;
+  m->ncoef_blocks = nblocks;
   m->pitch = P;
   m->fstride = fs;
   // P impedance range, for the field scales of reduced-precision storage
@@ -779,6 +783,18 @@ static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const 
   sd.gather = s->dgather;
   sd.nt = s->nt;
   sd.kp = s->dk;
+  // fused step, buffer addressing (see ShotDev)
+  const size_t org = (size_t)GUARD_ROWS * m->pitch + SLACK;
+  const size_t fstr = m->fstride * m->fes, cstr = m->fstride * m->es;
+  sd.fstr_b = (uint32_t)fstr;
+  sd.forg_b = (uint32_t)(org * m->fes);
+  sd.cstr_b = (uint32_t)cstr;
+  sd.corg_b = (uint32_t)(org * m->es);
+  sd.recF = (uint32_t)std::min<size_t>(5 * fstr, 0x7FFFFFFF);
+  sd.recC = (uint32_t)std::min<size_t>((size_t)m->ncoef_blocks * cstr, 0x7FFFFFFF);
+  sd.bufA = sd.fo[0] ? (const char*)sd.fo[0] - sd.forg_b : nullptr;
+  sd.bufB = sd.f[0] ? (char*)sd.f[0] - sd.forg_b : nullptr;
+  sd.bufC = m->dcoef;
   return sd;
 }
 
@@ -804,12 +820,13 @@ template <int K, int M, int CPT>
 static int stencil_slots_mc(int cus, int mode) {
   int bv = 0, bs = 0;
   if (mode == 1 || K != ST_F32) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused<M, CPT, WPB, K>, 64 * WPB, 0) !=
-        hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused<M, CPT, WPB, K, true>, 64 * WPB,
+                                                     0) != hipSuccess)
       bv = 1;
     bs = bv;
   } else if constexpr (K == ST_F32) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_velocity<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_velocity<M, CPT, WPB, true>, 64 * WPB, 0) !=
+        hipSuccess)
       bv = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bs, k_stress<M, CPT, WPB>, 64 * WPB, 0) != hipSuccess)
       bs = 1;
@@ -848,8 +865,10 @@ static int stencil_slots(int storage, int M, int cpt, int cus, int mode) {
 
 // Rows per stencil tile so that the tiles of all shots fill the resident
 // wavefront slots in whole rounds (no tail round), within [8, 64] rows when
-// the grid allows.
-static int auto_tz(const std::vector<int>& tiles_x, const std::vector<int>& nz, int slots) {
+// the grid allows.  `block` > 0: the fused kernel marches tz + 2M rows in
+// whole blocks of `block` rows, so tz + 2M should be a multiple of it.
+static int auto_tz(const std::vector<int>& tiles_x, const std::vector<int>& nz, int slots,
+                   int M, int block) {
   auto items = [&](int tz) {
     int64_t t = 0;
     for (size_t i = 0; i < nz.size(); ++i) t += (int64_t)tiles_x[i] * ((nz[i] + tz - 1) / tz);
@@ -861,8 +880,10 @@ static int auto_tz(const std::vector<int>& tiles_x, const std::vector<int>& nz, 
     const int64_t it = items(tz);
     const int64_t rounds = (it + slots - 1) / slots;
     // busy fraction of the last round, discounted by the z-halo re-read cost
+    // (two-pass) or by the rows the march computes per stored row (fused)
     const double fill = (double)it / (double)(rounds * slots);
-    const double eff = fill * (double)tz / (double)(tz + 4);
+    const double march = block > 0 ? (double)((tz + 2 * M + block - 1) / block * block) : tz + 4.0;
+    const double eff = fill * (double)tz / march;
     if (eff > best_eff + 1e-9) {
       best_eff = eff;
       best = tz;
@@ -894,11 +915,24 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
   b->tz_req = c->tz;
   b->graph_steps = c->graph_steps;
   b->step_mode = c->step_mode;
+  b->step_mode_req = c->step_mode;
   if (!ms[0]) return fail(c, FOMO_ERR_ARG, "NULL batch member");
   b->M = ms[0]->M;
   b->storage = ms[0]->storage;
   if (b->storage != FOMO_STORAGE_F32 && b->step_mode != 1)
     return fail(c, FOMO_ERR_UNSUPPORTED, "BF16/FP16 storage runs the fused step only (step mode 1)");
+  // The fused kernel addresses each state buffer (5 fields) and the
+  // coefficient blocks as byte ranges below 2 GiB; larger grids take the
+  // two-pass step (FP32) or are refused (reduced storage).
+  for (int i = 0; i < n && b->step_mode == 1; ++i) {
+    if (!ms[i]) break;
+    const size_t fb = 5 * ms[i]->fstride * ms[i]->fes, cb = (size_t)ms[i]->ncoef_blocks * ms[i]->fstride * ms[i]->es;
+    if (fb >= 0x7FFFFFFF || cb >= 0x7FFFFFFF) {
+      if (b->storage != FOMO_STORAGE_F32)
+        return fail(c, FOMO_ERR_UNSUPPORTED, "grid too large for the fused step with reduced storage");
+      b->step_mode = 0;
+    }
+  }
   for (int i = 0; i < n; ++i) {
     if (!ms[i] || !ws[i] || !ss[i]) return fail(c, FOMO_ERR_ARG, "NULL batch member");
     if (ms[i]->ctx != c || ws[i]->m != ms[i] || ss[i]->m != ms[i])
@@ -910,6 +944,7 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
       if (ws[j] == ws[i]) return fail(c, FOMO_ERR_ARG, "a wavefield appears twice in the batch");
     b->ws.push_back(ws[i]);
     b->ss.push_back(ss[i]);
+    if (ms[i]->d.dtx != ms[i]->d.dtz) b->same = false;
   }
   if (b->tz == 0) {
     std::vector<int> tx, nzs;
@@ -919,7 +954,8 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
       tx.push_back((ms[i]->d.nx + w - 1) / w);
       nzs.push_back(ms[i]->d.nz);
     }
-    b->tz = auto_tz(tx, nzs, stencil_slots(b->storage, b->M, b->cpt, c->cus, b->step_mode));
+    b->tz = auto_tz(tx, nzs, stencil_slots(b->storage, b->M, b->cpt, c->cus, b->step_mode), b->M,
+                    b->step_mode == 1 ? 2 * b->M + 2 : 0);
   }
   b->hs.resize((size_t)TBL_COUNT * n);
   for (int t = 0; t < TBL_COUNT; ++t)
@@ -974,7 +1010,7 @@ static int single_batch(fomo_model* m, fomo_wavefield* w, fomo_shot* s, fomo_bat
   const int q = (s == w->placeholder) ? 0 : 1;
   fomo_batch* b = w->single[q];
   if (b && w->single_id[q] == s->id && b->cpt == c->cpt && b->tz_req == c->tz &&
-      b->graph_steps == c->graph_steps && b->step_mode == c->step_mode) {
+      b->graph_steps == c->graph_steps && b->step_mode_req == c->step_mode) {
     *out = b;
     return FOMO_OK;
   }
@@ -1009,16 +1045,24 @@ static void launch_stencil_mc(const fomo_batch* b, int table, int kind, int flag
   dim3 block(64 * WPB);
   hipStream_t st = b->ctx->stream;
   if (kind == 2) {  // grid rounded up to whole XCD rounds (see xcd_block)
-    hipLaunchKernelGGL((k_fused<M, CPT, WPB, K>), dim3(round_up(blocks_for(b, PART_FUSED), NUM_XCD)),
-                       block, 0, st, args_for(b, table, PART_FUSED, k_off), flags);
+    const dim3 grid(round_up(blocks_for(b, PART_FUSED), NUM_XCD));
+    if (b->same)
+      hipLaunchKernelGGL((k_fused<M, CPT, WPB, K, true>), grid, block, 0, st,
+                         args_for(b, table, PART_FUSED, k_off), flags);
+    else
+      hipLaunchKernelGGL((k_fused<M, CPT, WPB, K, false>), grid, block, 0, st,
+                         args_for(b, table, PART_FUSED, k_off), flags);
   } else if (kind == 3) {
     if constexpr (CPT == 1)  // one instance per (K, M)
       hipLaunchKernelGGL((k_band<M, WPB, K>), dim3(blocks_for(b, PART_BAND)), block, 0, st,
                          args_for(b, table, PART_BAND, k_off), flags);
   } else if constexpr (K == ST_F32) {
-    if (kind == 0)
-      hipLaunchKernelGGL((k_velocity<M, CPT, WPB>), dim3(blocks_for(b, PART_STENCIL)), block, 0, st,
-                         args_for(b, table, PART_STENCIL, k_off), flags);
+    if (kind == 0 && b->same)
+      hipLaunchKernelGGL((k_velocity<M, CPT, WPB, true>), dim3(blocks_for(b, PART_STENCIL)), block, 0,
+                         st, args_for(b, table, PART_STENCIL, k_off), flags);
+    else if (kind == 0)
+      hipLaunchKernelGGL((k_velocity<M, CPT, WPB, false>), dim3(blocks_for(b, PART_STENCIL)), block, 0,
+                         st, args_for(b, table, PART_STENCIL, k_off), flags);
     else
       hipLaunchKernelGGL((k_stress<M, CPT, WPB>), dim3(blocks_for(b, PART_STENCIL)), block, 0, st,
                          args_for(b, table, PART_STENCIL, k_off), flags);
@@ -1267,7 +1311,7 @@ extern "C" int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot
                                      int32_t k, int32_t reps) {
   if (!m || !w || !s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   if (which < 0 || which > 8 || reps < 0) return fail(m->ctx, FOMO_ERR_ARG, "bad stage or reps");
-  if (which < 5 && m->storage != FOMO_STORAGE_F32)
+  if (which < 4 && m->storage != FOMO_STORAGE_F32)
     return fail(m->ctx, FOMO_ERR_UNSUPPORTED, "the in-place stages need FP32 storage");
   fomo_batch* b = nullptr;
   int rc = single_batch(m, w, s, &b);

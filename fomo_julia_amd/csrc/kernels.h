@@ -237,6 +237,58 @@ __device__ __forceinline__ void st1(typename Store<K>::T* p, float x, float s) {
   else *(gu16*)p = (uint16_t)to_bits<K>(x * s);
 }
 
+// ---- buffer addressing: a descriptor (base + length in SGPRs), a per-lane
+// byte offset and a uniform byte offset.  Loads past the length return 0 and
+// stores past it are dropped, so a lane is masked by giving it offset OOB:
+// every load and store stays unconditional, which keeps the compiler's
+// memory-counter waits exact across rows.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr uint32_t OOB = 0x80000000u;  // >= every buffer length used (< 2^31)
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <int K, int N>
+__device__ __forceinline__ V<N> bload(Rsrc r, uint32_t vo, uint32_t so, float s) {
+  V<N> x;
+  if constexpr (K == ST_F32) {
+    if constexpr (N == 1) {
+      x.v[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    } else if constexpr (N == 2) {
+      // elements go through a scalar: clang's __builtin_bit_cast of an
+      // ext-vector element reads element 0 whatever the index
+      const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0);
+      x.v[0] = __uint_as_float((uint32_t)w[0]);
+      x.v[1] = __uint_as_float((uint32_t)w[1]);
+    } else {
+      const auto w = __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) x.v[c] = __uint_as_float((uint32_t)w[c]);
+    }
+  } else {
+    if constexpr (N == 1) {
+      x.v[0] = from_bits<K>(__builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0)) * s;
+    } else if constexpr (N == 2) {
+      const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0);
+      x.v[0] = from_bits<K>(w & 0xFFFFu) * s;
+      x.v[1] = from_bits<K>(w >> 16) * s;
+    } else {
+      const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0);
+      x.v[0] = from_bits<K>(w[0] & 0xFFFFu) * s;
+      x.v[1] = from_bits<K>(w[0] >> 16) * s;
+      x.v[2] = from_bits<K>(w[1] & 0xFFFFu) * s;
+      x.v[3] = from_bits<K>(w[1] >> 16) * s;
+    }
+  }
+  return x;
+}
+template <int K>
+__device__ __forceinline__ void bstore(Rsrc r, uint32_t vo, uint32_t so, float x, float s) {
+  if constexpr (K == ST_F32)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, vo, so, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)to_bits<K>(x * s), r, vo, so, 0);
+}
+
 // x as the storage holds it (stored with scale s, read back with is = 1/s).
 template <int K>
 __device__ __forceinline__ float as_stored(float x, float s, float is) {
@@ -290,7 +342,7 @@ __device__ __forceinline__ int shot_step(const ShotDev& sd, int k_off) {
 // only; the stress kernel adds it to memory, so every reader sees
 // txx + w exactly once.
 // ---------------------------------------------------------------------------
-template <int M, int CPT, int WPB>
+template <int M, int CPT, int WPB, bool SAME>
 __global__ __launch_bounds__(64 * WPB) void k_velocity(BatchArgs ba, int flags) {
   constexpr int TW = 64 * CPT;
   constexpr int LW = TW + 2 * M;
@@ -334,7 +386,6 @@ __global__ __launch_bounds__(64 * WPB) void k_velocity(BatchArgs ba, int flags) 
     const float* b2p = uniform_p(sd.bvx_z);
     const float* b3p = uniform_p(sd.bvz_x);
     const float* b4p = uniform_p(sd.bvz_z);
-    const bool bsame = (b1p == b2p) && (b3p == b4p);
     const int nw = uniform(sd.nw), stype = uniform(sd.stype);
     const bool src_on = (flags & FLAG_INJECT) && k <= nw;
     const float wav = src_on ? uniform_f(uniform_p(sd.wavelet)[k - 1]) : 0.f;
@@ -368,7 +419,7 @@ __global__ __launch_bounds__(64 * WPB) void k_velocity(BatchArgs ba, int flags) 
     V<CPT> c_b1 = ldv<CPT>(row(b1p, j0, P) + ib);
     V<CPT> c_b3 = ldv<CPT>(row(b3p, j0, P) + ib);
     V<CPT> c_b2 = c_b1, c_b4 = c_b3;
-    if (!bsame) {
+    if constexpr (!SAME) {
       c_b2 = ldv<CPT>(row(b2p, j0, P) + ib);
       c_b4 = ldv<CPT>(row(b4p, j0, P) + ib);
     }
@@ -386,7 +437,7 @@ __global__ __launch_bounds__(64 * WPB) void k_velocity(BatchArgs ba, int flags) 
       V<CPT> n_b1 = ldv<CPT>(row(b1p, jn, P) + ib);
       V<CPT> n_b3 = ldv<CPT>(row(b3p, jn, P) + ib);
       V<CPT> n_b2 = n_b1, n_b4 = n_b3;
-      if (!bsame) {
+      if constexpr (!SAME) {
         n_b2 = ldv<CPT>(row(b2p, jn, P) + ib);
         n_b4 = ldv<CPT>(row(b4p, jn, P) + ib);
       }
@@ -650,7 +701,18 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
 // left to k_band (they need HABC-corrected velocities).
 // Arithmetic is that of k_velocity / k_stress exactly.
 // ---------------------------------------------------------------------------
-template <int M, int CPT, int WPB, int K>
+// SAME: dtx == dtz for every shot, so the dt/rho arrays of x and z are one
+// array each (a compile-time fact: a run-time choice between the two makes
+// the compiler copy the prefetched values right after issuing their loads).
+//
+// Register windows are rings of L = 2M+2 rows: row r lives in slot
+// (r - jb) mod L, and the z-march is unrolled by L so that every slot index
+// is a compile-time constant.  Rotating the windows is then register renaming
+// (no copies), and each prefetched row is first waited for one row later, at
+// its first use.  The march runs in whole blocks of L rows; rows past the
+// tile's last needed row are computed from clamped (zero guard) rows and not
+// stored.
+template <int M, int CPT, int WPB, int K, bool SAME>
 __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
   constexpr int KF = Mode<K>::F, KC = Mode<K>::C;
   typedef typename Store<KF>::T T;
@@ -658,6 +720,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
   constexpr int HL = ((M + CPT - 1) / CPT) * CPT;  // computed-not-written lane columns per side
   constexpr int LC = 64 * CPT;
   constexpr int LW = LC + 2 * M;
+  constexpr int L = 2 * M + 2;
   __shared__ float lds[WPB][4][LW];
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
@@ -680,31 +743,22 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     const int o0 = (t % ftiles_x) * ftw;      // first written column
     const int j0 = (t / ftiles_x) * tz;
     const int j1 = min(j0 + tz, nz);
-    const int ib = o0 - HL + lane * CPT;      // this lane's first column
+    const int ib0 = o0 - HL;                  // first lane column of the tile
+    const int lo = lane * CPT;                // this lane's column offset in the tile
+    const int ib = ib0 + lo;                  // this lane's first column
     int hx = -1;
     if (lane < M) hx = lane;
     else if (lane >= 64 - M) hx = LC + M + (lane - (64 - M));
     const bool has_halo = hx >= 0;
-    const int hcol = o0 - HL - M + hx;
-
-    const T* avx = (const T*)uniform_p(sd.fo[F_VX]);
-    const T* avz = (const T*)uniform_p(sd.fo[F_VZ]);
-    const T* atxx = (const T*)uniform_p(sd.fo[F_TXX]);
-    const T* atzz = (const T*)uniform_p(sd.fo[F_TZZ]);
-    const T* atxz = (const T*)uniform_p(sd.fo[F_TXZ]);
-    T* bvx = (T*)uniform_p(sd.f[F_VX]);
-    T* bvz = (T*)uniform_p(sd.f[F_VZ]);
-    T* btxx = (T*)uniform_p(sd.f[F_TXX]);
-    T* btzz = (T*)uniform_p(sd.f[F_TZZ]);
-    T* btxz = (T*)uniform_p(sd.f[F_TXZ]);
-    const TC* b1p = (const TC*)uniform_p(sd.bvx_x);
-    const TC* b2p = (const TC*)uniform_p(sd.bvx_z);
-    const TC* b3p = (const TC*)uniform_p(sd.bvz_x);
-    const TC* b4p = (const TC*)uniform_p(sd.bvz_z);
-    const bool bsame = (b1p == b2p) && (b3p == b4p);
-    const TC* lamp = (const TC*)uniform_p(sd.lam);
-    const TC* l2mp = (const TC*)uniform_p(sd.l2m);
-    const TC* mup = (const TC*)uniform_p(sd.mu_txz);
+    const int hcol = ib0 - M + hx;
+    constexpr int FES = (int)sizeof(T), CES = (int)sizeof(TC);
+    const Rsrc rA = make_rsrc(uniform_p(sd.bufA), (uint32_t)uniform((int)sd.recF));
+    const Rsrc rB = make_rsrc(uniform_p(sd.bufB), (uint32_t)uniform((int)sd.recF));
+    const Rsrc rC = make_rsrc(uniform_p(sd.bufC), (uint32_t)uniform((int)sd.recC));
+    const uint32_t fstr = (uint32_t)uniform((int)sd.fstr_b), forg = (uint32_t)uniform((int)sd.forg_b);
+    const uint32_t cstr = (uint32_t)uniform((int)sd.cstr_b), corg = (uint32_t)uniform((int)sd.corg_b);
+    // coefficient blocks (dtz arrays alias the dtx ones when SAME)
+    constexpr int QB1 = 0, QB3 = 1, QLAM = 2, QL2M = 3, QMU = 4, QB2 = SAME ? 0 : 5, QB4 = SAME ? 1 : 6;
     const int nw = uniform(sd.nw), stype = uniform(sd.stype);
     const bool src_on = (flags & FLAG_INJECT) && k <= nw;
     const float wav = src_on ? uniform_f(uniform_p(sd.wavelet)[k - 1]) : 0.f;
@@ -729,207 +783,204 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
 #pragma unroll
     for (int l = 0; l < M; ++l) a[l] = uniform_f(sd.a[l]);
     // ---- end of shot-table reads
-    // pressure source as seen by every reader of old txx / tzz
-    auto srcp = [&](int i, int j) { return (src_p && i == si && j == sj) ? wav : 0.f; };
-
-    float wtxz[CPT][2 * M];      // old txz rows jv-M .. jv+M-1
-    float wtzz[CPT][2 * M + 1];  // old tzz rows jv-M .. jv+M
-    float wtxx[CPT][M + 1];      // old txx rows jv-M .. jv
-    float wvx[CPT][2 * M];       // new vx rows jv-2M+1 .. jv
-    float wvz[CPT][2 * M + 1];   // new vz rows jv-2M .. jv
-    const int jb = j0 - M;       // first velocity row
-#pragma unroll
-    for (int q = 0; q < 2 * M; ++q) {
-      V<CPT> x = ldf<KF, CPT>(rowt(atxz, jb - M + q, P) + ib, rxz);
-#pragma unroll
-      for (int c = 0; c < CPT; ++c) wtxz[c][q] = x.v[c];
-    }
-#pragma unroll
-    for (int q = 0; q < 2 * M + 1; ++q) {
-      V<CPT> z = ldf<KF, CPT>(rowt(atzz, jb - M + q, P) + ib, rzz);
-#pragma unroll
-      for (int c = 0; c < CPT; ++c) wtzz[c][q] = z.v[c] + srcp(ib + c, jb - M + q);
-    }
-#pragma unroll
-    for (int q = 0; q < M + 1; ++q) {
-      V<CPT> x = ldf<KF, CPT>(rowt(atxx, jb - M + q, P) + ib, rxx);
-#pragma unroll
-      for (int c = 0; c < CPT; ++c) wtxx[c][q] = x.v[c] + srcp(ib + c, jb - M + q);
-    }
-#pragma unroll
-    for (int c = 0; c < CPT; ++c) {
-#pragma unroll
-      for (int q = 0; q < 2 * M; ++q) wvx[c][q] = 0.f;
-#pragma unroll
-      for (int q = 0; q < 2 * M + 1; ++q) wvz[c][q] = 0.f;
-    }
-    float c_htxx = has_halo ? ld1<KF>(rowt(atxx, jb, P) + hcol, rxx) + srcp(hcol, jb) : 0.f;
-    float c_htxz = has_halo ? ld1<KF>(rowt(atxz, jb, P) + hcol, rxz) : 0.f;
-    V<CPT> c_vx = ldf<KF, CPT>(rowt(avx, jb, P) + ib, rvx);
-    V<CPT> c_vz = ldf<KF, CPT>(rowt(avz, jb, P) + ib, rvz);
-    V<CPT> c_b1 = ldf<KC, CPT>(rowt(b1p, jb, P) + ib, r1);
-    V<CPT> c_b3 = ldf<KC, CPT>(rowt(b3p, jb, P) + ib, r3);
-    V<CPT> c_b2 = c_b1, c_b4 = c_b3;
-    if (!bsame) {
-      c_b2 = ldf<KC, CPT>(rowt(b2p, jb, P) + ib, r2);
-      c_b4 = ldf<KC, CPT>(rowt(b4p, jb, P) + ib, r4);
-    }
-    V<CPT> c_lam = ldf<KC, CPT>(rowt(lamp, jb - M, P) + ib, rl);
-    V<CPT> c_l2m = ldf<KC, CPT>(rowt(l2mp, jb - M, P) + ib, rl2);
-    V<CPT> c_mu = ldf<KC, CPT>(rowt(mup, jb - M, P) + ib, rmu);
-
-    const int jend = j1 + M;
-    for (int jv = jb; jv < jend; ++jv) {
-      // prefetch for jv+1 (velocity inputs) and stress row jv+1-M (coefficients)
-      const int jn = jv + 1;
-      float n_htxx = has_halo ? ld1<KF>(rowt(atxx, jn, P) + hcol, rxx) : 0.f;
-      float n_htxz = has_halo ? ld1<KF>(rowt(atxz, jn, P) + hcol, rxz) : 0.f;
-      V<CPT> n_txx = ldf<KF, CPT>(rowt(atxx, jn, P) + ib, rxx);
-      V<CPT> n_txz = ldf<KF, CPT>(rowt(atxz, jv + M, P) + ib, rxz);
-      V<CPT> n_tzz = ldf<KF, CPT>(rowt(atzz, jv + M + 1, P) + ib, rzz);
-      V<CPT> n_vx = ldf<KF, CPT>(rowt(avx, jn, P) + ib, rvx);
-      V<CPT> n_vz = ldf<KF, CPT>(rowt(avz, jn, P) + ib, rvz);
-      V<CPT> n_b1 = ldf<KC, CPT>(rowt(b1p, jn, P) + ib, r1);
-      V<CPT> n_b3 = ldf<KC, CPT>(rowt(b3p, jn, P) + ib, r3);
-      V<CPT> n_b2 = n_b1, n_b4 = n_b3;
-      if (!bsame) {
-        n_b2 = ldf<KC, CPT>(rowt(b2p, jn, P) + ib, r2);
-        n_b4 = ldf<KC, CPT>(rowt(b4p, jn, P) + ib, r4);
-      }
-      V<CPT> n_lam = ldf<KC, CPT>(rowt(lamp, jn - M, P) + ib, rl);
-      V<CPT> n_l2m = ldf<KC, CPT>(rowt(l2mp, jn - M, P) + ib, rl2);
-      V<CPT> n_mu = ldf<KC, CPT>(rowt(mup, jn - M, P) + ib, rmu);
-
-      // ---- new velocity, row jv, all lane columns
-#pragma unroll
-      for (int c = 0; c < CPT; ++c) {
-        s_txx[M + lane * CPT + c] = wtxx[c][M];
-        s_txz[M + lane * CPT + c] = wtxz[c][M];
-      }
-      if (has_halo) {
-        s_txx[hx] = c_htxx;
-        s_txz[hx] = c_htxz;
-      }
-      wave_sync();
-      V<CPT> nvx, nvz;
-#pragma unroll
-      for (int c = 0; c < CPT; ++c) {
-        const int x = M + lane * CPT + c;
-        float d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f;
-#pragma unroll
-        for (int l = 1; l <= M; ++l) {
-          d1 = fmaf_(a[l - 1], s_txx[x + l - 1] - s_txx[x - l], d1);
-          d2 = fmaf_(a[l - 1], wtxz[c][M + l - 1] - wtxz[c][M - l], d2);
-          d3 = fmaf_(a[l - 1], s_txz[x + l] - s_txz[x - l + 1], d3);
-          d4 = fmaf_(a[l - 1], wtzz[c][M + l] - wtzz[c][M - l + 1], d4);
-        }
-        const int i = ib + c;
-        const bool inside = (i >= M) & (i < nx - M) & (jv >= M) & (jv < nz - M);
-        float vxs = c_vx.v[c], vzs = c_vz.v[c];
-        if (i == si && jv == sj) {
-          if (src_vz) vzs = vzs + wav * src_scale;
-          if (src_vx) vxs = vxs + wav * src_scale;
-        }
-        const float ix = fmaf_(c_b2.v[c], d2, c_b1.v[c] * d1);
-        const float iz = fmaf_(c_b4.v[c], d4, c_b3.v[c] * d3);
-        // the stress half below consumes the velocity as stored
-        nvx.v[c] = as_stored<KF>(inside ? vxs + ix : vxs, wvxs, rvx);
-        nvz.v[c] = as_stored<KF>(inside ? vzs + iz : vzs, wvzs, rvz);
-      }
-      const bool own_lane = lane * CPT >= HL && lane * CPT < LC - HL;
-      if (jv >= j0 && jv < j1 && own_lane) {
+    const int zlast = nz + GUARD_ROWS - 1;  // last row a load may touch (zero guard)
+    // byte offsets: uniform part (block + row + tile column ib0), lane part
+    const uint32_t lvo = (uint32_t)(lo * FES), lco = (uint32_t)(lo * CES);
+    const uint32_t hvo = has_halo ? (uint32_t)(hx * FES) : OOB;  // halo lanes read column ib0-M+hx
+    auto frow = [&](int c, int r, int col) {
+      return (uint32_t)c * fstr + forg + (uint32_t)((min(r, zlast) * P + col) * FES);
+    };
+    auto fld = [&](int c, int r, float sc) { return bload<KF, CPT>(rA, lvo, frow(c, r, ib0), sc); };
+    auto cof = [&](int q, int r, float sc) {
+      return bload<KC, CPT>(rC, lco, (uint32_t)q * cstr + corg + (uint32_t)((min(r, zlast) * P + ib0) * CES),
+                            sc);
+    };
+    auto hal = [&](int c, int r, float sc) { return bload<KF, 1>(rA, hvo, frow(c, r, ib0 - M), sc).v[0]; };
+    // pressure source as seen by every reader of old txx / tzz at row r; it is
+    // added where a row is first used (one row after its load is issued), so
+    // that no prefetched value is touched before it is needed
+    auto add_src = [&](V<CPT>& x, int r) {
+      if (src_p && r == sj) {
 #pragma unroll
         for (int c = 0; c < CPT; ++c)
-          if (ib + c < nx) {
-            st1<KF>(rowt(bvx, jv, P) + ib + c, nvx.v[c], wvxs);
-            st1<KF>(rowt(bvz, jv, P) + ib + c, nvz.v[c], wvzs);
-          }
+          if (ib + c == si) x.v[c] = x.v[c] + wav;
       }
-#pragma unroll
-      for (int c = 0; c < CPT; ++c) {
-#pragma unroll
-        for (int q = 0; q < 2 * M - 1; ++q) wvx[c][q] = wvx[c][q + 1];
-        wvx[c][2 * M - 1] = nvx.v[c];
-#pragma unroll
-        for (int q = 0; q < 2 * M; ++q) wvz[c][q] = wvz[c][q + 1];
-        wvz[c][2 * M] = nvz.v[c];
-      }
+    };
 
-      // ---- new stress, row j = jv - M
-      const int j = jv - M;
-      if (j >= j0) {
+    V<CPT> rtxx[L], rtxz[L], rtzz[L];  // old stress rings
+    V<CPT> qvx[L], qvz[L];             // new velocity rings
+    V<CPT> cvx[2], cvz[2], cb1[2], cb2[2], cb3[2], cb4[2], clam[2], cl2m[2], cmu[2];
+    float htxx[2], htxz[2];
+    const int jb = j0 - M;  // first velocity row; row r sits in ring slot (r - jb) mod L
+    // rows the march reads before its first row: txx jb-M..jb, txz jb-M..jb+M-1,
+    // tzz jb-M..jb+M (the source is added here to all but txx row jb and tzz
+    // row jb+M, which the first iteration adds like every later row)
+#pragma unroll
+    for (int q = 0; q < 2 * M + 1; ++q) {
+      const int r = jb - M + q, sl = (q - M + L) % L;
+      if (q < 2 * M) rtxz[sl] = fld(F_TXZ, r, rxz);
+      rtzz[sl] = fld(F_TZZ, r, rzz);
+      if (q < 2 * M) add_src(rtzz[sl], r);
+      if (q <= M) {
+        rtxx[sl] = fld(F_TXX, r, rxx);
+        if (q < M) add_src(rtxx[sl], r);
+      }
+    }
+    htxx[0] = hal(F_TXX, jb, rxx);
+    htxz[0] = hal(F_TXZ, jb, rxz);
+    cvx[0] = fld(F_VX, jb, rvx);
+    cvz[0] = fld(F_VZ, jb, rvz);
+    cb1[0] = cof(QB1, jb, r1);
+    cb3[0] = cof(QB3, jb, r3);
+    if constexpr (!SAME) {
+      cb2[0] = cof(QB2, jb, r2);
+      cb4[0] = cof(QB4, jb, r4);
+    }
+    clam[0] = cof(QLAM, jb - M, rl);
+    cl2m[0] = cof(QL2M, jb - M, rl2);
+    cmu[0] = cof(QMU, jb - M, rmu);
+    const bool own_lane = lo >= HL && lo < LC - HL;
+
+    const int jend = j1 + M;
+    for (int jv0 = jb; jv0 < jend; jv0 += L) {
+#pragma unroll
+      for (int u = 0; u < L; ++u) {
+        const int jv = jv0 + u;
+        const int cu = u & 1, nu = cu ^ 1;  // L is even: parity of jv - jb
+        // ---- prefetch: velocity inputs of row jv+1, stress coefficients of row jv+1-M
+        const int jn = jv + 1;
+        htxx[nu] = hal(F_TXX, jn, rxx);
+        htxz[nu] = hal(F_TXZ, jn, rxz);
+        rtxx[(u + 1) % L] = fld(F_TXX, jn, rxx);
+        rtxz[(u + M) % L] = fld(F_TXZ, jv + M, rxz);
+        rtzz[(u + M + 1) % L] = fld(F_TZZ, jv + M + 1, rzz);
+        cvx[nu] = fld(F_VX, jn, rvx);
+        cvz[nu] = fld(F_VZ, jn, rvz);
+        cb1[nu] = cof(QB1, jn, r1);
+        cb3[nu] = cof(QB3, jn, r3);
+        if constexpr (!SAME) {
+          cb2[nu] = cof(QB2, jn, r2);
+          cb4[nu] = cof(QB4, jn, r4);
+        }
+        clam[nu] = cof(QLAM, jn - M, rl);
+        cl2m[nu] = cof(QL2M, jn - M, rl2);
+        cmu[nu] = cof(QMU, jn - M, rmu);
+
+        // ---- first uses of rows loaded one iteration ago: txx row jv, tzz row jv+M
+        add_src(rtxx[u % L], jv);
+        add_src(rtzz[(u + M) % L], jv + M);
+        // ---- new velocity, row jv, all lane columns
+        const V<CPT>& xr = rtxx[u % L];
+        const V<CPT>& zr = rtxz[u % L];
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
-          s_vx[lane * CPT + c] = wvx[c][M - 1];
-          s_vz[lane * CPT + c] = wvz[c][M];
+          s_txx[M + lo + c] = xr.v[c];
+          s_txz[M + lo + c] = zr.v[c];
+        }
+        if (has_halo) {
+          float hxx = htxx[cu];
+          if (src_p && jv == sj && hcol == si) hxx = hxx + wav;
+          s_txx[hx] = hxx;
+          s_txz[hx] = htxz[cu];
         }
         wave_sync();
-        const bool band_row = j >= nz - band || (!fs && j < band);
-        V<CPT> oxx, ozz, oxz;
+        V<CPT> nvx, nvz;
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
-          const int x = lane * CPT + c;
+          const int x = M + lo + c;
           float d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f;
-          if (own_lane) {
 #pragma unroll
-            for (int l = 1; l <= M; ++l) {
-              d1 = fmaf_(a[l - 1], s_vx[x + l] - s_vx[x - l + 1], d1);        // dvxdx
-              d2 = fmaf_(a[l - 1], wvz[c][M + l - 1] - wvz[c][M - l], d2);    // dvzdz
-              d3 = fmaf_(a[l - 1], wvx[c][M + l - 1] - wvx[c][M - l], d3);    // dvxdz
-              d4 = fmaf_(a[l - 1], s_vz[x + l - 1] - s_vz[x - l], d4);        // dvzdx
-            }
+          for (int l = 1; l <= M; ++l) {
+            d1 = fmaf_(a[l - 1], s_txx[x + l - 1] - s_txx[x - l], d1);
+            d2 = fmaf_(a[l - 1], rtxz[(u + l - 1) % L].v[c] - rtxz[(u - l + L) % L].v[c], d2);
+            d3 = fmaf_(a[l - 1], s_txz[x + l] - s_txz[x - l + 1], d3);
+            d4 = fmaf_(a[l - 1], rtzz[(u + l) % L].v[c] - rtzz[(u - l + 1 + L) % L].v[c], d4);
           }
           const int i = ib + c;
-          const bool inside = (i >= M) & (i < nx - M) & (j >= M) & (j < nz - M);
-          const float xx = wtxx[c][0], zz = wtzz[c][0], xz = wtxz[c][0];
-          const float t1 = d1 * dtx, t2 = d2 * dtz;
-          const float lv = c_lam.v[c], l2 = c_l2m.v[c];
-          const float ixx = fmaf_(lv, t2, l2 * t1);
-          const float izz = fmaf_(l2, t2, lv * t1);
-          const float sxz = fmaf_(d4, dtx, d3 * dtz);
-          const float ixz = c_mu.v[c] * sxz;
-          oxx.v[c] = inside ? xx + ixx : xx;
-          ozz.v[c] = inside ? zz + izz : zz;
-          oxz.v[c] = inside ? xz + ixz : xz;
+          const bool inside = (i >= M) & (i < nx - M) & (jv >= M) & (jv < nz - M);
+          float vxs = cvx[cu].v[c], vzs = cvz[cu].v[c];
+          if (i == si && jv == sj) {
+            if (src_vz) vzs = vzs + wav * src_scale;
+            if (src_vx) vxs = vxs + wav * src_scale;
+          }
+          const float b1 = cb1[cu].v[c], b3 = cb3[cu].v[c];
+          const float b2 = SAME ? b1 : cb2[cu].v[c], b4 = SAME ? b3 : cb4[cu].v[c];
+          const float ix = fmaf_(b2, d2, b1 * d1);
+          const float iz = fmaf_(b4, d4, b3 * d3);
+          // the stress half below consumes the velocity as stored
+          nvx.v[c] = as_stored<KF>(inside ? vxs + ix : vxs, wvxs, rvx);
+          nvz.v[c] = as_stored<KF>(inside ? vzs + iz : vzs, wvzs, rvz);
         }
-        if (own_lane && !band_row) {
+        {
+          const bool vrow = jv >= j0 && jv < j1;
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) {
+            const uint32_t vo = (vrow && own_lane && ib + c < nx) ? lvo + (uint32_t)(c * FES) : OOB;
+            bstore<KF>(rB, vo, frow(F_VX, jv, ib0), nvx.v[c], wvxs);
+            bstore<KF>(rB, vo, frow(F_VZ, jv, ib0), nvz.v[c], wvzs);
+          }
+        }
+        qvx[u % L] = nvx;
+        qvz[u % L] = nvz;
+
+        // ---- new stress, row j = jv - M (stores outside the branch, masked,
+        // so that every store is issued on every row)
+        const int j = jv - M;
+        const bool srow = j >= j0 && j < j1;
+        V<CPT> oxx, ozz, oxz;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) oxx.v[c] = ozz.v[c] = oxz.v[c] = 0.f;
+        if (srow) {
+          const int sm = (u - M + L) % L;  // slot of row j
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) {
+            s_vx[lo + c] = qvx[sm].v[c];
+            s_vz[lo + c] = qvz[sm].v[c];
+          }
+          wave_sync();
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) {
+            const int x = lo + c;
+            float d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f;
+            if (own_lane) {
+#pragma unroll
+              for (int l = 1; l <= M; ++l) {
+                d1 = fmaf_(a[l - 1], s_vx[x + l] - s_vx[x - l + 1], d1);  // dvxdx
+                d2 = fmaf_(a[l - 1], qvz[(u - M + l - 1 + L) % L].v[c] -
+                                         qvz[(u - M - l + L) % L].v[c], d2);  // dvzdz
+                d3 = fmaf_(a[l - 1], qvx[(u - M + l + L) % L].v[c] -
+                                         qvx[(u - M - l + 1 + L) % L].v[c], d3);  // dvxdz
+                d4 = fmaf_(a[l - 1], s_vz[x + l - 1] - s_vz[x - l], d4);  // dvzdx
+              }
+            }
+            const int i = ib + c;
+            const bool inside = (i >= M) & (i < nx - M) & (j >= M) & (j < nz - M);
+            const float xx = rtxx[sm].v[c], zz = rtzz[sm].v[c], xz = rtxz[sm].v[c];
+            const float t1 = d1 * dtx, t2 = d2 * dtz;
+            const float lv = clam[cu].v[c], l2 = cl2m[cu].v[c];
+            const float ixx = fmaf_(lv, t2, l2 * t1);
+            const float izz = fmaf_(l2, t2, lv * t1);
+            const float sxz = fmaf_(d4, dtx, d3 * dtz);
+            const float ixz = cmu[cu].v[c] * sxz;
+            oxx.v[c] = inside ? xx + ixx : xx;
+            ozz.v[c] = inside ? zz + izz : zz;
+            oxz.v[c] = inside ? xz + ixz : xz;
+          }
+        }
+        {
+          const bool band_row = j >= nz - band || (!fs && j < band);
+          const bool wrow = srow && !band_row;
 #pragma unroll
           for (int c = 0; c < CPT; ++c) {
             const int i = ib + c;
-            if (i >= band && i < nx - band) {
-              st1<KF>(rowt(btxx, j, P) + i, oxx.v[c], wxx);
-              st1<KF>(rowt(btzz, j, P) + i, ozz.v[c], wzz);
-              st1<KF>(rowt(btxz, j, P) + i, oxz.v[c], wxz);
-            }
+            const uint32_t vo = (wrow && own_lane && i >= band && i < nx - band) ? lvo + (uint32_t)(c * FES) : OOB;
+            bstore<KF>(rB, vo, frow(F_TXX, j, ib0), oxx.v[c], wxx);
+            bstore<KF>(rB, vo, frow(F_TZZ, j, ib0), ozz.v[c], wzz);
+            bstore<KF>(rB, vo, frow(F_TXZ, j, ib0), oxz.v[c], wxz);
           }
         }
+        wave_sync();
       }
-      wave_sync();
-
-      // ---- slide the old-state windows to jv+1
-#pragma unroll
-      for (int c = 0; c < CPT; ++c) {
-#pragma unroll
-        for (int q = 0; q < 2 * M - 1; ++q) wtxz[c][q] = wtxz[c][q + 1];
-        wtxz[c][2 * M - 1] = n_txz.v[c];
-#pragma unroll
-        for (int q = 0; q < 2 * M; ++q) wtzz[c][q] = wtzz[c][q + 1];
-        wtzz[c][2 * M] = n_tzz.v[c] + srcp(ib + c, jv + M + 1);
-#pragma unroll
-        for (int q = 0; q < M; ++q) wtxx[c][q] = wtxx[c][q + 1];
-        wtxx[c][M] = n_txx.v[c] + srcp(ib + c, jn);
-      }
-      c_htxx = n_htxx + srcp(hcol, jn);
-      c_htxz = n_htxz;
-      c_vx = n_vx;
-      c_vz = n_vz;
-      c_b1 = n_b1;
-      c_b2 = n_b2;
-      c_b3 = n_b3;
-      c_b4 = n_b4;
-      c_lam = n_lam;
-      c_l2m = n_l2m;
-      c_mu = n_mu;
     }
   }
 }
