@@ -75,6 +75,13 @@ SIGNATURES = {
     "fomo_batch_destroy": (c_int32, [c_void_p]),
     "fomo_batch_run": (c_int32, [c_void_p, c_int32, c_int32]),
     "fomo_launch_stage": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32]),
+    "fomo_pool_create": (c_int32, [c_void_p, c_int32, c_int32, c_int32, ctypes.POINTER(c_void_p)]),
+    "fomo_pool_destroy": (c_int32, [c_void_p]),
+    "fomo_pool_add": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fomo_pool_remove": (c_int32, [c_void_p, c_void_p]),
+    "fomo_pool_run": (c_int32, [c_void_p, c_int32]),
+    "fomo_pool_size": (c_int32, [c_void_p]),
+    "fomo_pool_next_step": (c_int32, [c_void_p, c_void_p, P_i32]),
 }
 
 _lib = None

@@ -116,6 +116,7 @@ struct fomo_batch {
   std::vector<int> hprefix;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // by the parity the chunk starts from
   hipGraph_t graph[2] = {nullptr, nullptr};
+  int grid_fixed[PART_COUNT] = {};  // > 0: fixed grid (pools: persistent over any membership)
 };
 
 static thread_local std::string g_err;
@@ -1026,6 +1027,7 @@ static int single_batch(fomo_model* m, fomo_wavefield* w, fomo_shot* s, fomo_bat
 
 // ------------------------------------------------------------------ launches
 static int blocks_for(const fomo_batch* b, int part, int wpb = WPB) {
+  if (b->grid_fixed[part] > 0) return b->grid_fixed[part];
   const int total = b->hprefix[(size_t)part * (b->n + 1) + b->n];
   return std::max(1, (total + wpb - 1) / wpb);
 }
@@ -1109,7 +1111,7 @@ static void launch_habc(const fomo_batch* b, int table, bool stress, int k_off) 
   const int part = stress ? PART_HABC_S : PART_HABC_V;
   const int items = b->hprefix[(size_t)part * (b->n + 1) + b->n];
   const BatchArgs a = args_for(b, table, part, k_off);
-  const dim3 g(std::max(1, items));
+  const dim3 g(b->grid_fixed[part] > 0 ? b->grid_fixed[part] : std::max(1, items));
   if (b->storage == FOMO_STORAGE_BF16)
     hipLaunchKernelGGL(k_habc<ST_BF16>, g, dim3(64), 0, b->ctx->stream, a, stress ? 1 : 0);
   else if (b->storage == FOMO_STORAGE_F16)
@@ -1155,6 +1157,7 @@ static void launch_set_step(const fomo_batch* b, int k) {
   hipLaunchKernelGGL(k_set_step, dim3((b->n + 255) / 256), dim3(256), 0, b->ctx->stream, b->ds,
                      b->dn, k);
 }
+// Reads the kp of rows [0, n) of table TBL_IN0 (filled for batches and pools).
 static void launch_advance(const fomo_batch* b, int by) {
   hipLaunchKernelGGL(k_advance, dim3((b->n + 255) / 256), dim3(256), 0, b->ctx->stream, b->ds,
                      b->dn, by);
@@ -1355,4 +1358,236 @@ extern "C" int32_t fomo_batch_destroy(fomo_batch* b) {
 extern "C" int32_t fomo_batch_run(fomo_batch* b, int32_t k_first, int32_t n_steps) {
   if (!b) return fail(nullptr, FOMO_ERR_ARG, "NULL batch");
   return run_batch(b, k_first, n_steps);
+}
+
+// ------------------------------------------------------------------ pools
+// A pool is a batch whose membership changes between runs: the serving
+// loop's continuous batching.  Its shot table has `capacity` rows, the live
+// members packed at the front; kernels run on fixed persistent grids and
+// read the live count and work lists from device memory, so one captured
+// graph per pool serves every membership.  Members join at step 1 with their
+// own step counter and leave when done; each member's rows in the two fused
+// tables are laid out from its own current buffer, so members need not share
+// a buffer parity.  Table updates are uploaded on the context's stream
+// between runs (never inside a captured graph).
+struct fomo_pool {
+  fomo_ctx* ctx = nullptr;
+  fomo_batch* b = nullptr;
+  int capacity = 0;
+  std::vector<fomo_model*> ms;
+  std::vector<fomo_wavefield*> ws;
+  std::vector<fomo_shot*> ss;
+  std::vector<int> c0;      // member's current buffer when it joined
+  std::vector<int> k_next;  // member's next step (1-based), host view
+  ShotDev* h_rows = nullptr;  // pinned staging: [2][capacity] fused-table rows
+  int* h_prefix = nullptr;    // pinned staging: [PART_COUNT][capacity+1], then the live count
+  hipEvent_t uploaded = nullptr;
+  bool dirty = false;
+};
+
+static void pool_free(fomo_pool* p) {
+  if (!p) return;
+  if (p->b) destroy_batch(p->b);
+  if (p->h_rows) (void)hipHostFree(p->h_rows);
+  if (p->h_prefix) (void)hipHostFree(p->h_prefix);
+  if (p->uploaded) (void)hipEventDestroy(p->uploaded);
+  delete p;
+}
+
+extern "C" int32_t fomo_pool_create(fomo_ctx* c, int32_t fd_order, int32_t storage, int32_t capacity,
+                                    fomo_pool** out) {
+  if (!c || !out) return fail(c, FOMO_ERR_ARG, "NULL argument");
+  if (fd_order < 2 || fd_order > 2 * MAX_M || fd_order % 2)
+    return fail(c, FOMO_ERR_UNSUPPORTED, "fd_order must be even, 2..16");
+  if (storage < FOMO_STORAGE_F32 || storage > FOMO_STORAGE_F16C)
+    return fail(c, FOMO_ERR_ARG, "unknown storage type");
+  const int M = fd_order / 2;
+  if (storage != FOMO_STORAGE_F32 && !reduced_m_ok(M))
+    return fail(c, FOMO_ERR_UNSUPPORTED, "reduced-precision storage supports fd_order 4, 8 and 12");
+  if (capacity < 1 || capacity > 4096) return fail(c, FOMO_ERR_ARG, "capacity must be 1..4096");
+  HIP_TRY(c, hipSetDevice(c->device));
+  auto p = std::unique_ptr<fomo_pool, void (*)(fomo_pool*)>(new fomo_pool, pool_free);
+  p->ctx = c;
+  p->capacity = capacity;
+  fomo_batch* b = new fomo_batch;
+  p->b = b;
+  b->ctx = c;
+  b->n = capacity;
+  b->M = M;
+  b->storage = storage;
+  b->same = true;
+  b->cpt = c->cpt;
+  b->graph_steps = c->graph_steps;
+  b->step_mode = 1;
+  b->step_mode_req = 1;
+  // rows per tile: about 32 with tz + 2M a whole number of march blocks (2M+2)
+  const int L = 2 * M + 2;
+  b->tz = c->tz > 0 ? c->tz : std::max(8, round_up(32 + 2 * M, L) - 2 * M);
+  b->tz_req = c->tz;
+  // fixed persistent grids: one round of resident wavefronts for the stencil
+  // kernels, a few blocks per CU for the strip kernels
+  const int cus = std::max(1, c->cus);
+  b->grid_fixed[PART_FUSED] = round_up(stencil_slots(storage, M, b->cpt, cus, 1) / WPB, NUM_XCD);
+  b->grid_fixed[PART_BAND] = 2 * cus;
+  b->grid_fixed[PART_HABC_V] = 4 * cus;
+  b->grid_fixed[PART_HABC_S] = 4 * cus;
+  b->grid_fixed[PART_FSREC] = cus;
+  b->hs.assign((size_t)TBL_COUNT * capacity, ShotDev{});
+  b->hprefix.assign((size_t)PART_COUNT * (capacity + 1), 0);
+  HIP_TRY(c, hipMalloc(&b->ds, b->hs.size() * sizeof(ShotDev)));
+  HIP_TRY(c, hipMemset(b->ds, 0, b->hs.size() * sizeof(ShotDev)));
+  HIP_TRY(c, hipMalloc(&b->dn, sizeof(int)));
+  HIP_TRY(c, hipMemset(b->dn, 0, sizeof(int)));
+  HIP_TRY(c, hipMalloc(&b->dprefix, b->hprefix.size() * sizeof(int)));
+  HIP_TRY(c, hipMemset(b->dprefix, 0, b->hprefix.size() * sizeof(int)));
+  HIP_TRY(c, hipHostMalloc((void**)&p->h_rows, 2 * (size_t)capacity * sizeof(ShotDev)));
+  HIP_TRY(c, hipHostMalloc((void**)&p->h_prefix, (b->hprefix.size() + 1) * sizeof(int)));
+  HIP_TRY(c, hipEventCreateWithFlags(&p->uploaded, hipEventDisableTiming));
+  HIP_TRY(c, hipEventRecord(p->uploaded, c->stream));
+  *out = p.release();
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_pool_destroy(fomo_pool* p) {
+  if (!p) return FOMO_OK;
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipStreamSynchronize(p->ctx->stream);
+  pool_free(p);
+  return FOMO_OK;
+}
+
+static int pool_find(const fomo_pool* p, const fomo_shot* s) {
+  for (size_t i = 0; i < p->ss.size(); ++i)
+    if (p->ss[i] == s) return (int)i;
+  return -1;
+}
+
+extern "C" int32_t fomo_pool_add(fomo_pool* p, fomo_model* m, fomo_wavefield* w, fomo_shot* s) {
+  if (!p || !m || !w || !s) return fail(p ? p->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_ctx* c = p->ctx;
+  if (m->ctx != c || w->m != m || s->m != m)
+    return fail(c, FOMO_ERR_STATE, "pool member belongs to another model or context");
+  if (m->M != p->b->M || m->storage != p->b->storage)
+    return fail(c, FOMO_ERR_UNSUPPORTED, "member's fd_order or storage differs from the pool's");
+  if (m->d.dtx != m->d.dtz) return fail(c, FOMO_ERR_UNSUPPORTED, "pool members need dtx == dtz");
+  const size_t fb = 5 * m->fstride * m->fes, cb = (size_t)m->ncoef_blocks * m->fstride * m->es;
+  if (fb >= 0x7FFFFFFF || cb >= 0x7FFFFFFF)
+    return fail(c, FOMO_ERR_UNSUPPORTED, "grid too large for the fused step");
+  if ((int)p->ss.size() >= p->capacity) return fail(c, FOMO_ERR_ARG, "pool is full");
+  for (size_t i = 0; i < p->ss.size(); ++i)
+    if (p->ws[i] == w || p->ss[i] == s)
+      return fail(c, FOMO_ERR_ARG, "wavefield or shot already in the pool");
+  int rc = adopt_scales(w, s);
+  if (rc) return rc;
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, s->dk, 1);
+  p->ms.push_back(m);
+  p->ws.push_back(w);
+  p->ss.push_back(s);
+  p->c0.push_back(w->cur);
+  p->k_next.push_back(1);
+  p->dirty = true;
+  return check_launch(c, "pool_add");
+}
+
+extern "C" int32_t fomo_pool_remove(fomo_pool* p, fomo_shot* s) {
+  if (!p || !s) return fail(p ? p->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  const int i = pool_find(p, s);
+  if (i < 0) return fail(p->ctx, FOMO_ERR_ARG, "shot is not in the pool");
+  HIP_TRY(p->ctx, hipSetDevice(p->ctx->device));
+  HIP_TRY(p->ctx, hipStreamSynchronize(p->ctx->stream));  // its last steps have run
+  const int done = std::min(p->k_next[i] - 1, s->nt);
+  fomo_wavefield* w = p->ws[i];
+  w->cur = p->c0[i] ^ (done & 1);
+  w->backup_is_prev = done > 0;
+  p->ms.erase(p->ms.begin() + i);
+  p->ws.erase(p->ws.begin() + i);
+  p->ss.erase(p->ss.begin() + i);
+  p->c0.erase(p->c0.begin() + i);
+  p->k_next.erase(p->k_next.begin() + i);
+  p->dirty = true;
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_pool_size(const fomo_pool* p) { return p ? (int32_t)p->ss.size() : 0; }
+
+extern "C" int32_t fomo_pool_next_step(const fomo_pool* p, const fomo_shot* s, int32_t* k_next) {
+  if (!p || !s || !k_next) return fail(p ? p->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  const int i = pool_find(p, s);
+  if (i < 0) return fail(p->ctx, FOMO_ERR_ARG, "shot is not in the pool");
+  *k_next = p->k_next[i];
+  return FOMO_OK;
+}
+
+// Rebuilds the live rows of the two fused tables and the work lists, and
+// queues their upload (staging is reused once the previous upload is done).
+static int pool_upload(fomo_pool* p) {
+  fomo_ctx* c = p->ctx;
+  fomo_batch* b = p->b;
+  const int C = p->capacity, n = (int)p->ss.size();
+  HIP_TRY(c, hipEventSynchronize(p->uploaded));
+  for (int t = 0; t < 2; ++t)
+    for (int i = 0; i < n; ++i) {
+      // table t of the member: reads buffer c0 ^ t, writes the other
+      const int table = TBL_FUSED0 + (t ^ p->c0[i]);
+      p->h_rows[(size_t)t * C + i] = make_shotdev(p->ms[i], p->ws[i], p->ss[i], b->cpt, b->tz, table);
+    }
+  for (int q = 0; q < PART_COUNT; ++q) {
+    int acc = 0;
+    int* pre = p->h_prefix + (size_t)q * (C + 1);
+    for (int i = 0; i < n; ++i) {
+      pre[i] = acc;
+      const int64_t next = (int64_t)acc + part_items(p->h_rows[i], q);
+      if (next > (1 << 30)) return fail(c, FOMO_ERR_ARG, "pool work list too large");
+      acc = (int)next;
+    }
+    for (int i = n; i <= C; ++i) pre[i] = acc;
+  }
+  p->h_prefix[(size_t)PART_COUNT * (C + 1)] = n;
+  std::copy(p->h_prefix, p->h_prefix + (size_t)PART_COUNT * (C + 1), b->hprefix.begin());
+  if (n > 0) {
+    // table TBL_IN0 only serves the step counters (k_advance reads its kp)
+    HIP_TRY(c, hipMemcpyAsync(b->ds + (size_t)TBL_IN0 * C, p->h_rows, (size_t)n * sizeof(ShotDev),
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(b->ds + (size_t)TBL_FUSED0 * C, p->h_rows, (size_t)n * sizeof(ShotDev),
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(b->ds + (size_t)TBL_FUSED1 * C, p->h_rows + C, (size_t)n * sizeof(ShotDev),
+                              hipMemcpyHostToDevice, c->stream));
+  }
+  HIP_TRY(c, hipMemcpyAsync(b->dprefix, p->h_prefix, b->hprefix.size() * sizeof(int),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(b->dn, p->h_prefix + (size_t)PART_COUNT * (C + 1), sizeof(int),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipEventRecord(p->uploaded, c->stream));
+  p->dirty = false;
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_pool_run(fomo_pool* p, int32_t n_steps) {
+  if (!p) return fail(nullptr, FOMO_ERR_ARG, "NULL pool");
+  fomo_ctx* c = p->ctx;
+  fomo_batch* b = p->b;
+  if (n_steps < 0 || n_steps % 2) return fail(c, FOMO_ERR_ARG, "n_steps must be even and >= 0");
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (p->dirty) {
+    int rc = pool_upload(p);
+    if (rc) return rc;
+  }
+  if (p->ss.empty() || n_steps == 0) return FOMO_OK;
+  const int G = b->graph_steps;  // even: every chunk starts on table 0
+  const int chunks = n_steps / G, rem = n_steps % G;
+  if (chunks > 0 && !b->gexec[0]) {
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int g = 0; g < G; ++g) launch_step(b, g & 1, g);
+    launch_advance(b, G);
+    hipGraph_t graph = nullptr;
+    HIP_TRY(c, hipStreamEndCapture(c->stream, &graph));
+    b->graph[0] = graph;
+    HIP_TRY(c, hipGraphInstantiate(&b->gexec[0], graph, nullptr, nullptr, 0));
+  }
+  for (int i = 0; i < chunks; ++i) HIP_TRY(c, hipGraphLaunch(b->gexec[0], c->stream));
+  for (int g = 0; g < rem; ++g) launch_step(b, g & 1, g);
+  if (rem) launch_advance(b, rem);
+  for (int& k : p->k_next) k += n_steps;
+  return check_launch(c, "pool_run");
 }

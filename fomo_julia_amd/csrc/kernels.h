@@ -1615,6 +1615,11 @@ __global__ void k_inject(BatchArgs ba) {
   }
 }
 
+// One step counter set on the stream (a pool member joining between runs).
+__global__ void k_set_int(int* p, int v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
 // Step counters: set (before a run) and advance (last node of a captured chunk).
 __global__ void k_set_step(const ShotDev* shots, const int* n_shots_p, int k) {
   const int n = *n_shots_p;

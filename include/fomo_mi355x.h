@@ -188,6 +188,27 @@ int32_t fomo_batch_destroy(fomo_batch* b);
 /* Steps k_first .. k_first+n_steps-1 for every shot of the batch. */
 int32_t fomo_batch_run(fomo_batch* b, int32_t k_first, int32_t n_steps);
 
+/* ---- pools: continuous batching for a serving loop ----------------------
+ * No counterpart in the reference.  A pool runs the fused step for a changing
+ * set of shots (members) of one fd_order and storage type, uneven grids
+ * allowed, up to `capacity` at once.  A member joins with fomo_pool_add
+ * (it starts at step 1 on the next run; reset its wavefield and shot first),
+ * advances with every fomo_pool_run, and leaves with fomo_pool_remove
+ * (which waits for the stream).  Members past their nt are skipped until
+ * removed.  Members need dtx == dtz.  Runs are asynchronous and take an even
+ * number of steps; one captured graph per pool serves every membership. */
+typedef struct fomo_pool fomo_pool;
+int32_t fomo_pool_create(fomo_ctx* ctx, int32_t fd_order, int32_t storage, int32_t capacity,
+                         fomo_pool** out);
+int32_t fomo_pool_destroy(fomo_pool* p);
+int32_t fomo_pool_add(fomo_pool* p, fomo_model* m, fomo_wavefield* w, fomo_shot* s);
+int32_t fomo_pool_remove(fomo_pool* p, fomo_shot* s);
+int32_t fomo_pool_run(fomo_pool* p, int32_t n_steps);
+/* Number of members. */
+int32_t fomo_pool_size(const fomo_pool* p);
+/* The member's next step (1-based): it is done when this exceeds its nt. */
+int32_t fomo_pool_next_step(const fomo_pool* p, const fomo_shot* s, int32_t* k_next);
+
 /* ---- tuning ------------------------------------------------------------- */
 /* Step implementation for runs created after the call on this context:
  * 1 (default) = fused: velocity and stress in one pass over ping-pong state

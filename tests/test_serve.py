@@ -1,0 +1,91 @@
+"""Serving loop: continuous batching in fomo_pool.
+
+Every served shot must equal the same shot run alone through the plain
+single-shot path (bit for bit: the pool runs each shot's arithmetic exactly
+as the single-shot kernels do), whatever else is in the pool, whenever it
+joined and whatever its storage and FD order."""
+import numpy as np
+import pytest
+
+from fomo_julia_amd import serve as S
+
+SMALL = dict(nx=(60, 140), nz=(50, 100), nt=(60, 160))
+
+
+def test_stream_is_deterministic_and_in_range():
+    a = S.c5_stream(40, seed=5, rate=2.0)
+    b = S.c5_stream(40, seed=5, rate=2.0)
+    assert [r.__dict__ for r in a] == [r.__dict__ for r in b]
+    assert all(300 <= r.nx <= 2000 and 300 <= r.nz <= 1200 and 1000 <= r.nt <= 6000 for r in a)
+    assert {r.fd_order for r in a} <= {4, 8, 12} and {r.storage for r in a} == {"f32", "bf16"}
+    t = [r.arrival for r in a]
+    assert t == sorted(t) and t[0] > 0
+    assert all(r.arrival == 0.0 for r in S.c5_stream(5, rate=None))
+
+
+def test_prepare_matches_request():
+    r = S.c5_stream(1, seed=9, **SMALL)[0]
+    p = S.prepare(r)
+    pad = r.nbc + r.fd_order // 2
+    assert (p.M.nx, p.M.nz) == (r.nx + 2 * pad, r.nz + 2 * pad)
+    assert p.p.nt == r.nt and len(p.src.wavelet) == r.nt and p.M.storage == r.storage
+    assert p.bytes == S.device_bytes(p.M.nx, p.M.nz, r.storage, r.nt, len(p.rec.i))
+
+
+gpu = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def b():
+    from fomo_julia_amd import elastic2d as E
+    bk = E.backend("mi355x")
+    bk.set_tiling(1, 0, 16)
+    bk.set_step_mode(1)
+    return bk
+
+
+def _alone(b, r):
+    return S.serial([r], b, storage=None)[0].gather
+
+
+@gpu
+def test_served_equals_alone_mixed_orders_and_storage(b):
+    reqs = S.c5_stream(10, seed=21, storages=("f32", "bf16", "f16", "f16c"), **SMALL)
+    print("case served mix of", [(r.nx, r.nz, r.nt, r.fd_order, r.storage) for r in reqs], flush=True)
+    srv = S.Server(b, budget_bytes=8 << 30, chunk=16, capacity=8)
+    done = srv.serve(reqs, realtime=False)
+    assert sorted(c.rid for c in done) == list(range(10))
+    for c in done:
+        want = _alone(b, reqs[c.rid])
+        assert np.array_equal(c.gather, want), f"request {c.rid} differs from its solo run"
+
+
+@gpu
+def test_staggered_joins_and_small_budget(b):
+    reqs = S.c5_stream(8, seed=33, orders=(8,), storages=("f32",), **SMALL)
+    # joins at different chunk boundaries: arrivals spread over the run
+    reqs = [S.ShotRequest(**{**r.__dict__, "arrival": 0.02 * k}) for k, r in enumerate(reqs)]
+    print("case staggered joins", flush=True)
+    one = max(S.prepare(r).bytes for r in reqs)
+    srv = S.Server(b, budget_bytes=2 * one, chunk=8, capacity=8)
+    seen = []
+    done = srv.serve(reqs, realtime=True, on_complete=lambda c: seen.append(c.rid))
+    assert len(done) == 8 and seen == [c.rid for c in done]
+    for c in done:
+        assert np.array_equal(c.gather, _alone(b, reqs[c.rid]))
+        assert c.admitted >= c.arrival and c.done >= c.admitted
+
+
+@gpu
+def test_device_bytes_estimate_matches_library(b):
+    from fomo_julia_amd import elastic2d as E
+    for storage in ("f32", "bf16", "f16c"):
+        r = S.c5_stream(1, seed=3, storages=(storage,), **SMALL)[0]
+        p = S.prepare(r)
+        dm = E._DeviceModel(b, p.M, p.H, p.a, p.p)
+        W = E.Wavefield(p.M.nx, p.M.nz, b)
+        W._bind(dm)
+        lib = b.lib
+        got = lib.fomo_model_device_bytes(dm.h) + lib.fomo_wavefield_device_bytes(W._h)
+        gather = r.nt * len(p.rec.i) * 4
+        assert abs(p.bytes - (got + gather)) <= 3 * 4096 + 4 * (p.M.nz + 36 + 8), storage
