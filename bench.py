@@ -21,6 +21,7 @@ CPU oracle timed on a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -84,6 +85,106 @@ def cpu_baseline(case, steps: int, threads: int):
     return cells / el / 1e6, el
 
 
+def bench_batch(args) -> dict:
+    """Batched shots: 64 shots on the Marmousi-shaped 1700 x 350 grid
+    (dx = dz = 10 m, fd_order 12, nbc 40, free surface, nt 4000, 25 Hz Ricker
+    pressure sources evenly spaced on the surface, 1700 surface vz receivers),
+    all in one persistent launch per kernel with captured steps and an
+    autotuned tile shape.  Timed: the batch's nt steps (tuning excluded)."""
+    import torch
+    from fomo_julia_amd import elastic2d as E
+    from fomo_julia_amd import init as I
+    from fomo_julia_amd import media
+    from fomo_julia_amd import tune as T
+    F32 = np.float32
+    n_shots, nt = args.shots, args.steps if args.steps != 11520 else 4000
+    dx = F32(10.0)
+    vp, vs, rho = media.marmousi_like()
+    dt = media.cfl_dt(vp, dx, 0.4)
+    M = E.init_medium(vp, vs, rho, dx, dx, 40, 12, None, free_surface=True, storage=args.storage)
+    H = E.init_habc(M.nx, M.nz, 40, dt, dx, dx, I.julia_mean_f32(vp), None, weights=False)
+    p = E.SimParams(dt, nt, dx, dx, 12)
+    a = I.get_fd_coefficients(12)
+    wav = I.ricker_wavelet(F32(25.0), dt, nt)
+    b = E.backend("mi355x")
+    W = E.Wavefield(M.nx, M.nz, b)
+    dm = E.bind(b, W, M, H, a, p)
+    nx_in = vp.shape[0]
+    rec = E.setup_receivers(np.arange(nx_in, dtype=F32) * dx, np.zeros(nx_in, F32), M, type="vz")
+    xs = ((np.arange(n_shots) + 0.5) * (float(M.x_max) / n_shots)).astype(F32)
+    wfs, dss = [], []
+    for x in xs:
+        src = E.Source(I.coord_to_index(F32(x), M.dx, M.pad), I.coord_to_index(F32(0.0), M.dz, M.pad),
+                       wav, "pressure")
+        w = E.Wavefield(M.nx, M.nz, b)
+        w._bind(dm)
+        wfs.append(w)
+        dss.append(E._DeviceShot(dm, src, rec, nt))
+    tuning = T.autotune_batch(b, [dm] * n_shots, wfs, dss)
+    lib = b.lib
+    arr = lambda hs: (ctypes.c_void_p * n_shots)(*[h.value for h in hs])
+    bh = ctypes.c_void_p()
+    b.check(lib.fomo_batch_create(b.ctx, n_shots, arr([dm.h] * n_shots), arr([w._h for w in wfs]),
+                                  arr([d.h for d in dss]), ctypes.byref(bh)), "batch_create")
+    stream = torch.cuda.ExternalStream(b.stream_handle())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    b.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    b.check(lib.fomo_batch_run(bh, 1, nt), "batch_run")
+    e1.record(stream)
+    b.synchronize()
+    gpu_s = e0.elapsed_time(e1) / 1e3
+    gathers = [d.gather() for d in dss]
+    wall = time.perf_counter() - t0
+    lib.fomo_batch_destroy(bh)
+    cells = M.nx * M.nz * nt * n_shots
+    return {"metric": "shots/hour and Mcell-updates/s, batched shots, 1 GPU",
+            "value": round(n_shots / gpu_s * 3600.0, 1), "unit": "shots/hour", "n_gpus": 1,
+            "steps": nt, "higher_is_better": True, "dtype": "f32", "storage": args.storage,
+            "data": "synthetic (seeded Marmousi-like medium)",
+            "config": {"workload": f"{n_shots} batched shots, Marmousi-shaped 1700x350, dx=dz=10 m, "
+                                   f"fd_order=12, nbc=40, nt={nt}, 25 Hz Ricker, 1700 receivers, "
+                                   "one persistent launch per kernel, captured steps",
+                       "padded_cells_per_shot": M.nx * M.nz},
+            "mcell_updates_per_s": round(cells / gpu_s / 1e6, 1),
+            "ms_per_step": round(gpu_s * 1e3 / nt, 4), "gpu_seconds": round(gpu_s, 3),
+            "wall_seconds_with_gathers": round(wall, 3),
+            "tiling": {"cols_per_lane": tuning["tiling"][0], "rows_per_tile": tuning["tiling"][1],
+                       "trials_ms_per_step": tuning["trials"]},
+            "gathers_finite": bool(all(np.isfinite(g).all() for g in gathers))}
+
+
+def bench_serve(n_req: int, seed: int = 5, load: float = 0.8) -> dict:
+    """Served stream (requests with grids nx 300-2000, nz 300-1200, nt
+    1000-6000, fd_order 4/8/12, FP32 and BF16 storage alternating): serial
+    plain FP32 execution, then the continuous-batching server with every
+    request queued at once (capacity), then with Poisson arrivals at `load`
+    times the serial service rate (latency)."""
+    from fomo_julia_amd import elastic2d as E
+    from fomo_julia_amd import serve as S
+    b = E.backend("mi355x")
+    reqs = S.c5_stream(n_req, seed=seed)
+    t = time.perf_counter()
+    ser = S.serial(reqs, b, storage="f32")
+    w_ser = time.perf_counter() - t
+    srv = S.Server(b, chunk=64, capacity=64)
+    t = time.perf_counter()
+    sat = srv.serve(reqs, realtime=False)
+    w_sat = time.perf_counter() - t
+    rate = load / (w_ser / n_req)
+    srv2 = S.Server(b, chunk=64, capacity=64)
+    t = time.perf_counter()
+    poi = srv2.serve(S.with_arrivals(reqs, rate), realtime=True)
+    w_poi = time.perf_counter() - t
+    s_ser, s_sat, s_poi = S.summarize(ser, w_ser), S.summarize(sat, w_sat), S.summarize(poi, w_poi)
+    return {"requests": n_req, "serial_f32": s_ser, "served_capacity": s_sat,
+            "served_poisson": {**s_poi, "arrival_rate_per_s": round(rate, 3), "load": load},
+            "speedup_shots_per_hour": round(s_sat["shots_per_hour"] / s_ser["shots_per_hour"], 3),
+            "server_stats": {k: (round(v, 3) if isinstance(v, float) else v)
+                             for k, v in srv.stats.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,11 +197,25 @@ def main():
     ap.add_argument("--two-pass", action="store_true", help="time the two-pass in-place step instead")
     ap.add_argument("--storage", default="f32", choices=sorted(BYTES_FUSED_STORAGE),
                     help="device storage of coefficients/fields (arithmetic stays FP32)")
+    ap.add_argument("--workload", default="seam", choices=("seam", "batch", "serve"),
+                    help="seam: the single SEAM-shaped shot (default); batch: 64 batched "
+                         "Marmousi-shaped shots; serve: the served request stream")
+    ap.add_argument("--shots", type=int, default=64, help="shots of the batch workload")
+    ap.add_argument("--requests", type=int, default=16,
+                    help="requests of the served sample (default run) or stream (serve)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        print(json.dumps(bench_batch(args)))
+        return
+    if args.workload == "serve":
+        print(json.dumps({"metric": "shots/hour + p50/p99 latency (served), 1 GPU",
+                          **bench_serve(args.requests)}))
+        return
 
     import torch
     from fomo_julia_amd import elastic2d as E
@@ -199,6 +314,13 @@ def main():
             cpu = {"value": None, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
                    "sample": f"failed: {e}"}
 
+    served = None
+    if not args.no_serve and args.requests > 0:
+        try:
+            served = bench_serve(args.requests)
+        except Exception as e:  # reported, never fatal
+            served = {"error": str(e)}
+
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -231,6 +353,7 @@ def main():
         "step_gbs": round(step_gbs, 1),
         "step_frac_hbm": round(step_gbs / HBM_PEAK_GBS, 4),
         "cpu_baseline": cpu,
+        "served": served,
         "reference_published": {"cpu_wall_min": 35, "rtx3060_wall_min_upper": 3,
                                 "derived_cpu_mcells": 98.1, "derived_rtx3060_mcells_lower": 1144,
                                 "note": "README wall times for one SEAM shot, other hardware"},

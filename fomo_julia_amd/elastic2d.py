@@ -593,7 +593,7 @@ def _create_source(M: Medium, shot: ShotConfig, wavelet, source_type: str) -> So
 
 def run_shots(b, W, M, H, a, rec_template: Receivers, shot_config: MultiShotConfig, p: SimParams,
               on_shot_complete=None, on_step=None, progress: bool = False,
-              batch: Optional[int] = None) -> list:
+              batch: Optional[int] = None, tune: bool = False) -> list:
     """``run_shots!`` (simulation/shot_manager.jl:145-180).
 
     Results come back in shot order and ``on_shot_complete`` is called in shot
@@ -602,7 +602,9 @@ def run_shots(b, W, M, H, a, rec_template: Receivers, shot_config: MultiShotConf
     all their tiles), each on its own device wavefield; ``batch=1`` (or an
     ``on_step`` callback) gives the reference's one-shot-at-a-time order on W.
     Default batch: as many shots as fit in a quarter of the free GPU memory,
-    at most 64."""
+    at most 64.  ``tune=True`` chooses the tile shape per batch by timing
+    candidates on the batch itself (``fomo_julia_amd.tune``); the choice is
+    in ``run_shots.last_tuning``."""
     shots = list(shot_config.shots)
     n = len(shots)
     results = [None] * n
@@ -636,6 +638,9 @@ def run_shots(b, W, M, H, a, rec_template: Receivers, shot_config: MultiShotConf
             srcs.append((src, rec))
             dss.append(_DeviceShot(dm, src, rec, p.nt))
         k = len(group)
+        if tune:
+            from . import tune as T
+            run_shots.last_tuning = T.autotune_batch(b, [dm] * k, wfs, dss)
         arr = lambda hs: (ctypes.c_void_p * k)(*[h.value for h in hs])
         bh = ctypes.c_void_p()
         rc = lib.fomo_batch_create(b.ctx, k, arr([dm.h] * k), arr([w._h for w in wfs]),
@@ -656,6 +661,9 @@ def run_shots(b, W, M, H, a, rec_template: Receivers, shot_config: MultiShotConf
                 on_shot_complete(res)
         done += k
     return results
+
+
+run_shots.last_tuning = None
 
 
 def _free_device_bytes() -> int:

@@ -224,6 +224,10 @@ class Server:
         self.capacity = capacity
         self.workers = workers
         self.pools: dict = {}
+        # seconds spent: preparing requests (worker threads, summed), creating
+        # device objects, waiting for the GPU; chunks launched
+        self.stats = {"prepare_s": 0.0, "admit_s": 0.0, "gpu_wait_s": 0.0, "chunks": 0,
+                      "max_resident": 0}
 
     def _pool(self, key) -> Pool:
         if key not in self.pools:
@@ -242,41 +246,83 @@ class Server:
         done: list = []
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
+        st = self.stats
+
+        creating: deque = deque()  # (pool, future of _Member): memory reserved, objects being built
+        slots: dict = {}           # pool key -> members + members being created
+
+        def make_member(prep, t_admit):
+            ta = time.perf_counter()
+            m = _Member(self.b, prep, t_admit)
+            m.create_s = time.perf_counter() - ta
+            return m
+
+        def admit(ex):
+            # first come first served among the prepared requests that fit; the
+            # device objects are built in a worker thread while the GPU runs
+            nonlocal used, waiting
+            keep = deque()
+            while waiting:
+                r, fut = waiting.popleft()
+                if fut.done():
+                    prep = fut.result()
+                    key = (r.fd_order, r.storage)
+                    pool = self._pool(key)
+                    fits = used + prep.bytes <= self.budget or not (active or creating)
+                    if fits and slots.get(key, 0) < pool.capacity:
+                        used += prep.bytes
+                        slots[key] = slots.get(key, 0) + 1
+                        st["prepare_s"] += prep.prep_s
+                        creating.append((pool, ex.submit(make_member, prep, now())))
+                        continue
+                keep.append((r, fut))
+            waiting = keep
+            # members whose device objects are ready join their pools
+            still = deque()
+            while creating:
+                pool, fut = creating.popleft()
+                if fut.done():
+                    m = fut.result()
+                    st["admit_s"] += m.create_s
+                    pool.add(m)
+                    active[m.rid] = (pool, m)
+                else:
+                    still.append((pool, fut))
+            creating.extend(still)
+            st["max_resident"] = max(st["max_resident"], len(active))
+
+        def arrivals(ex):
+            t = now()
+            while pending and (not realtime or pending[0].arrival <= t):
+                r = pending.popleft()
+                waiting.append((r, ex.submit(prepare, r)))
+
         with ThreadPoolExecutor(max_workers=self.workers) as ex:
-            while pending or waiting or active:
-                t = now()
-                while pending and (not realtime or pending[0].arrival <= t):
-                    r = pending.popleft()
-                    waiting.append((r, ex.submit(prepare, r)))
-                # admission: first come first served among prepared requests that fit
-                keep = deque()
-                while waiting:
-                    r, fut = waiting.popleft()
-                    if fut.done():
-                        prep = fut.result()
-                        pool = self._pool((r.fd_order, r.storage))
-                        fits = used + prep.bytes <= self.budget or not active
-                        if fits and len(pool.members) < pool.capacity:
-                            m = _Member(self.b, prep, now())
-                            pool.add(m)
-                            active[r.rid] = (pool, m)
-                            used += prep.bytes
-                            continue
-                    keep.append((r, fut))
-                waiting = keep
+            while pending or waiting or active or creating:
+                arrivals(ex)
                 if not active:
-                    if waiting:
-                        waiting[0][1].result()  # wait for a preparation
-                    elif pending and realtime:
-                        time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
-                    continue
-                # one chunk for every pool with members (asynchronous launches)
+                    admit(ex)
+                    if not active:
+                        if creating:
+                            creating[0][1].result()  # wait for device objects
+                        elif waiting:
+                            waiting[0][1].result()  # wait for a preparation
+                        elif pending and realtime:
+                            time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
+                        continue
+                # one chunk for every pool with members (asynchronous launches);
+                # requests admitted meanwhile join at the next chunk
                 for pool in self.pools.values():
                     if pool.members:
                         pool.run(self.chunk)
+                        st["chunks"] += 1
                 finished = [(pool, m) for pool, m in active.values()
-                             if pool.next_step(m) > m.nt]
+                            if pool.next_step(m) > m.nt]
+                arrivals(ex)
+                admit(ex)
+                tw = time.perf_counter()
                 self.b.synchronize()
+                st["gpu_wait_s"] += time.perf_counter() - tw
                 t_done = now()
                 for pool, m in finished:
                     pool.remove(m)
@@ -289,6 +335,7 @@ class Server:
                         on_complete(c)
                     del active[m.rid]
                     used -= m.prep.bytes
+                    slots[pool.key] -= 1
                     del m
         return done
 
