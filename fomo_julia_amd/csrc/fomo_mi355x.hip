@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -120,6 +121,13 @@ struct fomo_batch {
 };
 
 static thread_local std::string g_err;
+
+// Held while a graph is captured and around every call that allocates, frees
+// or copies device memory outside the context's stream: such a call from
+// another thread during a capture (hipMalloc, hipFree, synchronous copies)
+// invalidates the capture.  Recursive: destroy paths nest.
+static std::recursive_mutex g_api_mu;
+#define API_LOCK std::lock_guard<std::recursive_mutex> api_lock_(g_api_mu)
 
 static int fail(fomo_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -329,6 +337,7 @@ extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, cons
                                      const float* rho_vx, const float* rho_vz, const float* lam,
                                      const float* mu_txx, const float* mu_txz, const float* w_vx,
                                      const float* w_vz, const float* w_tau, fomo_model** out) {
+  API_LOCK;
   if (!c || !d || !a || !out) return fail(c, FOMO_ERR_ARG, "NULL argument");
   if (!rho_vx || !rho_vz || !lam || !mu_txx || !mu_txz)
     return fail(c, FOMO_ERR_ARG, "medium array is NULL");
@@ -448,6 +457,7 @@ extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, cons
 }
 
 extern "C" int32_t fomo_model_destroy(fomo_model* m) {
+  API_LOCK;
   if (!m) return FOMO_OK;
   (void)hipSetDevice(m->ctx->device);
   (void)hipStreamSynchronize(m->ctx->stream);
@@ -466,6 +476,7 @@ static int refresh_backup(fomo_wavefield* w);
 static void default_field_scales(const fomo_model* m, float* fsc);
 
 extern "C" int32_t fomo_wavefield_create(fomo_model* m, fomo_wavefield** out) {
+  API_LOCK;
   if (!m || !out) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   fomo_ctx* c = m->ctx;
   auto w = std::make_unique<fomo_wavefield>();
@@ -488,6 +499,7 @@ extern "C" int32_t fomo_wavefield_create(fomo_model* m, fomo_wavefield** out) {
 }
 
 extern "C" int32_t fomo_wavefield_destroy(fomo_wavefield* w) {
+  API_LOCK;
   if (!w) return FOMO_OK;
   (void)hipSetDevice(w->m->ctx->device);
   (void)hipStreamSynchronize(w->m->ctx->stream);
@@ -596,6 +608,7 @@ extern "C" int32_t fomo_shot_create(fomo_model* m, int32_t src_i, int32_t src_j,
                                     const float* wavelet, int32_t nw, int32_t nt,
                                     const int32_t* rec_i, const int32_t* rec_j,
                                     const int32_t* rec_type, int32_t n_rec, fomo_shot** out) {
+  API_LOCK;
   if (!m || !out) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   fomo_ctx* c = m->ctx;
   const int nx = m->d.nx, nz = m->d.nz;
@@ -674,6 +687,7 @@ extern "C" int32_t fomo_shot_field_scales(const fomo_shot* s, float* out) {
 }
 
 extern "C" int32_t fomo_shot_destroy(fomo_shot* s) {
+  API_LOCK;
   if (!s) return FOMO_OK;
   (void)hipSetDevice(s->m->ctx->device);
   (void)hipStreamSynchronize(s->m->ctx->stream);
@@ -1163,6 +1177,29 @@ static void launch_advance(const fomo_batch* b, int by) {
                      b->dn, by);
 }
 
+// Captures one chunk of G whole steps, starting from state buffer p, into
+// b->gexec[p] (the step counters advance by G as the last node).
+static int capture_chunk(fomo_batch* b, int p) {
+  API_LOCK;
+  fomo_ctx* c = b->ctx;
+  const int G = b->graph_steps;
+  const int flip = b->step_mode == 1 ? 1 : 0;
+  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (int g = 0; g < G; ++g) launch_step(b, (p + flip * g) & 1, g);
+  launch_advance(b, G);
+  const hipError_t le = hipGetLastError();
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (le != hipSuccess || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(c, FOMO_ERR_HIP, std::string("step capture: ") +
+                                     hipGetErrorString(le != hipSuccess ? le : e));
+  }
+  b->graph[p] = graph;
+  HIP_TRY(c, hipGraphInstantiate(&b->gexec[p], graph, nullptr, nullptr, 0));
+  return FOMO_OK;
+}
+
 static int check_launch(fomo_ctx* c, const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(c, FOMO_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -1207,13 +1244,8 @@ static int run_batch(fomo_batch* b, int k_first, int n_steps) {
   const int chunks = n_steps / G, rem = n_steps % G;
   const int flip = b->step_mode == 1 ? 1 : 0;
   if (chunks > 0 && !b->gexec[parity]) {
-    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int g = 0; g < G; ++g) launch_step(b, (parity + flip * g) & 1, g);
-    launch_advance(b, G);
-    hipGraph_t graph = nullptr;
-    HIP_TRY(c, hipStreamEndCapture(c->stream, &graph));
-    b->graph[parity] = graph;
-    HIP_TRY(c, hipGraphInstantiate(&b->gexec[parity], graph, nullptr, nullptr, 0));
+    int rc = capture_chunk(b, parity);
+    if (rc) return rc;
   }
   for (int i = 0; i < chunks; ++i) HIP_TRY(c, hipGraphLaunch(b->gexec[parity], c->stream));
   for (int g = 0; g < rem; ++g) launch_step(b, (parity + flip * g) & 1, g);
@@ -1342,12 +1374,14 @@ extern "C" int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot
 extern "C" int32_t fomo_batch_create(fomo_ctx* c, int32_t n, fomo_model* const* ms,
                                      fomo_wavefield* const* ws, fomo_shot* const* ss,
                                      fomo_batch** out) {
+  API_LOCK;
   if (!c || !ms || !ws || !ss || !out) return fail(c, FOMO_ERR_ARG, "NULL argument");
   HIP_TRY(c, hipSetDevice(c->device));
   return build_batch(c, n, ms, ws, ss, out);
 }
 
 extern "C" int32_t fomo_batch_destroy(fomo_batch* b) {
+  API_LOCK;
   if (!b) return FOMO_OK;
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
@@ -1396,6 +1430,7 @@ static void pool_free(fomo_pool* p) {
 
 extern "C" int32_t fomo_pool_create(fomo_ctx* c, int32_t fd_order, int32_t storage, int32_t capacity,
                                     fomo_pool** out) {
+  API_LOCK;
   if (!c || !out) return fail(c, FOMO_ERR_ARG, "NULL argument");
   if (fd_order < 2 || fd_order > 2 * MAX_M || fd_order % 2)
     return fail(c, FOMO_ERR_UNSUPPORTED, "fd_order must be even, 2..16");
@@ -1449,6 +1484,7 @@ extern "C" int32_t fomo_pool_create(fomo_ctx* c, int32_t fd_order, int32_t stora
 }
 
 extern "C" int32_t fomo_pool_destroy(fomo_pool* p) {
+  API_LOCK;
   if (!p) return FOMO_OK;
   (void)hipSetDevice(p->ctx->device);
   (void)hipStreamSynchronize(p->ctx->stream);
@@ -1577,13 +1613,8 @@ extern "C" int32_t fomo_pool_run(fomo_pool* p, int32_t n_steps) {
   const int G = b->graph_steps;  // even: every chunk starts on table 0
   const int chunks = n_steps / G, rem = n_steps % G;
   if (chunks > 0 && !b->gexec[0]) {
-    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int g = 0; g < G; ++g) launch_step(b, g & 1, g);
-    launch_advance(b, G);
-    hipGraph_t graph = nullptr;
-    HIP_TRY(c, hipStreamEndCapture(c->stream, &graph));
-    b->graph[0] = graph;
-    HIP_TRY(c, hipGraphInstantiate(&b->gexec[0], graph, nullptr, nullptr, 0));
+    int rc = capture_chunk(b, 0);
+    if (rc) return rc;
   }
   for (int i = 0; i < chunks; ++i) HIP_TRY(c, hipGraphLaunch(b->gexec[0], c->stream));
   for (int g = 0; g < rem; ++g) launch_step(b, g & 1, g);
