@@ -230,9 +230,6 @@ def main():
     cells = M.nx * M.nz
 
     b = E.backend("mi355x")
-    if args.tiling:
-        cpt, tz, gs = (int(v) for v in args.tiling.split(","))
-        b.set_tiling(cpt, tz, gs)
     fused = not args.two_pass
     b.set_step_mode(1 if fused else 0)
     W = E.Wavefield(M.nx, M.nz, b)
@@ -240,6 +237,13 @@ def main():
     ds = E._shot_for(dm, src, rec, p.nt)
     lib = b.lib
     stream = torch.cuda.ExternalStream(b.stream_handle())
+    tuning = None
+    if args.tiling:
+        cpt, tz, gs = (int(v) for v in args.tiling.split(","))
+        b.set_tiling(cpt, tz, gs)
+    elif fused:
+        from fomo_julia_amd import tune as T
+        tuning = T.autotune_batch(b, [dm], [W], [ds])
 
     # warm-up: graph capture, clocks
     if args.warmup:
@@ -349,6 +353,9 @@ def main():
                      "kernel": main_name, "kernel_ms": round(main_ms, 5),
                      "algorithmic_bytes_per_launch": main_bytes * cells},
         "kernels_ms": {k: round(v, 5) for k, v in kernels.items()},
+        "tiling": ({"cols_per_lane": tuning["tiling"][0], "rows_per_tile": tuning["tiling"][1],
+                    "autotuned": True, "trials_ms_per_step": tuning["trials"]} if tuning else
+                   {"fixed": args.tiling or "library default"}),
         "step_mode": "fused" if fused else "two-pass",
         "step_gbs": round(step_gbs, 1),
         "step_frac_hbm": round(step_gbs / HBM_PEAK_GBS, 4),

@@ -81,6 +81,7 @@ SIGNATURES = {
     "fomo_pool_remove": (c_int32, [c_void_p, c_void_p]),
     "fomo_pool_run": (c_int32, [c_void_p, c_int32]),
     "fomo_pool_size": (c_int32, [c_void_p]),
+    "fomo_pool_synchronize": (c_int32, [c_void_p]),
     "fomo_pool_next_step": (c_int32, [c_void_p, c_void_p, P_i32]),
 }
 

@@ -118,7 +118,10 @@ struct fomo_batch {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // by the parity the chunk starts from
   hipGraph_t graph[2] = {nullptr, nullptr};
   int grid_fixed[PART_COUNT] = {};  // > 0: fixed grid (pools: persistent over any membership)
+  hipStream_t ls = nullptr;         // stream of all its launches (null: the context's)
 };
+
+static hipStream_t lstream(const fomo_batch* b) { return b->ls ? b->ls : b->ctx->stream; }
 
 static thread_local std::string g_err;
 
@@ -1059,7 +1062,7 @@ static BatchArgs args_for(const fomo_batch* b, int table, int part, int k_off) {
 template <int K, int M, int CPT>
 static void launch_stencil_mc(const fomo_batch* b, int table, int kind, int flags, int k_off) {
   dim3 block(64 * WPB);
-  hipStream_t st = b->ctx->stream;
+  hipStream_t st = lstream(b);
   if (kind == 2) {  // grid rounded up to whole XCD rounds (see xcd_block)
     const dim3 grid(round_up(blocks_for(b, PART_FUSED), NUM_XCD));
     if (b->same)
@@ -1127,22 +1130,22 @@ static void launch_habc(const fomo_batch* b, int table, bool stress, int k_off) 
   const BatchArgs a = args_for(b, table, part, k_off);
   const dim3 g(b->grid_fixed[part] > 0 ? b->grid_fixed[part] : std::max(1, items));
   if (b->storage == FOMO_STORAGE_BF16)
-    hipLaunchKernelGGL(k_habc<ST_BF16>, g, dim3(64), 0, b->ctx->stream, a, stress ? 1 : 0);
+    hipLaunchKernelGGL(k_habc<ST_BF16>, g, dim3(64), 0, lstream(b), a, stress ? 1 : 0);
   else if (b->storage == FOMO_STORAGE_F16)
-    hipLaunchKernelGGL(k_habc<ST_F16>, g, dim3(64), 0, b->ctx->stream, a, stress ? 1 : 0);
+    hipLaunchKernelGGL(k_habc<ST_F16>, g, dim3(64), 0, lstream(b), a, stress ? 1 : 0);
   else
-    hipLaunchKernelGGL(k_habc<ST_F32>, g, dim3(64), 0, b->ctx->stream, a, stress ? 1 : 0);
+    hipLaunchKernelGGL(k_habc<ST_F32>, g, dim3(64), 0, lstream(b), a, stress ? 1 : 0);
 }
 
 static void launch_fsrec(const fomo_batch* b, int table, int mode, int k_off) {
   const BatchArgs a = args_for(b, table, PART_FSREC, k_off);
   const dim3 g(blocks_for(b, PART_FSREC)), blk(64 * WPB);
   if (b->storage == FOMO_STORAGE_BF16)
-    hipLaunchKernelGGL((k_fsrec<WPB, ST_BF16>), g, blk, 0, b->ctx->stream, a, mode);
+    hipLaunchKernelGGL((k_fsrec<WPB, ST_BF16>), g, blk, 0, lstream(b), a, mode);
   else if (b->storage == FOMO_STORAGE_F16)
-    hipLaunchKernelGGL((k_fsrec<WPB, ST_F16>), g, blk, 0, b->ctx->stream, a, mode);
+    hipLaunchKernelGGL((k_fsrec<WPB, ST_F16>), g, blk, 0, lstream(b), a, mode);
   else
-    hipLaunchKernelGGL((k_fsrec<WPB, ST_F32>), g, blk, 0, b->ctx->stream, a, mode);
+    hipLaunchKernelGGL((k_fsrec<WPB, ST_F32>), g, blk, 0, lstream(b), a, mode);
 }
 
 // One step (time_stepper.jl:39-63) of every shot, starting from state buffer
@@ -1168,12 +1171,12 @@ static void launch_step(const fomo_batch* b, int parity, int k_off) {
 }
 
 static void launch_set_step(const fomo_batch* b, int k) {
-  hipLaunchKernelGGL(k_set_step, dim3((b->n + 255) / 256), dim3(256), 0, b->ctx->stream, b->ds,
+  hipLaunchKernelGGL(k_set_step, dim3((b->n + 255) / 256), dim3(256), 0, lstream(b), b->ds,
                      b->dn, k);
 }
 // Reads the kp of rows [0, n) of table TBL_IN0 (filled for batches and pools).
 static void launch_advance(const fomo_batch* b, int by) {
-  hipLaunchKernelGGL(k_advance, dim3((b->n + 255) / 256), dim3(256), 0, b->ctx->stream, b->ds,
+  hipLaunchKernelGGL(k_advance, dim3((b->n + 255) / 256), dim3(256), 0, lstream(b), b->ds,
                      b->dn, by);
 }
 
@@ -1184,12 +1187,13 @@ static int capture_chunk(fomo_batch* b, int p) {
   fomo_ctx* c = b->ctx;
   const int G = b->graph_steps;
   const int flip = b->step_mode == 1 ? 1 : 0;
-  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipStream_t st = lstream(b);
+  HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   for (int g = 0; g < G; ++g) launch_step(b, (p + flip * g) & 1, g);
   launch_advance(b, G);
   const hipError_t le = hipGetLastError();
   hipGraph_t graph = nullptr;
-  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  const hipError_t e = hipStreamEndCapture(st, &graph);
   if (le != hipSuccess || e != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
     return fail(c, FOMO_ERR_HIP, std::string("step capture: ") +
@@ -1413,6 +1417,7 @@ struct fomo_pool {
   std::vector<fomo_shot*> ss;
   std::vector<int> c0;      // member's current buffer when it joined
   std::vector<int> k_next;  // member's next step (1-based), host view
+  hipStream_t stream = nullptr;  // the pool's own stream: pools run beside each other
   ShotDev* h_rows = nullptr;  // pinned staging: [2][capacity] fused-table rows
   int* h_prefix = nullptr;    // pinned staging: [PART_COUNT][capacity+1], then the live count
   hipEvent_t uploaded = nullptr;
@@ -1421,7 +1426,9 @@ struct fomo_pool {
 
 static void pool_free(fomo_pool* p) {
   if (!p) return;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->b) destroy_batch(p->b);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
   if (p->h_rows) (void)hipHostFree(p->h_rows);
   if (p->h_prefix) (void)hipHostFree(p->h_prefix);
   if (p->uploaded) (void)hipEventDestroy(p->uploaded);
@@ -1477,8 +1484,10 @@ extern "C" int32_t fomo_pool_create(fomo_ctx* c, int32_t fd_order, int32_t stora
   HIP_TRY(c, hipMemset(b->dprefix, 0, b->hprefix.size() * sizeof(int)));
   HIP_TRY(c, hipHostMalloc((void**)&p->h_rows, 2 * (size_t)capacity * sizeof(ShotDev)));
   HIP_TRY(c, hipHostMalloc((void**)&p->h_prefix, (b->hprefix.size() + 1) * sizeof(int)));
+  HIP_TRY(c, hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+  b->ls = p->stream;
   HIP_TRY(c, hipEventCreateWithFlags(&p->uploaded, hipEventDisableTiming));
-  HIP_TRY(c, hipEventRecord(p->uploaded, c->stream));
+  HIP_TRY(c, hipEventRecord(p->uploaded, p->stream));
   *out = p.release();
   return FOMO_OK;
 }
@@ -1516,7 +1525,7 @@ extern "C" int32_t fomo_pool_add(fomo_pool* p, fomo_model* m, fomo_wavefield* w,
   int rc = adopt_scales(w, s);
   if (rc) return rc;
   HIP_TRY(c, hipSetDevice(c->device));
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, s->dk, 1);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, p->stream, s->dk, 1);
   p->ms.push_back(m);
   p->ws.push_back(w);
   p->ss.push_back(s);
@@ -1531,7 +1540,7 @@ extern "C" int32_t fomo_pool_remove(fomo_pool* p, fomo_shot* s) {
   const int i = pool_find(p, s);
   if (i < 0) return fail(p->ctx, FOMO_ERR_ARG, "shot is not in the pool");
   HIP_TRY(p->ctx, hipSetDevice(p->ctx->device));
-  HIP_TRY(p->ctx, hipStreamSynchronize(p->ctx->stream));  // its last steps have run
+  HIP_TRY(p->ctx, hipStreamSynchronize(p->stream));  // its last steps have run
   const int done = std::min(p->k_next[i] - 1, s->nt);
   fomo_wavefield* w = p->ws[i];
   w->cur = p->c0[i] ^ (done & 1);
@@ -1546,6 +1555,13 @@ extern "C" int32_t fomo_pool_remove(fomo_pool* p, fomo_shot* s) {
 }
 
 extern "C" int32_t fomo_pool_size(const fomo_pool* p) { return p ? (int32_t)p->ss.size() : 0; }
+
+extern "C" int32_t fomo_pool_synchronize(fomo_pool* p) {
+  if (!p) return fail(nullptr, FOMO_ERR_ARG, "NULL pool");
+  HIP_TRY(p->ctx, hipSetDevice(p->ctx->device));
+  HIP_TRY(p->ctx, hipStreamSynchronize(p->stream));
+  return FOMO_OK;
+}
 
 extern "C" int32_t fomo_pool_next_step(const fomo_pool* p, const fomo_shot* s, int32_t* k_next) {
   if (!p || !s || !k_next) return fail(p ? p->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
@@ -1584,17 +1600,17 @@ static int pool_upload(fomo_pool* p) {
   if (n > 0) {
     // table TBL_IN0 only serves the step counters (k_advance reads its kp)
     HIP_TRY(c, hipMemcpyAsync(b->ds + (size_t)TBL_IN0 * C, p->h_rows, (size_t)n * sizeof(ShotDev),
-                              hipMemcpyHostToDevice, c->stream));
+                              hipMemcpyHostToDevice, p->stream));
     HIP_TRY(c, hipMemcpyAsync(b->ds + (size_t)TBL_FUSED0 * C, p->h_rows, (size_t)n * sizeof(ShotDev),
-                              hipMemcpyHostToDevice, c->stream));
+                              hipMemcpyHostToDevice, p->stream));
     HIP_TRY(c, hipMemcpyAsync(b->ds + (size_t)TBL_FUSED1 * C, p->h_rows + C, (size_t)n * sizeof(ShotDev),
-                              hipMemcpyHostToDevice, c->stream));
+                              hipMemcpyHostToDevice, p->stream));
   }
   HIP_TRY(c, hipMemcpyAsync(b->dprefix, p->h_prefix, b->hprefix.size() * sizeof(int),
-                            hipMemcpyHostToDevice, c->stream));
+                            hipMemcpyHostToDevice, p->stream));
   HIP_TRY(c, hipMemcpyAsync(b->dn, p->h_prefix + (size_t)PART_COUNT * (C + 1), sizeof(int),
-                            hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(c, hipEventRecord(p->uploaded, c->stream));
+                            hipMemcpyHostToDevice, p->stream));
+  HIP_TRY(c, hipEventRecord(p->uploaded, p->stream));
   p->dirty = false;
   return FOMO_OK;
 }
@@ -1616,7 +1632,7 @@ extern "C" int32_t fomo_pool_run(fomo_pool* p, int32_t n_steps) {
     int rc = capture_chunk(b, 0);
     if (rc) return rc;
   }
-  for (int i = 0; i < chunks; ++i) HIP_TRY(c, hipGraphLaunch(b->gexec[0], c->stream));
+  for (int i = 0; i < chunks; ++i) HIP_TRY(c, hipGraphLaunch(b->gexec[0], p->stream));
   for (int g = 0; g < rem; ++g) launch_step(b, g & 1, g);
   if (rem) launch_advance(b, rem);
   for (int& k : p->k_next) k += n_steps;

@@ -183,6 +183,9 @@ class Pool:
     def run(self, n_steps: int) -> None:
         self.b.check(self.b.lib.fomo_pool_run(self.h, n_steps), "fomo_pool_run")
 
+    def synchronize(self) -> None:
+        self.b.check(self.b.lib.fomo_pool_synchronize(self.h), "fomo_pool_synchronize")
+
     def next_step(self, m) -> int:
         k = ctypes.c_int32()
         self.b.check(self.b.lib.fomo_pool_next_step(self.h, m.ds.h, ctypes.byref(k)), "next_step")
@@ -321,7 +324,8 @@ class Server:
                 arrivals(ex)
                 admit(ex)
                 tw = time.perf_counter()
-                self.b.synchronize()
+                for pool in self.pools.values():
+                    pool.synchronize()
                 st["gpu_wait_s"] += time.perf_counter() - tw
                 t_done = now()
                 for pool, m in finished:

@@ -196,7 +196,10 @@ int32_t fomo_batch_run(fomo_batch* b, int32_t k_first, int32_t n_steps);
  * advances with every fomo_pool_run, and leaves with fomo_pool_remove
  * (which waits for the stream).  Members past their nt are skipped until
  * removed.  Members need dtx == dtz.  Runs are asynchronous and take an even
- * number of steps; one captured graph per pool serves every membership. */
+ * number of steps; one captured graph per pool serves every membership.
+ * Each pool has its own stream, so the runs of different pools (other FD
+ * orders or storage types) proceed side by side on the GPU; wait for a pool
+ * with fomo_pool_synchronize (fomo_ctx_synchronize does not). */
 typedef struct fomo_pool fomo_pool;
 int32_t fomo_pool_create(fomo_ctx* ctx, int32_t fd_order, int32_t storage, int32_t capacity,
                          fomo_pool** out);
@@ -206,6 +209,8 @@ int32_t fomo_pool_remove(fomo_pool* p, fomo_shot* s);
 int32_t fomo_pool_run(fomo_pool* p, int32_t n_steps);
 /* Number of members. */
 int32_t fomo_pool_size(const fomo_pool* p);
+/* Waits until the pool's queued runs are done. */
+int32_t fomo_pool_synchronize(fomo_pool* p);
 /* The member's next step (1-based): it is done when this exceeds its nt. */
 int32_t fomo_pool_next_step(const fomo_pool* p, const fomo_shot* s, int32_t* k_next);
 
