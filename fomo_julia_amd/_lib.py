@@ -45,6 +45,9 @@ SIGNATURES = {
     "fomo_model_create": (c_int32, [c_void_p, ctypes.POINTER(ModelDesc), P_f32, P_f32, P_f32,
                                     P_f32, P_f32, P_f32, P_f32, P_f32, P_f32,
                                     ctypes.POINTER(c_void_p)]),
+    "fomo_model_create_from_velocity": (c_int32, [c_void_p, ctypes.POINTER(ModelDesc), P_f32, P_f32,
+                                                   P_f32, P_f32, c_int32, c_int32,
+                                                   ctypes.POINTER(c_void_p)]),
     "fomo_model_destroy": (c_int32, [c_void_p]),
     "fomo_model_device_bytes": (c_int64, [c_void_p]),
     "fomo_wavefield_create": (c_int32, [c_void_p, ctypes.POINTER(c_void_p)]),

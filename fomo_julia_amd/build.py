@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_NAME = "libfomo_mi355x.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 SOURCES = [os.path.join(HERE, "csrc", "fomo_mi355x.hip")]
-DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("kernels.h", "fomo_device.h")] + [
+DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("kernels.h", "fomo_device.h", "setup.h")] + [
     os.path.join(os.path.dirname(HERE), "include", "fomo_mi355x.h")]
 
 

@@ -24,6 +24,7 @@
 
 #include "../../include/fomo_mi355x.h"
 #include "kernels.h"
+#include "setup.h"
 
 using namespace fomo;
 
@@ -54,6 +55,8 @@ struct fomo_model {
   float zmin = 0.f, zmax = 0.f;          // P impedance sqrt((lam+2mu) rho) extremes
   float* dramp = nullptr;
   std::vector<float> rho_vx, rho_vz;  // host copies, for force-source scaling
+  std::vector<float> rho_in;          // or: the unpadded rho of a device-side setup
+  int nxi = 0, nzi = 0;
   float a[MAX_M] = {};
   int64_t bytes = 0;
 };
@@ -459,6 +462,127 @@ extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, cons
   return FOMO_OK;
 }
 
+// Device-side setup (see setup.h): the same model as fomo_model_create on the
+// arrays init_medium builds from (vp, vs, rho), without building them on the host.
+extern "C" int32_t fomo_model_create_from_velocity(fomo_ctx* c, const fomo_model_desc* d,
+                                                   const float* a, const float* vp, const float* vs,
+                                                   const float* rho, int32_t nx_in, int32_t nz_in,
+                                                   fomo_model** out) {
+  API_LOCK;
+  if (!c || !d || !a || !out || !vp || !vs || !rho) return fail(c, FOMO_ERR_ARG, "NULL argument");
+  if (d->fd_order < 2 || d->fd_order > 2 * MAX_M || d->fd_order % 2)
+    return fail(c, FOMO_ERR_UNSUPPORTED, "fd_order must be even, 2..16");
+  if (d->nbc < 1 || d->nbc > MAX_NBC) return fail(c, FOMO_ERR_UNSUPPORTED, "nbc must be in 1..63");
+  const int M = d->fd_order / 2;
+  if (nx_in < 2 || nz_in < 2 || d->pad < 0 || d->nx != nx_in + 2 * d->pad || d->nz != nz_in + 2 * d->pad)
+    return fail(c, FOMO_ERR_ARG, "nx/nz must be nx_in/nz_in + 2*pad");
+  const int nmin = 2 * (d->nbc + 3) + 2 * M;
+  if (d->nx < nmin || d->nz < nmin) return fail(c, FOMO_ERR_ARG, "grid too small for nbc and fd_order");
+  if (d->nx > (1 << 20) || d->nz > (1 << 20) || (int64_t)d->nx * d->nz > (int64_t)1 << 31)
+    return fail(c, FOMO_ERR_ARG, "grid too large");
+  if (d->storage < FOMO_STORAGE_F32 || d->storage > FOMO_STORAGE_F16C)
+    return fail(c, FOMO_ERR_ARG, "unknown storage type");
+  if (d->storage != FOMO_STORAGE_F32 && !reduced_m_ok(M))
+    return fail(c, FOMO_ERR_UNSUPPORTED, "reduced-precision storage supports fd_order 4, 8 and 12");
+  auto m = std::make_unique<fomo_model>();
+  m->ctx = c;
+  m->d = *d;
+  m->M = M;
+  m->storage = d->storage;
+  m->ckind = coef_kind(d->storage);
+  m->fkind = field_kind(d->storage);
+  m->es = m->ckind == ST_F32 ? 4 : 2;
+  m->fes = m->fkind == ST_F32 ? 4 : 2;
+  for (int l = 0; l < M; ++l) m->a[l] = a[l];
+  const int nx = d->nx, nz = d->nz, rl = d->nbc + 2;
+  std::vector<float> ramps(3 * rl);
+  ramp_closed_form(d->nbc, 0.0, &ramps[0]);
+  ramp_closed_form(d->nbc, 0.5, &ramps[rl]);
+  ramp_closed_form(d->nbc, 0.75, &ramps[2 * rl]);
+  const bool same = d->dtx == d->dtz;
+  const int P = pitch_for(nx);
+  const size_t fs = field_stride(nz, P);
+  const int nblocks = same ? 5 : 7, es = m->es;
+  m->ncoef_blocks = nblocks;
+  m->pitch = P;
+  m->fstride = fs;
+  m->nxi = nx_in;
+  m->nzi = nz_in;
+  const size_t nin = (size_t)nx_in * nz_in;
+  m->rho_in.assign(rho, rho + nin);
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMalloc(&m->dcoef, nblocks * fs * es));
+  HIP_TRY(c, hipMemset(m->dcoef, 0, nblocks * fs * es));
+  const int blk_of[7] = {0, same ? 0 : 5, 1, same ? 1 : 6, 2, 3, 4};
+  for (int k = 0; k < 7; ++k) m->coef[k] = block_origin(m->dcoef, fs, blk_of[k], P, es);
+  // inputs and the per-block maxima of the scan pass in one scratch buffer
+  const int blocks = std::max(1, std::min(4 * std::max(c->cus, 1), (int)((size_t)nx * nz / 256 + 1)));
+  float* scratch = nullptr;
+  HIP_TRY(c, hipMalloc(&scratch, (3 * nin + (size_t)blocks * 9) * sizeof(float)));
+  std::unique_ptr<float, hipError_t (*)(void*)> hold(scratch, hipFree);
+  HIP_TRY(c, hipMemcpy(scratch, vp, nin * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(scratch + nin, vs, nin * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(scratch + 2 * nin, rho, nin * sizeof(float), hipMemcpyHostToDevice));
+  SetupArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.vp = scratch;
+  A.vs = scratch + nin;
+  A.rho = scratch + 2 * nin;
+  A.nxi = nx_in;
+  A.nzi = nz_in;
+  A.pad = d->pad;
+  A.nx = nx;
+  A.nz = nz;
+  A.pitch = P;
+  A.dtx = d->dtx;
+  A.dtz = d->dtz;
+  A.same = same ? 1 : 0;
+  for (int k = 0; k < 7; ++k) {
+    A.out[k] = const_cast<float*>(m->coef[k]);
+    A.sc[k] = 1.f;
+  }
+  A.partial = scratch + 3 * nin;
+  // pass 0: coefficient maxima (FP16 scales) and the impedance range
+  auto launch = [&](int pass) {
+    if (m->ckind == ST_BF16)
+      hipLaunchKernelGGL(k_setup<ST_BF16>, dim3(blocks), dim3(256), 0, c->stream, A, pass);
+    else if (m->ckind == ST_F16)
+      hipLaunchKernelGGL(k_setup<ST_F16>, dim3(blocks), dim3(256), 0, c->stream, A, pass);
+    else
+      hipLaunchKernelGGL(k_setup<ST_F32>, dim3(blocks), dim3(256), 0, c->stream, A, pass);
+  };
+  launch(0);
+  std::vector<float> part((size_t)blocks * 9);
+  HIP_TRY(c, hipMemcpyAsync(part.data(), A.partial, part.size() * sizeof(float), hipMemcpyDeviceToHost,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  float mx[7] = {0, 0, 0, 0, 0, 0, 0}, zmin = INFINITY, zmax = 0.f;
+  for (int q = 0; q < blocks; ++q) {
+    for (int k = 0; k < 7; ++k) mx[k] = std::max(mx[k], part[(size_t)q * 9 + k]);
+    zmin = std::min(zmin, part[(size_t)q * 9 + 7]);
+    zmax = std::max(zmax, part[(size_t)q * 9 + 8]);
+  }
+  m->zmin = std::isfinite(zmin) ? zmin : 1.f;
+  m->zmax = zmax > 0 ? zmax : 1.f;
+  for (int k = 0; k < 7; ++k) {
+    const float sc = m->ckind == ST_F16 ? pow2_scale((double)mx[k], 14) : 1.f;
+    A.sc[k] = sc;
+    m->csc[k] = 1.0f / sc;
+  }
+  if (same) {
+    m->csc[1] = m->csc[0];
+    m->csc[3] = m->csc[2];
+  }
+  launch(1);
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMalloc(&m->dramp, ramps.size() * sizeof(float)));
+  HIP_TRY(c, hipMemcpy(m->dramp, ramps.data(), ramps.size() * sizeof(float), hipMemcpyHostToDevice));
+  m->bytes = (int64_t)nblocks * fs * es + (int64_t)ramps.size() * sizeof(float);
+  *out = m.release();
+  return FOMO_OK;
+}
+
 extern "C" int32_t fomo_model_destroy(fomo_model* m) {
   API_LOCK;
   if (!m) return FOMO_OK;
@@ -606,6 +730,21 @@ static int adopt_scales(fomo_wavefield* w, const fomo_shot* s) {
   return FOMO_OK;
 }
 
+// rho_vx / rho_vz of padded cell (i, j) (0-based): from the host copies, or
+// for a device-side setup from the unpadded rho with the setup's formulas.
+static float model_rho(const fomo_model* m, bool vz, int i, int j) {
+  const int nx = m->d.nx;
+  if (!m->rho_vx.empty()) return (vz ? m->rho_vz : m->rho_vx)[(size_t)i + (size_t)j * nx];
+  auto r = [&](int ii, int jj) {
+    const int ci = std::min(std::max(ii - m->d.pad, 0), m->nxi - 1);
+    const int cj = std::min(std::max(jj - m->d.pad, 0), m->nzi - 1);
+    return m->rho_in[(size_t)ci + (size_t)cj * m->nxi];
+  };
+  if (!vz) return r(i, j);
+  const int i2 = std::min(i, nx - 2), j2 = std::min(j, m->d.nz - 2);
+  return 0.25f * (((r(i2, j2) + r(i2 + 1, j2)) + r(i2, j2 + 1)) + r(i2 + 1, j2 + 1));
+}
+
 // ------------------------------------------------------------------ shot
 extern "C" int32_t fomo_shot_create(fomo_model* m, int32_t src_i, int32_t src_j, int32_t src_type,
                                     const float* wavelet, int32_t nw, int32_t nt,
@@ -643,9 +782,8 @@ extern "C" int32_t fomo_shot_create(fomo_model* m, int32_t src_i, int32_t src_j,
   s->nt = nt;
   s->n_rec = n_rec;
   s->id = __atomic_fetch_add(&g_next_shot_id, 1, __ATOMIC_RELAXED);
-  const size_t o = (size_t)(src_i - 1) + (size_t)(src_j - 1) * nx;
-  if (src_type == FOMO_SRC_FORCE_Z) s->src_scale = m->d.dt / m->rho_vz[o];
-  if (src_type == FOMO_SRC_FORCE_X) s->src_scale = m->d.dt / m->rho_vx[o];
+  if (src_type == FOMO_SRC_FORCE_Z) s->src_scale = m->d.dt / model_rho(m, true, src_i - 1, src_j - 1);
+  if (src_type == FOMO_SRC_FORCE_X) s->src_scale = m->d.dt / model_rho(m, false, src_i - 1, src_j - 1);
   if (m->fkind == ST_F16) {
     // Bound of the injected total: sum |w| (into stress) or sum |w| dt / rho
     // (into velocity); the other fields by the plane-wave relation

@@ -277,6 +277,31 @@ def init_medium(vp, vs, rho, dx, dz, nbc: int, fd_order: int, b=None,
                   _host_f32(mu_txz), storage)
 
 
+def init_medium_device(vp, vs, rho, dx, dz, nbc: int, fd_order: int, b=None,
+                       free_surface: bool = True, storage: str = "f32") -> Medium:
+    """``init_medium`` with the padding and staggered averages left to the
+    device (``fomo_model_create_from_velocity``): the returned Medium has the
+    grid scalars but no host coefficient arrays, and carries the unpadded
+    (nx, nz) vp, vs, rho; the device model built from it is the one
+    ``init_medium``'s arrays give."""
+    if fd_order < 2 or fd_order % 2:
+        raise ValueError(f"Unsupported FD order: {fd_order}")
+    if storage not in _lib.STORAGE:
+        raise ValueError(f"storage must be one of {sorted(_lib.STORAGE)}")
+    M = fd_order // 2
+    pad = nbc + M
+    vp, vs, rho = (_host_f32(x) for x in (vp, vs, rho))
+    if not (vp.shape == vs.shape == rho.shape):
+        raise ValueError("vp, vs, rho must have one shape")
+    nx_in, nz_in = vp.shape
+    x_max = F32((nx_in - 1) * _jl(dx))
+    z_max = F32((nz_in - 1) * _jl(dz))
+    med = Medium(nx_in + 2 * pad, nz_in + 2 * pad, F32(dx), F32(dz), x_max, z_max, M, pad,
+                 bool(free_surface), None, None, None, None, None, storage)
+    med._velocity = (vp, vs, rho)
+    return med
+
+
 def init_habc(nx: int, nz: int, nbc: int, dt, dx, dz, v_ref, b=None,
               weights: bool = True) -> HABCConfig:
     """``init_habc`` (utils/init.jl:171-203).  ``weights=False`` skips building
@@ -310,6 +335,18 @@ class _DeviceModel:
                            float(M.dx), float(M.dz), float(p.dt), float(p.dtx), float(p.dtz),
                            float(H.qx), float(H.qz), float(H.qt_x), float(H.qt_z), float(H.qxt),
                            _lib.STORAGE[getattr(M, "storage", "f32")])
+        h = ctypes.c_void_p()
+        vel = getattr(M, "_velocity", None)
+        if M.rho_vx is None and vel is not None:
+            if H.w_vx is not None:
+                raise ValueError("a device-side medium uses the init_habc ramps (weights=False)")
+            vp, vs, rho = vel
+            rc = lib.fomo_model_create_from_velocity(b.ctx, ctypes.byref(d), _f32ptr(self.a),
+                                                     _f32ptr(vp), _f32ptr(vs), _f32ptr(rho),
+                                                     vp.shape[0], vp.shape[1], ctypes.byref(h))
+            b.check(rc, "fomo_model_create_from_velocity")
+            self.h = h
+            return
         med = [_host_f32(x) for x in (M.rho_vx, M.rho_vz, M.lam, M.mu_txx, M.mu_txz)]
         for x in med:
             if x.shape != (M.nx, M.nz):
@@ -319,7 +356,6 @@ class _DeviceModel:
             wp = [_f32ptr(w) for w in ws]
         else:
             wp = [None, None, None]
-        h = ctypes.c_void_p()
         rc = lib.fomo_model_create(b.ctx, ctypes.byref(d), _f32ptr(self.a), *[_f32ptr(x) for x in med],
                                    *wp, ctypes.byref(h))
         b.check(rc, "fomo_model_create")

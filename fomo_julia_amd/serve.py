@@ -143,8 +143,9 @@ def prepare(req: ShotRequest) -> Prepared:
     vp, vs, rho = media.layered_with_lenses(req.nx, req.nz, float(dx), float(dx), seed=req.seed,
                                             water_depth=5 * float(dx))
     dt = media.cfl_dt(vp, dx, req.cfl)
-    M = E.init_medium(vp, vs, rho, dx, dx, req.nbc, req.fd_order, None, free_surface=True,
-                      storage=req.storage)
+    # padding, staggered averages and coefficients are computed on the GPU at admission
+    M = E.init_medium_device(vp, vs, rho, dx, dx, req.nbc, req.fd_order, None, free_surface=True,
+                             storage=req.storage)
     v_ref = F32(np.mean(vp, dtype=np.float64))  # served requests: numpy's mean
     H = E.init_habc(M.nx, M.nz, req.nbc, dt, dx, dx, v_ref, None, weights=False)
     p = E.SimParams(dt, req.nt, dx, dx, req.fd_order)

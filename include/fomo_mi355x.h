@@ -120,6 +120,17 @@ int32_t fomo_model_create(fomo_ctx* ctx, const fomo_model_desc* desc, const floa
                           const float* rho_vx, const float* rho_vz, const float* lam,
                           const float* mu_txx, const float* mu_txz, const float* w_vx,
                           const float* w_vz, const float* w_tau, fomo_model** out);
+/* The same model built on the device from the unpadded (nx_in, nz_in) vp, vs,
+ * rho arrays of init_medium's arguments (init.jl:68-99 with _pad_array and
+ * _compute_staggered_params, init.jl:119-158): padding, staggered averages
+ * and coefficients are computed by a kernel, so only the three unpadded
+ * arrays cross the bus.  desc->nx/nz must be nx_in/nz_in + 2*pad.  HABC
+ * weights are the init_habc ramps of desc->nbc.  Identical results to
+ * fomo_model_create on init_medium's arrays. */
+int32_t fomo_model_create_from_velocity(fomo_ctx* ctx, const fomo_model_desc* desc,
+                                        const float* fd_coeffs, const float* vp, const float* vs,
+                                        const float* rho, int32_t nx_in, int32_t nz_in,
+                                        fomo_model** out);
 int32_t fomo_model_destroy(fomo_model* m);
 /* Bytes of device memory the model holds. */
 int64_t fomo_model_device_bytes(const fomo_model* m);
