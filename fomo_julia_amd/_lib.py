@@ -64,6 +64,14 @@ SIGNATURES = {
     "fomo_shot_gather": (c_int32, [c_void_p, P_f32]),
     "fomo_time_step": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32]),
     "fomo_run_time_loop": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32]),
+    "fomo_frames_create": (c_int32, [c_void_p, c_int32, c_int32, ctypes.POINTER(c_int32), c_int32,
+                                     c_int32, c_int32, c_int32, ctypes.POINTER(c_void_p)]),
+    "fomo_frames_destroy": (c_int32, [c_void_p]),
+    "fomo_frames_shape": (c_int32, [c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32)]),
+    "fomo_frames_device_bytes": (ctypes.c_int64, [c_void_p]),
+    "fomo_frames_download": (c_int32, [c_void_p, c_int32, c_int32, P_f32]),
+    "fomo_run_time_loop_frames": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_void_p]),
+    "fomo_wavefield_frame": (c_int32, [c_void_p, c_void_p, c_int32]),
     "fomo_backup_boundary": (c_int32, [c_void_p, c_void_p]),
     "fomo_inject_source": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32]),
     "fomo_update_velocity": (c_int32, [c_void_p, c_void_p]),

@@ -122,7 +122,20 @@ struct fomo_batch {
   hipGraph_t graph[2] = {nullptr, nullptr};
   int grid_fixed[PART_COUNT] = {};  // > 0: fixed grid (pools: persistent over any membership)
   hipStream_t ls = nullptr;         // stream of all its launches (null: the context's)
+  // frame store written after every step (one-shot batches); 0: none.  The
+  // captured chunks hold the store's arguments, so they are keyed by its id.
+  uint64_t frames_id = 0;
+  FrameArgs fargs{};
 };
+
+struct fomo_frames {
+  fomo_ctx* ctx = nullptr;
+  FrameArgs a{};          // out, fields, sizes; slot -1
+  int nx = 0, nz = 0;     // the grid size the frames were laid out for
+  int64_t bytes = 0;
+  uint64_t id = 0;
+};
+static uint64_t g_next_frames_id = 1;
 
 static hipStream_t lstream(const fomo_batch* b) { return b->ls ? b->ls : b->ctx->stream; }
 
@@ -1286,6 +1299,20 @@ static void launch_fsrec(const fomo_batch* b, int table, int mode, int k_off) {
     hipLaunchKernelGGL((k_fsrec<WPB, ST_F32>), g, blk, 0, lstream(b), a, mode);
 }
 
+static void launch_frame(const fomo_batch* b, const FrameArgs& fa0, int table, int k_off, int slot) {
+  FrameArgs fa = fa0;
+  fa.slot = slot;
+  const size_t total = (size_t)fa.nx_d * fa.nz_d * fa.nfields;
+  const dim3 g((unsigned)std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 2048)));
+  const BatchArgs a = args_for(b, table, PART_FSREC, k_off);
+  if (b->storage == FOMO_STORAGE_BF16)
+    hipLaunchKernelGGL(k_frame<ST_BF16>, g, dim3(256), 0, lstream(b), a, fa);
+  else if (b->storage == FOMO_STORAGE_F16)
+    hipLaunchKernelGGL(k_frame<ST_F16>, g, dim3(256), 0, lstream(b), a, fa);
+  else
+    hipLaunchKernelGGL(k_frame<ST_F32>, g, dim3(256), 0, lstream(b), a, fa);
+}
+
 // One step (time_stepper.jl:39-63) of every shot, starting from state buffer
 // `parity`.  Fused: velocity+stress in one pass (backup = the read buffer),
 // velocity HABC, band stress, stress HABC, free surface + receivers.
@@ -1298,6 +1325,7 @@ static void launch_step(const fomo_batch* b, int parity, int k_off) {
     launch_stencil(b, t, 3, FLAG_INJECT, k_off);
     launch_habc(b, t, true, k_off);
     launch_fsrec(b, t, MODE_FS | MODE_RECORD, k_off);
+    if (b->frames_id) launch_frame(b, b->fargs, t, k_off, -1);
   } else {
     const int t = TBL_IN0 + parity;
     launch_stencil(b, t, 0, FLAG_BACKUP | FLAG_INJECT, k_off);
@@ -1305,6 +1333,7 @@ static void launch_step(const fomo_batch* b, int parity, int k_off) {
     launch_stencil(b, t, 1, FLAG_BACKUP | FLAG_INJECT, k_off);
     launch_habc(b, t, true, k_off);
     launch_fsrec(b, t, MODE_FS | MODE_RECORD, k_off);
+    if (b->frames_id) launch_frame(b, b->fargs, t, k_off, -1);
   }
 }
 
@@ -1401,13 +1430,131 @@ static int run_batch(fomo_batch* b, int k_first, int n_steps) {
   return check_launch(c, "time loop");
 }
 
+// Attaches frame store f (null: none) to a one-shot batch; its captured
+// chunks are dropped when the attachment changes.
+static void set_frames(fomo_batch* b, const fomo_frames* f) {
+  const uint64_t id = f ? f->id : 0;
+  if (b->frames_id == id) return;
+  for (int q = 0; q < 2; ++q) {
+    if (b->gexec[q]) (void)hipGraphExecDestroy(b->gexec[q]);
+    if (b->graph[q]) (void)hipGraphDestroy(b->graph[q]);
+    b->gexec[q] = nullptr;
+    b->graph[q] = nullptr;
+  }
+  b->frames_id = id;
+  b->fargs = f ? f->a : FrameArgs{};
+}
+
 extern "C" int32_t fomo_run_time_loop(fomo_model* m, fomo_wavefield* w, fomo_shot* s,
                                       int32_t k_first, int32_t n_steps) {
   if (!m || !w || !s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   fomo_batch* b = nullptr;
   int rc = single_batch(m, w, s, &b);
   if (rc) return rc;
+  set_frames(b, nullptr);
   return run_batch(b, k_first, n_steps);
+}
+
+// ------------------------------------------------------------------ frames
+extern "C" int32_t fomo_frames_create(fomo_ctx* c, int32_t nx, int32_t nz, const int32_t* fields,
+                                      int32_t n_fields, int32_t skip, int32_t downsample,
+                                      int32_t capacity, fomo_frames** out) {
+  API_LOCK;
+  if (!c || !fields || !out) return fail(c, FOMO_ERR_ARG, "NULL argument");
+  if (n_fields < 1 || n_fields > 4) return fail(c, FOMO_ERR_ARG, "n_fields must be 1..4");
+  if (nx < 1 || nz < 1 || skip < 1 || downsample < 1 || capacity < 1)
+    return fail(c, FOMO_ERR_ARG, "nx, nz, skip, downsample and capacity must be >= 1");
+  auto f = std::make_unique<fomo_frames>();
+  f->ctx = c;
+  f->nx = nx;
+  f->nz = nz;
+  FrameArgs& a = f->a;
+  for (int q = 0; q < n_fields; ++q) {
+    if (fields[q] < FOMO_FRAME_P || fields[q] > FOMO_FRAME_VZ)
+      return fail(c, FOMO_ERR_ARG, "unknown frame field (p, vel, vx, vz)");
+    a.field[q] = fields[q];
+  }
+  a.nfields = n_fields;
+  a.skip = skip;
+  a.ds = downsample;
+  a.nx_d = (nx + downsample - 1) / downsample;  // cld(nx, downsample)
+  a.nz_d = (nz + downsample - 1) / downsample;
+  a.capacity = capacity;
+  a.slot = -1;
+  const size_t bytes = (size_t)capacity * n_fields * a.nx_d * a.nz_d * sizeof(float);
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMalloc(&a.out, bytes));
+  HIP_TRY(c, hipMemset(a.out, 0, bytes));
+  f->bytes = (int64_t)bytes;
+  f->id = g_next_frames_id++;
+  *out = f.release();
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_frames_destroy(fomo_frames* f) {
+  API_LOCK;
+  if (!f) return FOMO_OK;
+  (void)hipSetDevice(f->ctx->device);
+  (void)hipDeviceSynchronize();  // batches of any stream may still write it
+  (void)hipFree(f->a.out);
+  delete f;
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_frames_shape(const fomo_frames* f, int32_t* nx_d, int32_t* nz_d) {
+  if (!f || !nx_d || !nz_d) return FOMO_ERR_ARG;
+  *nx_d = f->a.nx_d;
+  *nz_d = f->a.nz_d;
+  return FOMO_OK;
+}
+
+extern "C" int64_t fomo_frames_device_bytes(const fomo_frames* f) { return f ? f->bytes : 0; }
+
+extern "C" int32_t fomo_frames_download(fomo_frames* f, int32_t first, int32_t count, float* host) {
+  if (!f || !host) return fail(f ? f->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  if (first < 0 || count < 0 || first + count > f->a.capacity)
+    return fail(f->ctx, FOMO_ERR_ARG, "frame range outside the store");
+  fomo_ctx* c = f->ctx;
+  const size_t per = (size_t)f->a.nfields * f->a.nx_d * f->a.nz_d;
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMemcpyAsync(host, f->a.out + (size_t)first * per, (size_t)count * per * sizeof(float),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FOMO_OK;
+}
+
+static int check_frames_ctx(fomo_ctx* c, const fomo_frames* f) {
+  if (f->ctx != c) return fail(c, FOMO_ERR_STATE, "frame store of another context");
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_run_time_loop_frames(fomo_model* m, fomo_wavefield* w, fomo_shot* s,
+                                             int32_t k_first, int32_t n_steps, fomo_frames* f) {
+  if (!m || !w || !s || !f) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  int rc = check_frames_ctx(m->ctx, f);
+  if (rc) return rc;
+  fomo_batch* b = nullptr;
+  rc = single_batch(m, w, s, &b);
+  if (rc) return rc;
+  set_frames(b, f);
+  return run_batch(b, k_first, n_steps);
+}
+
+extern "C" int32_t fomo_wavefield_frame(fomo_wavefield* w, fomo_frames* f, int32_t slot) {
+  if (!w || !f) return fail(w ? w->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_model* m = w->m;
+  int rc = check_frames_ctx(m->ctx, f);
+  if (rc) return rc;
+  if (slot < 0 || slot >= f->a.capacity) return fail(m->ctx, FOMO_ERR_ARG, "slot outside the store");
+  fomo_batch* b = nullptr;
+  rc = single_batch(m, w, nullptr, &b);
+  if (rc) return rc;
+  FrameArgs a = f->a;
+  a.own_sc = 1;  // the placeholder shot's row does not carry the state's scales
+  for (int c = 0; c < 5; ++c) a.isc[c] = 1.0f / w->fsc[c];
+  HIP_TRY(m->ctx, hipSetDevice(m->ctx->device));
+  launch_frame(b, a, TBL_IN0 + w->cur, 0, slot);
+  return check_launch(m->ctx, "wavefield_frame");
 }
 
 extern "C" int32_t fomo_time_step(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t k) {
@@ -1421,6 +1568,7 @@ extern "C" int32_t fomo_time_step(fomo_model* m, fomo_wavefield* w, fomo_shot* s
   }
   rc = adopt_scales(w, s);
   if (rc) return rc;
+  set_frames(b, nullptr);
   HIP_TRY(m->ctx, hipSetDevice(m->ctx->device));
   launch_set_step(b, k);
   launch_step(b, w->cur, 0);

@@ -1573,6 +1573,60 @@ __global__ __launch_bounds__(64 * WPB) void k_fsrec(BatchArgs ba, int mode) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wavefield frames (visualization/video_recorder.jl:207-283): on every step k
+// with k % skip == 0 the decimated fields go to frame slot
+// (k/skip - 1) mod capacity of a device frame store (slot >= 0 in the
+// arguments: that slot, whatever the step).  frame[j_d][i_d] = field(min(i_d*ds, nx-1), min(j_d*ds, nz-1)),
+// 0-based (_extract_frame!, :260-268); p = (txx + tzz) * 0.5 and
+// vel = sqrt(vx*vx + vz*vz) (_get_field_data, :270-283).  One shot (row 0).
+// ---------------------------------------------------------------------------
+struct FrameArgs {
+  float* out;     // [capacity][nfields][nz_d][nx_d]
+  int field[4];   // FRAME_* per stored field
+  int nfields, skip, ds, nx_d, nz_d, capacity, slot;
+  int own_sc;     // 1: decode with isc (the state's scales) instead of the shot row's
+  float isc[5];
+};
+enum FrameField { FRAME_P = 0, FRAME_VEL = 1, FRAME_VX = 2, FRAME_VZ = 3 };
+
+template <int K>
+__global__ __launch_bounds__(256) void k_frame(BatchArgs ba, FrameArgs fa) {
+  constexpr int KF = Mode<K>::F;
+  typedef typename Store<KF>::T T;
+  const ShotDev& sd = ba.shots[0];
+  int slot = fa.slot;
+  if (slot < 0) {
+    const int k = shot_step(sd, ba.k_off);
+    if (k == 0 || k % fa.skip != 0) return;
+    slot = (k / fa.skip - 1) % fa.capacity;  // a ring of frames
+  }
+  const size_t per = (size_t)fa.nx_d * fa.nz_d;
+  float* base = fa.out + (size_t)slot * fa.nfields * per;
+  const int P = sd.pitch;
+  for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < per * fa.nfields;
+       o += (size_t)gridDim.x * blockDim.x) {
+    const int q = (int)(o / per);
+    const size_t r = o - (size_t)q * per;
+    const int id = (int)(r % fa.nx_d), jd = (int)(r / fa.nx_d);
+    const int i = min(id * fa.ds, sd.nx - 1), j = min(jd * fa.ds, sd.nz - 1);
+    auto fld = [&](int c) {
+      return ld1<KF>(rowt((const T*)sd.f[c], j, P) + i, fa.own_sc ? fa.isc[c] : sd.fisc[c]);
+    };
+    const int which = fa.field[q];
+    float v;
+    if (which == FRAME_P) {
+      v = (fld(F_TXX) + fld(F_TZZ)) * 0.5f;
+    } else if (which == FRAME_VEL) {
+      const float x = fld(F_VX), z = fld(F_VZ);
+      v = __builtin_sqrtf(x * x + z * z);
+    } else {
+      v = fld(which == FRAME_VX ? F_VX : F_VZ);
+    }
+    base[o] = v;
+  }
+}
+
 // Standalone strip backup (backup_boundary!, boundary.jl:45-56) for the per-stage API.
 template <int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_backup(BatchArgs ba) {

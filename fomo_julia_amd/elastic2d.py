@@ -585,13 +585,18 @@ def run_time_loop(b, W, M, H, a, src: Source, rec: Receivers, p: SimParams,
     Without a callback the nt steps run as replayed HIP graphs.  With
     ``on_step(W, TimeStepInfo)`` the loop breaks into segments of
     ``on_step_every`` steps (default 1, the reference's per-step callback);
-    returning False stops the simulation early.  ``rec.data`` receives the
-    [nt x n_rec] gather."""
+    returning False stops the simulation early.  A ``video.MultiFieldRecorder``
+    as ``on_step`` keeps the graph replay: its frames are taken on the device
+    after every ``skip``-th step.  ``rec.data`` receives the [nt x n_rec]
+    gather."""
+    from .video import MultiFieldRecorder
     dm = bind(b, W, M, H, a, p)
     ds = _shot_for(dm, src, rec, p.nt)
     lib = b.lib
     nt = p.nt
-    if on_step is None:
+    if isinstance(on_step, MultiFieldRecorder):
+        on_step.run_loop(b, dm, W, ds, nt, p.dt)
+    elif on_step is None:
         b.check(lib.fomo_run_time_loop(dm.h, W._h, ds.h, 1, nt), "run_time_loop")
     else:
         every = max(1, int(on_step_every))

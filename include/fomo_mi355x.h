@@ -175,6 +175,43 @@ int32_t fomo_time_step(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t k
 int32_t fomo_run_time_loop(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t k_first,
                            int32_t n_steps);
 
+/* ---- wavefield frames: the on_step video callback without leaving the graphs
+ * MultiFieldRecorder (video_recorder.jl:113-148,207-228) called as on_step
+ * (time_stepper.jl:112-119) takes, on every step k with k % skip == 0, the
+ * frame of each field (_extract_frame!, video_recorder.jl:260-268):
+ *   frame[i_d, j_d] = raw(min(i_d*downsample, nx-1), min(j_d*downsample, nz-1))
+ * (0-based; raw over the whole padded grid) for i_d < cld(nx_frame, downsample),
+ * j_d < cld(nz_frame, downsample), with raw = (txx + tzz) * 0.5 (P),
+ * sqrt(vx^2 + vz^2) (VEL), vx, vz (_get_field_data, :270-283).  A frame store
+ * keeps `capacity` such frames in device memory, [slot][field][j_d][i_d]
+ * (x fastest), so a run records them inside its captured graphs with no host
+ * round trip per frame. */
+#define FOMO_FRAME_P 0
+#define FOMO_FRAME_VEL 1
+#define FOMO_FRAME_VX 2
+#define FOMO_FRAME_VZ 3
+typedef struct fomo_frames fomo_frames;
+/* VideoConfig(fields, skip, downsample) (video_recorder.jl:68-81) + MultiFieldRecorder(nx, nz, ...)
+ * (:113-127, 164-200), without the plotting. */
+int32_t fomo_frames_create(fomo_ctx* ctx, int32_t nx_frame, int32_t nz_frame, const int32_t* fields,
+                           int32_t n_fields, int32_t skip, int32_t downsample, int32_t capacity,
+                           fomo_frames** out);
+int32_t fomo_frames_destroy(fomo_frames* f);
+/* Frame size per field: cld(nx_frame, downsample), cld(nz_frame, downsample). */
+int32_t fomo_frames_shape(const fomo_frames* f, int32_t* nx_d, int32_t* nz_d);
+int64_t fomo_frames_device_bytes(const fomo_frames* f);
+/* Slots first .. first+count-1 into host ([count][n_fields][nz_d][nx_d]); synchronous. */
+int32_t fomo_frames_download(fomo_frames* f, int32_t first, int32_t count, float* host);
+/* fomo_run_time_loop that, after every step k with k % skip == 0, writes the
+ * frames of the state after step k into slot (k/skip - 1) mod capacity (a
+ * ring: download before a later frame reuses the slot):
+ * run_time_loop!(...; on_step = recorder) (time_stepper.jl:112-119). */
+int32_t fomo_run_time_loop_frames(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t k_first,
+                                  int32_t n_steps, fomo_frames* f);
+/* The frames of the wavefield's current state into `slot` (_SingleRecorder
+ * called on W, video_recorder.jl:207-228). */
+int32_t fomo_wavefield_frame(fomo_wavefield* w, fomo_frames* f, int32_t slot);
+
 /* Per-stage entry points, each one kernel of the step (Elastic2D.jl:55-58). */
 int32_t fomo_backup_boundary(fomo_model* m, fomo_wavefield* w);             /* boundary.jl:45-56 */
 int32_t fomo_inject_source(fomo_model* m, fomo_wavefield* w, fomo_shot* s,
