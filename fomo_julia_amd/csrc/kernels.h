@@ -1219,9 +1219,14 @@ __device__ __forceinline__ HabcField habc_field(const ShotDev& sd, int stress, i
 constexpr int HS = 65;  // LDS row stride: odd, so row and column walks are bank-conflict free
 
 // rows [r0, r0+nr) x cols [c0, c0+nc) of the field g and its backup go -> s, so
-// (s[r*HS + c], so[r*HS + c]), nc <= 64.  Loads of both arrays go out 32 rows
-// at a time before any LDS store: a chunk costs one memory latency.
-constexpr int STAGE_CHUNK = 32;
+// (s[r*HS + c], so[r*HS + c]), nc <= 64.  Loads of both arrays go out 64 rows
+// at a time before any LDS store: a piece (at most 64 rows) costs one memory
+// latency.
+constexpr int STAGE_CHUNK = 64;
+// Recurrence loops read their operands (pre-HABC values, which no earlier
+// step of the loop writes) this many steps ahead, so LDS latency stays off the
+// chain of dependent steps.
+constexpr int HPF = 4;
 template <int K>
 __device__ __forceinline__ void lds_in2(float* s, float* so, const typename Store<K>::T* g,
                                         const typename Store<K>::T* go, float rs, int P, int r0,
@@ -1323,17 +1328,38 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
         float* fr = s_f + lane * HS;
         const float* orow = s_o + lane * HS;
         float prev = fr[0], op = orow[0];
-        float fv = fr[1], ov = orow[1], w = W(c0 + 1, j0 + lane);
-        for (int c = 1; c <= n; ++c) {
-          const int cn = min(c + 1, n);  // next step's operands (pre-values)
-          const float nfv = fr[cn], nov = orow[cn], nw = W(c0 + cn, j0 + lane);
-          const float out = q.edge(w, fv, q.sumx(prev, ov, op));
-          fr[c] = out;
-          prev = out;
-          op = ov;
-          fv = nfv;
-          ov = nov;
-          w = nw;
+        float pf[HPF], po[HPF], pw[HPF];
+#pragma unroll
+        for (int k = 0; k < HPF; ++k) {
+          const int c = min(1 + k, n);
+          pf[k] = fr[c];
+          po[k] = orow[c];
+          pw[k] = W(c0 + c, j0 + lane);
+        }
+        for (int cg = 1; cg <= n; cg += HPF) {
+          float nf[HPF], no[HPF], nw[HPF];
+#pragma unroll
+          for (int k = 0; k < HPF; ++k) {  // the next group's pre-values
+            const int c = min(cg + HPF + k, n);
+            nf[k] = fr[c];
+            no[k] = orow[c];
+            nw[k] = W(c0 + c, j0 + lane);
+          }
+#pragma unroll
+          for (int k = 0; k < HPF; ++k) {
+            if (cg + k <= n) {
+              const float out = q.edge(pw[k], pf[k], q.sumx(prev, po[k], op));
+              fr[cg + k] = out;
+              prev = out;
+              op = po[k];
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < HPF; ++k) {
+            pf[k] = nf[k];
+            po[k] = no[k];
+            pw[k] = nw[k];
+          }
         }
       }
       __syncthreads();
@@ -1354,17 +1380,38 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
         if (!top) {
           // bottom edge: recurrence along +z from row nz-n-2
           float prev = s_f[lane], op = s_o[lane];
-          float fv = s_f[HS + lane], ov = s_o[HS + lane], w = W(i, r0 + 1);
-          for (int r = 1; r <= n; ++r) {
-            const int rn = min(r + 1, n);  // next step's operands (pre-values)
-            const float nfv = s_f[rn * HS + lane], nov = s_o[rn * HS + lane], nw = W(i, r0 + rn);
-            const float out = q.edge(w, fv, q.sumz(prev, ov, op));
-            s_f[r * HS + lane] = out;
-            prev = out;
-            op = ov;
-            fv = nfv;
-            ov = nov;
-            w = nw;
+          float pf[HPF], po[HPF], pw[HPF];
+#pragma unroll
+          for (int k = 0; k < HPF; ++k) {
+            const int r = min(1 + k, n);
+            pf[k] = s_f[r * HS + lane];
+            po[k] = s_o[r * HS + lane];
+            pw[k] = W(i, r0 + r);
+          }
+          for (int rg = 1; rg <= n; rg += HPF) {
+            float nf[HPF], no[HPF], nw[HPF];
+#pragma unroll
+            for (int k = 0; k < HPF; ++k) {  // the next group's pre-values
+              const int r = min(rg + HPF + k, n);
+              nf[k] = s_f[r * HS + lane];
+              no[k] = s_o[r * HS + lane];
+              nw[k] = W(i, r0 + r);
+            }
+#pragma unroll
+            for (int k = 0; k < HPF; ++k) {
+              if (rg + k <= n) {
+                const float out = q.edge(pw[k], pf[k], q.sumz(prev, po[k], op));
+                s_f[(rg + k) * HS + lane] = out;
+                prev = out;
+                op = po[k];
+              }
+            }
+#pragma unroll
+            for (int k = 0; k < HPF; ++k) {
+              pf[k] = nf[k];
+              po[k] = no[k];
+              pw[k] = nw[k];
+            }
           }
         } else {
           // top edge (no free surface), rows 1..n: Jacobi in z, rows ascending
@@ -1399,18 +1446,45 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       }
       __syncthreads();
       float up = act ? s_f[lane] : 0.f, ou = act ? s_o[lane] : 0.f;
-      for (int r = 1; r <= n; ++r) {
-        const float fv = act ? s_f[r * HS + lane] : 0.f;
-        const float ov = act ? s_o[r * HS + lane] : 0.f;
-        const float w = act ? W(c0 + lane, r0 + r) : 0.f;
-        float x = fv;
-        if (lane == n) x = q.edge(w, fv, q.sumz(up, ov, ou));
-        const float fr = from_upper_lane(x);
-        const float orr = from_upper_lane(ov);
-        const float out = (lane == n) ? x : q.corner(w, fv, q.sumx(fr, ov, orr), q.sumz(up, ov, ou));
-        if (act) s_f[r * HS + lane] = out;
-        up = out;
-        ou = ov;
+      const int la = act ? lane : 0;  // idle lanes read a valid cell and never store
+      float pf[HPF], po[HPF], pw[HPF];
+#pragma unroll
+      for (int k = 0; k < HPF; ++k) {
+        const int r = min(1 + k, n);
+        pf[k] = s_f[r * HS + la];
+        po[k] = s_o[r * HS + la];
+        pw[k] = W(c0 + la, r0 + r);
+      }
+      for (int rg = 1; rg <= n; rg += HPF) {
+        float nf[HPF], no[HPF], nw[HPF];
+#pragma unroll
+        for (int k = 0; k < HPF; ++k) {  // the next group's pre-values (rows not yet written)
+          const int r = min(rg + HPF + k, n);
+          nf[k] = s_f[r * HS + la];
+          no[k] = s_o[r * HS + la];
+          nw[k] = W(c0 + la, r0 + r);
+        }
+#pragma unroll
+        for (int k = 0; k < HPF; ++k) {
+          if (rg + k <= n) {
+            const float fv = pf[k], ov = po[k], w = pw[k];
+            float x = fv;
+            if (lane == n) x = q.edge(w, fv, q.sumz(up, ov, ou));
+            const float fr = from_upper_lane(x);
+            const float orr = from_upper_lane(ov);
+            const float out =
+                (lane == n) ? x : q.corner(w, fv, q.sumx(fr, ov, orr), q.sumz(up, ov, ou));
+            if (act) s_f[(rg + k) * HS + lane] = out;
+            up = out;
+            ou = ov;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < HPF; ++k) {
+          pf[k] = nf[k];
+          po[k] = no[k];
+          pw[k] = nw[k];
+        }
       }
     } else if (bottom) {
       // BR: one wavefront over the block; block row 0 is the right-edge row
@@ -1429,30 +1503,42 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
         ou = s_o[max(rr - 1, 0) * HS + L];
         w = W(c0 + L, r0 + rr);
       };
-      float fv, ov, ol, ou, w;
-      fetch(1, fv, ov, ol, ou, w);
-      for (int d = 1; d <= 2 * n; ++d) {
-        float nfv, nov, nol, nou, nw;
-        fetch(d + 1, nfv, nov, nol, nou, nw);
-        const float left = from_lower_lane(last);
-        const int r = d - L;
-        // One formula for the three cell kinds: the bottom-edge column has no
-        // x-term, the right-edge row no z-term, and their weight factor is
-        // (1-w) instead of (1-w)/2; adding an exact 0 keeps each result
-        // identical to its own formula.
-        const float sx = (L == 0) ? 0.f : q.sumx(left, ov, ol);
-        const float sz = (r == 0) ? 0.f : q.sumz(last, ov, ou);
-        const float cw = (L == 0 || r == 0) ? (1.0f - w) : (1.0f - w) * 0.5f;
-        const float out = fmaf_(cw, sx + sz, w * fv);
-        if (act && r >= 0 && r <= n && !(L == 0 && r == 0)) {
-          s_f[r * HS + L] = out;
-          last = out;
+      float pf[HPF], po[HPF], pl[HPF], pu[HPF], pw[HPF];
+#pragma unroll
+      for (int k = 0; k < HPF; ++k) fetch(1 + k, pf[k], po[k], pl[k], pu[k], pw[k]);
+      for (int dg = 1; dg <= 2 * n; dg += HPF) {
+        float nf[HPF], no[HPF], nl[HPF], nu[HPF], nw[HPF];
+#pragma unroll
+        for (int k = 0; k < HPF; ++k) fetch(dg + HPF + k, nf[k], no[k], nl[k], nu[k], nw[k]);
+#pragma unroll
+        for (int k = 0; k < HPF; ++k) {
+          const int d = dg + k;
+          if (d <= 2 * n) {
+            const float fv = pf[k], ov = po[k], ol = pl[k], ou = pu[k], w = pw[k];
+            const float left = from_lower_lane(last);
+            const int r = d - L;
+            // One formula for the three cell kinds: the bottom-edge column has
+            // no x-term, the right-edge row no z-term, and their weight factor
+            // is (1-w) instead of (1-w)/2; adding an exact 0 keeps each result
+            // identical to its own formula.
+            const float sx = (L == 0) ? 0.f : q.sumx(left, ov, ol);
+            const float sz = (r == 0) ? 0.f : q.sumz(last, ov, ou);
+            const float cw = (L == 0 || r == 0) ? (1.0f - w) : (1.0f - w) * 0.5f;
+            const float out = fmaf_(cw, sx + sz, w * fv);
+            if (act && r >= 0 && r <= n && !(L == 0 && r == 0)) {
+              s_f[r * HS + L] = out;
+              last = out;
+            }
+          }
         }
-        fv = nfv;
-        ov = nov;
-        ol = nol;
-        ou = nou;
-        w = nw;
+#pragma unroll
+        for (int k = 0; k < HPF; ++k) {
+          pf[k] = nf[k];
+          po[k] = no[k];
+          pl[k] = nl[k];
+          pu[k] = nu[k];
+          pw[k] = nw[k];
+        }
       }
     } else if (leftc) {
       // TL (no free surface): block rows 0..n = rows 1..n+1, columns 0..n =
