@@ -13,7 +13,8 @@ wavefield and gather are reset and exactly K steps are timed from step 1.
 The step is the fused single-pass form (one velocity+stress kernel over
 ping-pong buffers, then velocity HABC, band stress, stress HABC, free
 surface + receivers).  Prints one JSON line: Mcell-updates/s over the padded
-grid, the fused kernel's HBM roofline (algorithmic 60 B per cell-update over
+grid, the fused kernel's HBM roofline (algorithmic 52 B per cell-update: read
+5 fields + rho_vx, lam, mu_txx, write 5 fields; over
 its mean duration measured with HIP events on the library's stream; HBM
 traffic per launch from the committed rocprofv3 counter summary), and the
 CPU oracle timed on a bounded sample of the same workload.
@@ -36,6 +37,9 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per padded cell-update (FP32, dx == dz so dt/dx/rho and
 # dt/dz/rho are one array each):
 BYTES_FUSED = 60     # read 5 old fields + dt/rho_vx, dt/rho_vz, lam, lam+2mu, mu_txz; write 5 fields
+# FP32 media whose rho_vz / mu_txz are init_medium's averages: the fused kernel
+# reads rho_vx, lam, mu_txx and forms the other coefficients (5 + 3 reads, 5 writes)
+BYTES_FUSED_DERIVED = 52
 # the same 15 array accesses per cell with 2-byte coefficients and/or fields
 BYTES_FUSED_STORAGE = {"f32": 60, "bf16": 30, "f16": 30, "f16c": 50}
 BYTES_VELOCITY = 36  # two-pass velocity: read txx tzz txz, 2 buoyancies; read+write vx vz
@@ -62,6 +66,14 @@ def build_seam(nt: int, nbc=50, fd_order=8, with_weights=False, storage="f32"):
     nx_in = vp.shape[0]
     rec = E.setup_receivers(np.arange(nx_in, dtype=F32) * dx, np.zeros(nx_in, F32), M, type="vz")
     return dict(M=M, H=H, p=p, a=a, src=src, rec=rec, vp=vp)
+
+
+def model_blocks(b, dm, M, H) -> int:
+    """Pitched coefficient arrays the device model holds (5 stored; 7 when it
+    also holds rho_vx and mu_txx for the derived-coefficient fused kernel)."""
+    pitch = (M.nx + 272 + 255) // 256 * 256
+    block = 4 * ((M.nz + 36) * pitch + 512)
+    return round((int(b.lib.fomo_model_device_bytes(dm.h)) - 12 * (H.nbc + 2)) / block)
 
 
 def cpu_baseline(case, steps: int, threads: int):
@@ -204,7 +216,20 @@ def main():
     ap.add_argument("--requests", type=int, default=16,
                     help="requests of the served sample (default run) or stream (serve)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served sample")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 18  # float32 elements
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
@@ -288,7 +313,10 @@ def main():
                    "habc_velocity": stage_ms(7, reps), "habc_stress": stage_ms(8, reps),
                    "fs_record": stage_ms(4, reps)}
         main_bytes = BYTES_FUSED_STORAGE[args.storage]
-        main_name = f"k_fused ({main_bytes} B/cell-update x padded cells per launch)"
+        form = "stored coefficients"
+        if args.storage == "f32" and model_blocks(b, dm, M, H) == 7:
+            main_bytes, form = BYTES_FUSED_DERIVED, "coefficients formed from rho_vx, lam, mu_txx"
+        main_name = f"k_fused, {form} ({main_bytes} B/cell-update x padded cells per launch)"
     else:
         main_ms = stage_ms(1, reps)
         kernels = {"velocity": stage_ms(0, reps), "stress": main_ms,

@@ -52,6 +52,10 @@ struct fomo_model {
   const float* coef[7] = {};  // bvx_x bvx_z bvz_x bvz_z lam l2m mu_txz (element type per storage)
   float csc[7] = {1, 1, 1, 1, 1, 1, 1};  // load scales 2^-e of the coefficient arrays
   int ncoef_blocks = 5;
+  // FP32 with dtx == dtz and rho_vz / mu_txz the setup's averages of rho_vx /
+  // mu_txx: blocks 5 and 6 hold rho_vx and mu_txx, and the fused kernel forms
+  // the other coefficients itself (12 B per cell read instead of 20)
+  bool compact = false;
   float zmin = 0.f, zmax = 0.f;          // P impedance sqrt((lam+2mu) rho) extremes
   float* dramp = nullptr;
   std::vector<float> rho_vx, rho_vz;  // host copies, for force-source scaling
@@ -111,6 +115,9 @@ struct fomo_batch {
   int tz_req = 0;  // the context's rows_per_tile when built (0 = auto)
   int storage = FOMO_STORAGE_F32;
   bool same = true;  // dtx == dtz for every member
+  // every member's model holds rho_vx / mu_txx for the derived-coefficient
+  // fused kernel (FP32, dtx == dtz)
+  bool compact = false;
   std::vector<fomo_wavefield*> ws;
   std::vector<fomo_shot*> ss;
   std::vector<ShotDev> hs;  // [TBL_COUNT][n]
@@ -352,6 +359,25 @@ static void ramp_closed_form(int nbc, double off, float* ramp) {
   }
 }
 
+// rho_vz and mu_txz are bit for bit the staggered averages init_medium forms
+// from rho_vx and mu_txx (init.jl:146-155, last row / column repeated).
+static bool averages_match(const float* rho_vx, const float* rho_vz, const float* mu_txx,
+                           const float* mu_txz, int nx, int nz) {
+  auto same_bits = [](float a, float b) { return std::memcmp(&a, &b, sizeof(float)) == 0; };
+  auto at = [nx](const float* x, int i, int j) { return x[(size_t)i + (size_t)j * nx]; };
+  for (int j = 0; j < nz; ++j) {
+    const int j2 = std::min(j, nz - 2);
+    for (int i = 0; i < nx; ++i) {
+      const int i2 = std::min(i, nx - 2);
+      const float rvz = 0.25f * (((at(rho_vx, i2, j2) + at(rho_vx, i2 + 1, j2)) + at(rho_vx, i2, j2 + 1)) +
+                                 at(rho_vx, i2 + 1, j2 + 1));
+      const float mxz = 2.0f / (1.0f / at(mu_txx, i, j2) + 1.0f / at(mu_txx, i, j2 + 1));
+      if (!same_bits(rvz, at(rho_vz, i, j)) || !same_bits(mxz, at(mu_txz, i, j))) return false;
+    }
+  }
+  return true;
+}
+
 extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, const float* a,
                                      const float* rho_vx, const float* rho_vz, const float* lam,
                                      const float* mu_txx, const float* mu_txz, const float* w_vx,
@@ -420,7 +446,11 @@ extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, cons
   }
   const int P = pitch_for(nx);
   const size_t fs = field_stride(nz, P);
-  const int nblocks = same ? 5 : 7;  // dtx == dtz: the z-scaled arrays alias the x-scaled ones
+  // FP32, dtx == dtz and init_medium's averages (init.jl:146-155): the fused
+  // kernel can form the coefficients from rho_vx, lam and mu_txx
+  m->compact = same && m->ckind == ST_F32 && averages_match(rho_vx, rho_vz, mu_txx, mu_txz, nx, nz);
+  // dtx == dtz: the z-scaled arrays alias the x-scaled ones; compact: + rho_vx, mu_txx
+  const int nblocks = (same ? 5 : 7) + (m->compact ? 2 : 0);
   const int es = m->es;
   m->ncoef_blocks = nblocks;
   m->pitch = P;
@@ -465,6 +495,12 @@ extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, cons
     }
     HIP_TRY(c, hipMemcpy2D(dst, (size_t)P * es, src, (size_t)nx * es, (size_t)nx * es, nz,
                            hipMemcpyHostToDevice));
+  }
+  if (m->compact) {  // blocks 5, 6: rho_vx, mu_txx
+    const float* extra[2] = {rho_vx, mu_txx};
+    for (int q = 0; q < 2; ++q)
+      HIP_TRY(c, hipMemcpy2D(block_origin(m->dcoef, fs, 5 + q, P, es), (size_t)P * es, extra[q],
+                             (size_t)nx * es, (size_t)nx * es, nz, hipMemcpyHostToDevice));
   }
   HIP_TRY(c, hipMalloc(&m->dramp, ramps.size() * sizeof(float)));
   HIP_TRY(c, hipMemcpy(m->dramp, ramps.data(), ramps.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -515,7 +551,8 @@ extern "C" int32_t fomo_model_create_from_velocity(fomo_ctx* c, const fomo_model
   const bool same = d->dtx == d->dtz;
   const int P = pitch_for(nx);
   const size_t fs = field_stride(nz, P);
-  const int nblocks = same ? 5 : 7, es = m->es;
+  m->compact = same && m->ckind == ST_F32;  // + rho_vx, mu_txx blocks for the fused kernel
+  const int nblocks = (same ? 5 : 7) + (m->compact ? 2 : 0), es = m->es;
   m->ncoef_blocks = nblocks;
   m->pitch = P;
   m->fstride = fs;
@@ -555,6 +592,8 @@ extern "C" int32_t fomo_model_create_from_velocity(fomo_ctx* c, const fomo_model
     A.sc[k] = 1.f;
   }
   A.partial = scratch + 3 * nin;
+  A.out_rho = m->compact ? block_origin(m->dcoef, fs, 5, P, es) : nullptr;
+  A.out_mu = m->compact ? block_origin(m->dcoef, fs, 6, P, es) : nullptr;
   // pass 0: coefficient maxima (FP16 scales) and the impedance range
   auto launch = [&](int pass) {
     if (m->ckind == ST_BF16)
@@ -986,12 +1025,18 @@ constexpr int WPB = 4;  // wavefronts per block of the stencil kernels
 
 // Resident wavefronts of the stencil kernels for (M, cpt) on the whole device.
 template <int K, int M, int CPT>
-static int stencil_slots_mc(int cus, int mode) {
+static int stencil_slots_mc(int cus, int mode, bool compact) {
   int bv = 0, bs = 0;
   if (mode == 1 || K != ST_F32) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused<M, CPT, WPB, K, true>, 64 * WPB,
-                                                     0) != hipSuccess)
-      bv = 1;
+    hipError_t e;
+    if constexpr (K == ST_F32)
+      e = compact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused<M, CPT, WPB, K, true, true>,
+                                                                 64 * WPB, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused<M, CPT, WPB, K, true, false>,
+                                                                 64 * WPB, 0);
+    else
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused<M, CPT, WPB, K, true, false>, 64 * WPB, 0);
+    if (e != hipSuccess) bv = 1;
     bs = bv;
   } else if constexpr (K == ST_F32) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_velocity<M, CPT, WPB, true>, 64 * WPB, 0) !=
@@ -1003,10 +1048,10 @@ static int stencil_slots_mc(int cus, int mode) {
   return std::max(1, std::min(bv, bs)) * WPB * std::max(cus, 1);
 }
 template <int K, int M>
-static int stencil_slots_m(int cpt, int cus, int mode) {
-  return cpt == 1   ? stencil_slots_mc<K, M, 1>(cus, mode)
-         : cpt == 2 ? stencil_slots_mc<K, M, 2>(cus, mode)
-                    : stencil_slots_mc<K, M, 4>(cus, mode);
+static int stencil_slots_m(int cpt, int cus, int mode, bool compact = false) {
+  return cpt == 1   ? stencil_slots_mc<K, M, 1>(cus, mode, compact)
+         : cpt == 2 ? stencil_slots_mc<K, M, 2>(cus, mode, compact)
+                    : stencil_slots_mc<K, M, 4>(cus, mode, compact);
 }
 template <int K>
 static int stencil_slots_reduced(int M, int cpt, int cus) {
@@ -1016,19 +1061,19 @@ static int stencil_slots_reduced(int M, int cpt, int cus) {
     default: return stencil_slots_m<K, 6>(cpt, cus, 1);
   }
 }
-static int stencil_slots(int storage, int M, int cpt, int cus, int mode) {
+static int stencil_slots(int storage, int M, int cpt, int cus, int mode, bool compact = false) {
   if (storage == FOMO_STORAGE_BF16) return stencil_slots_reduced<ST_BF16>(M, cpt, cus);
   if (storage == FOMO_STORAGE_F16) return stencil_slots_reduced<ST_F16>(M, cpt, cus);
   if (storage == FOMO_STORAGE_F16C) return stencil_slots_reduced<ST_F16C>(M, cpt, cus);
   switch (M) {
-    case 1: return stencil_slots_m<ST_F32, 1>(cpt, cus, mode);
-    case 2: return stencil_slots_m<ST_F32, 2>(cpt, cus, mode);
-    case 3: return stencil_slots_m<ST_F32, 3>(cpt, cus, mode);
-    case 4: return stencil_slots_m<ST_F32, 4>(cpt, cus, mode);
-    case 5: return stencil_slots_m<ST_F32, 5>(cpt, cus, mode);
-    case 6: return stencil_slots_m<ST_F32, 6>(cpt, cus, mode);
-    case 7: return stencil_slots_m<ST_F32, 7>(cpt, cus, mode);
-    default: return stencil_slots_m<ST_F32, 8>(cpt, cus, mode);
+    case 1: return stencil_slots_m<ST_F32, 1>(cpt, cus, mode, compact);
+    case 2: return stencil_slots_m<ST_F32, 2>(cpt, cus, mode, compact);
+    case 3: return stencil_slots_m<ST_F32, 3>(cpt, cus, mode, compact);
+    case 4: return stencil_slots_m<ST_F32, 4>(cpt, cus, mode, compact);
+    case 5: return stencil_slots_m<ST_F32, 5>(cpt, cus, mode, compact);
+    case 6: return stencil_slots_m<ST_F32, 6>(cpt, cus, mode, compact);
+    case 7: return stencil_slots_m<ST_F32, 7>(cpt, cus, mode, compact);
+    default: return stencil_slots_m<ST_F32, 8>(cpt, cus, mode, compact);
   }
 }
 
@@ -1115,6 +1160,8 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
     b->ss.push_back(ss[i]);
     if (ms[i]->d.dtx != ms[i]->d.dtz) b->same = false;
   }
+  b->compact = b->same && b->storage == FOMO_STORAGE_F32;
+  for (int i = 0; i < n; ++i) b->compact = b->compact && ms[i]->compact;
   if (b->tz == 0) {
     std::vector<int> tx, nzs;
     for (int i = 0; i < n; ++i) {
@@ -1123,7 +1170,7 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
       tx.push_back((ms[i]->d.nx + w - 1) / w);
       nzs.push_back(ms[i]->d.nz);
     }
-    b->tz = auto_tz(tx, nzs, stencil_slots(b->storage, b->M, b->cpt, c->cus, b->step_mode), b->M,
+    b->tz = auto_tz(tx, nzs, stencil_slots(b->storage, b->M, b->cpt, c->cus, b->step_mode, b->compact), b->M,
                     b->step_mode == 1 ? 2 * b->M + 2 : 0);
   }
   b->hs.resize((size_t)TBL_COUNT * n);
@@ -1216,12 +1263,17 @@ static void launch_stencil_mc(const fomo_batch* b, int table, int kind, int flag
   hipStream_t st = lstream(b);
   if (kind == 2) {  // grid rounded up to whole XCD rounds (see xcd_block)
     const dim3 grid(round_up(blocks_for(b, PART_FUSED), NUM_XCD));
-    if (b->same)
-      hipLaunchKernelGGL((k_fused<M, CPT, WPB, K, true>), grid, block, 0, st,
+    if (b->same && b->compact) {
+      if constexpr (K == ST_F32)
+        hipLaunchKernelGGL((k_fused<M, CPT, WPB, K, true, true>), grid, block, 0, st,
+                           args_for(b, table, PART_FUSED, k_off), flags);
+    } else if (b->same) {
+      hipLaunchKernelGGL((k_fused<M, CPT, WPB, K, true, false>), grid, block, 0, st,
                          args_for(b, table, PART_FUSED, k_off), flags);
-    else
-      hipLaunchKernelGGL((k_fused<M, CPT, WPB, K, false>), grid, block, 0, st,
+    } else {
+      hipLaunchKernelGGL((k_fused<M, CPT, WPB, K, false, false>), grid, block, 0, st,
                          args_for(b, table, PART_FUSED, k_off), flags);
+    }
   } else if (kind == 3) {
     if constexpr (CPT == 1)  // one instance per (K, M)
       hipLaunchKernelGGL((k_band<M, WPB, K>), dim3(blocks_for(b, PART_BAND)), block, 0, st,
@@ -1744,6 +1796,7 @@ extern "C" int32_t fomo_pool_create(fomo_ctx* c, int32_t fd_order, int32_t stora
   b->M = M;
   b->storage = storage;
   b->same = true;
+  b->compact = storage == FOMO_STORAGE_F32;  // until a member without rho_vx / mu_txx blocks joins
   b->cpt = c->cpt;
   b->graph_steps = c->graph_steps;
   b->step_mode = 1;
@@ -1755,7 +1808,7 @@ extern "C" int32_t fomo_pool_create(fomo_ctx* c, int32_t fd_order, int32_t stora
   // fixed persistent grids: one round of resident wavefronts for the stencil
   // kernels, a few blocks per CU for the strip kernels
   const int cus = std::max(1, c->cus);
-  b->grid_fixed[PART_FUSED] = round_up(stencil_slots(storage, M, b->cpt, cus, 1) / WPB, NUM_XCD);
+  b->grid_fixed[PART_FUSED] = round_up(stencil_slots(storage, M, b->cpt, cus, 1, b->compact) / WPB, NUM_XCD);
   b->grid_fixed[PART_BAND] = 2 * cus;
   b->grid_fixed[PART_HABC_V] = 4 * cus;
   b->grid_fixed[PART_HABC_S] = 4 * cus;
@@ -1811,6 +1864,19 @@ extern "C" int32_t fomo_pool_add(fomo_pool* p, fomo_model* m, fomo_wavefield* w,
   int rc = adopt_scales(w, s);
   if (rc) return rc;
   HIP_TRY(c, hipSetDevice(c->device));
+  if (p->b->compact && !m->compact) {
+    // this member has no rho_vx / mu_txx blocks: recapture with the kernel
+    // that reads the five stored coefficient arrays
+    HIP_TRY(c, hipStreamSynchronize(p->stream));
+    fomo_batch* b = p->b;
+    for (int q = 0; q < 2; ++q) {
+      if (b->gexec[q]) (void)hipGraphExecDestroy(b->gexec[q]);
+      if (b->graph[q]) (void)hipGraphDestroy(b->graph[q]);
+      b->gexec[q] = nullptr;
+      b->graph[q] = nullptr;
+    }
+    b->compact = false;
+  }
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, p->stream, s->dk, 1);
   p->ms.push_back(m);
   p->ws.push_back(w);

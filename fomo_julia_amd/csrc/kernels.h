@@ -705,6 +705,18 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
 // array each (a compile-time fact: a run-time choice between the two makes
 // the compiler copy the prefetched values right after issuing their loads).
 //
+// CMP (FP32 with dx == dz only): the medium comes as three arrays, rho_vx,
+// lam and mu_txx (12 B per cell instead of the 20 B of dt/rho_vx, dt/rho_vz,
+// lam, lam+2mu, mu_txz), and the kernel forms the rest per cell with the
+// setup's formulas and IEEE operations: dtx/rho_vx; rho_vz =
+// 0.25*(((r(i,j) + r(i+1,j)) + r(i,j+1)) + r(i+1,j+1)) (init.jl:146-149);
+// lam + 2mu; mu_txz = 2/(1/mu(i,j) + 1/mu(i,j+1)) (init.jl:151-155), with
+// 1/mu of row j+1 carried to the next row.  The model uses it only when its
+// rho_vz and mu_txz are exactly those averages, so results are unchanged.
+// The column right of a lane's last column comes from the next lane (DPP);
+// the tile's last column gets 0 there, and its vz is neither stored nor
+// read by a stored stress.
+//
 // Register windows are rings of L = 2M+2 rows: row r lives in slot
 // (r - jb) mod L, and the z-march is unrolled by L so that every slot index
 // is a compile-time constant.  Rotating the windows is then register renaming
@@ -712,8 +724,9 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
 // its first use.  The march runs in whole blocks of L rows; rows past the
 // tile's last needed row are computed from clamped (zero guard) rows and not
 // stored.
-template <int M, int CPT, int WPB, int K, bool SAME>
+template <int M, int CPT, int WPB, int K, bool SAME, bool CMP>
 __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
+  static_assert(!CMP || (K == ST_F32 && SAME), "derived coefficients: FP32, dtx == dtz");
   constexpr int KF = Mode<K>::F, KC = Mode<K>::C;
   typedef typename Store<KF>::T T;
   typedef typename Store<KC>::T TC;
@@ -759,6 +772,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     const uint32_t cstr = (uint32_t)uniform((int)sd.cstr_b), corg = (uint32_t)uniform((int)sd.corg_b);
     // coefficient blocks (dtz arrays alias the dtx ones when SAME)
     constexpr int QB1 = 0, QB3 = 1, QLAM = 2, QL2M = 3, QMU = 4, QB2 = SAME ? 0 : 5, QB4 = SAME ? 1 : 6;
+    constexpr int QRHO = 5, QMUX = 6;  // CMP: rho_vx and mu_txx blocks (after the five of SAME)
     const int nw = uniform(sd.nw), stype = uniform(sd.stype);
     const bool src_on = (flags & FLAG_INJECT) && k <= nw;
     const float wav = src_on ? uniform_f(uniform_p(sd.wavelet)[k - 1]) : 0.f;
@@ -810,6 +824,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     V<CPT> rtxx[L], rtxz[L], rtzz[L];  // old stress rings
     V<CPT> qvx[L], qvz[L];             // new velocity rings
     V<CPT> cvx[2], cvz[2], cb1[2], cb2[2], cb3[2], cb4[2], clam[2], cl2m[2], cmu[2];
+    V<CPT> crho[L], cmux[L], imu;  // CMP: rho rows (velocity), mu rows (stress), 1/mu of row j
     float htxx[2], htxz[2];
     const int jb = j0 - M;  // first velocity row; row r sits in ring slot (r - jb) mod L
     // rows the march reads before its first row: txx jb-M..jb, txz jb-M..jb+M-1,
@@ -830,15 +845,25 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     htxz[0] = hal(F_TXZ, jb, rxz);
     cvx[0] = fld(F_VX, jb, rvx);
     cvz[0] = fld(F_VZ, jb, rvz);
-    cb1[0] = cof(QB1, jb, r1);
-    cb3[0] = cof(QB3, jb, r3);
-    if constexpr (!SAME) {
-      cb2[0] = cof(QB2, jb, r2);
-      cb4[0] = cof(QB4, jb, r4);
+    if constexpr (CMP) {
+      // rho rows sit in ring slot (r - jb) mod L, mu rows likewise
+      crho[0] = cof(QRHO, jb, 1.f);
+      crho[1] = cof(QRHO, jb + 1, 1.f);
+      cmux[(L - M) % L] = cof(QMUX, jb - M, 1.f);
+      cmux[(L - M + 1) % L] = cof(QMUX, jb - M + 1, 1.f);
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) imu.v[c] = 1.0f / cmux[(L - M) % L].v[c];
+    } else {
+      cb1[0] = cof(QB1, jb, r1);
+      cb3[0] = cof(QB3, jb, r3);
+      if constexpr (!SAME) {
+        cb2[0] = cof(QB2, jb, r2);
+        cb4[0] = cof(QB4, jb, r4);
+      }
+      cl2m[0] = cof(QL2M, jb - M, rl2);
+      cmu[0] = cof(QMU, jb - M, rmu);
     }
     clam[0] = cof(QLAM, jb - M, rl);
-    cl2m[0] = cof(QL2M, jb - M, rl2);
-    cmu[0] = cof(QMU, jb - M, rmu);
     const bool own_lane = lo >= HL && lo < LC - HL;
 
     const int jend = j1 + M;
@@ -856,15 +881,20 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
         rtzz[(u + M + 1) % L] = fld(F_TZZ, jv + M + 1, rzz);
         cvx[nu] = fld(F_VX, jn, rvx);
         cvz[nu] = fld(F_VZ, jn, rvz);
-        cb1[nu] = cof(QB1, jn, r1);
-        cb3[nu] = cof(QB3, jn, r3);
-        if constexpr (!SAME) {
-          cb2[nu] = cof(QB2, jn, r2);
-          cb4[nu] = cof(QB4, jn, r4);
+        if constexpr (CMP) {
+          crho[(u + 2) % L] = cof(QRHO, jv + 2, 1.f);
+          cmux[(u - M + 2 + L) % L] = cof(QMUX, jv - M + 2, 1.f);
+        } else {
+          cb1[nu] = cof(QB1, jn, r1);
+          cb3[nu] = cof(QB3, jn, r3);
+          if constexpr (!SAME) {
+            cb2[nu] = cof(QB2, jn, r2);
+            cb4[nu] = cof(QB4, jn, r4);
+          }
+          cl2m[nu] = cof(QL2M, jn - M, rl2);
+          cmu[nu] = cof(QMU, jn - M, rmu);
         }
         clam[nu] = cof(QLAM, jn - M, rl);
-        cl2m[nu] = cof(QL2M, jn - M, rl2);
-        cmu[nu] = cof(QMU, jn - M, rmu);
 
         // ---- first uses of rows loaded one iteration ago: txx row jv, tzz row jv+M
         add_src(rtxx[u % L], jv);
@@ -885,6 +915,11 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
         }
         wave_sync();
         V<CPT> nvx, nvz;
+        float nb0 = 0.f, nb1 = 0.f;  // CMP: rho right of this lane's last column, rows jv, jv+1
+        if constexpr (CMP) {
+          nb0 = from_upper_lane(crho[u % L].v[0]);
+          nb1 = from_upper_lane(crho[(u + 1) % L].v[0]);
+        }
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
           const int x = M + lo + c;
@@ -903,7 +938,17 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
             if (src_vz) vzs = vzs + wav * src_scale;
             if (src_vx) vxs = vxs + wav * src_scale;
           }
-          const float b1 = cb1[cu].v[c], b3 = cb3[cu].v[c];
+          float b1, b3;
+          if constexpr (CMP) {
+            const float q0 = crho[u % L].v[c], q1 = crho[(u + 1) % L].v[c];
+            const float q0n = c + 1 < CPT ? crho[u % L].v[min(c + 1, CPT - 1)] : nb0;
+            const float q1n = c + 1 < CPT ? crho[(u + 1) % L].v[min(c + 1, CPT - 1)] : nb1;
+            b1 = dtx / q0;
+            b3 = dtx / (0.25f * (((q0 + q0n) + q1) + q1n));
+          } else {
+            b1 = cb1[cu].v[c];
+            b3 = cb3[cu].v[c];
+          }
           const float b2 = SAME ? b1 : cb2[cu].v[c], b4 = SAME ? b3 : cb4[cu].v[c];
           const float ix = fmaf_(b2, d2, b1 * d1);
           const float iz = fmaf_(b4, d4, b3 * d3);
@@ -927,6 +972,17 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
         // so that every store is issued on every row)
         const int j = jv - M;
         const bool srow = j >= j0 && j < j1;
+        V<CPT> dl2, dmt;  // CMP: lam + 2mu and mu_txz of row j
+        if constexpr (CMP) {
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) {
+            const float m0 = cmux[(u - M + L) % L].v[c], m1 = cmux[(u - M + 1 + L) % L].v[c];
+            const float i1 = 1.0f / m1;
+            dmt.v[c] = 2.0f / (imu.v[c] + i1);
+            dl2.v[c] = clam[cu].v[c] + 2.0f * m0;
+            imu.v[c] = i1;
+          }
+        }
         V<CPT> oxx, ozz, oxz;
 #pragma unroll
         for (int c = 0; c < CPT; ++c) oxx.v[c] = ozz.v[c] = oxz.v[c] = 0.f;
@@ -957,11 +1013,11 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
             const bool inside = (i >= M) & (i < nx - M) & (j >= M) & (j < nz - M);
             const float xx = rtxx[sm].v[c], zz = rtzz[sm].v[c], xz = rtxz[sm].v[c];
             const float t1 = d1 * dtx, t2 = d2 * dtz;
-            const float lv = clam[cu].v[c], l2 = cl2m[cu].v[c];
+            const float lv = clam[cu].v[c], l2 = CMP ? dl2.v[c] : cl2m[cu].v[c];
             const float ixx = fmaf_(lv, t2, l2 * t1);
             const float izz = fmaf_(l2, t2, lv * t1);
             const float sxz = fmaf_(d4, dtx, d3 * dtz);
-            const float ixz = cmu[cu].v[c] * sxz;
+            const float ixz = (CMP ? dmt.v[c] : cmu[cu].v[c]) * sxz;
             oxx.v[c] = inside ? xx + ixx : xx;
             ozz.v[c] = inside ? zz + izz : zz;
             oxz.v[c] = inside ? xz + ixz : xz;

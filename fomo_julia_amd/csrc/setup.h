@@ -27,6 +27,8 @@ struct SetupArgs {
   float dtx, dtz;
   int same;         // dtx == dtz: arrays 1 and 3 alias 0 and 2 (not written)
   void* out[7];     // coefficient block origins (element type per kind)
+  float* out_rho;   // FP32 models of the derived-coefficient kernel: rho_vx and
+  float* out_mu;    // mu_txx blocks (null otherwise)
   float sc[7];      // FP16 store scales (1 for the other kinds)
   float* partial;   // [gridDim.x][9]: max |c_k| (k < 7), min and max impedance
 };
@@ -42,10 +44,11 @@ __device__ __forceinline__ void medium_at(const SetupArgs& A, int i, int j, floa
 }
 
 __device__ __forceinline__ void coefficients_at(const SetupArgs& A, int i, int j, float c[7],
-                                                float& rho_vx) {
+                                                float& rho_vx, float& mu_txx) {
   float r, mu, lam, r2, mu2, lam2;
   medium_at(A, i, j, r, mu, lam);
   rho_vx = r;
+  mu_txx = mu;
   const int i2 = min(i, A.nx - 2), j2 = min(j, A.nz - 2);
   float a, b, e, f, tmp;
   medium_at(A, i2, j2, a, tmp, tmp);
@@ -96,8 +99,8 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A, int pass) {
   for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < n;
        o += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(o % A.nx), j = (int)(o / A.nx);
-    float c[7], rvx;
-    coefficients_at(A, i, j, c, rvx);
+    float c[7], rvx, mux;
+    coefficients_at(A, i, j, c, rvx, mux);
     if (pass == 0) {
 #pragma unroll
       for (int k = 0; k < 7; ++k) mx[k] = fmaxf(mx[k], fabsf(c[k]));
@@ -107,6 +110,12 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A, int pass) {
       continue;
     }
     const size_t off = (size_t)j * A.pitch + i;
+    if constexpr (KC == ST_F32) {
+      if (A.out_rho) {
+        A.out_rho[off] = rvx;
+        A.out_mu[off] = mux;
+      }
+    }
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
       if (A.same && (k == 1 || k == 3)) continue;

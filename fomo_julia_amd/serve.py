@@ -132,7 +132,9 @@ def device_bytes(nx_pad: int, nz_pad: int, storage: str, nt: int, n_rec: int) ->
     ces = 4 if storage == "f32" else 2
     fes = 2 if storage in ("bf16", "f16") else 4
     fields = 15 if storage == "f32" else 10
-    return 5 * stride * ces + fields * stride * fes + nt * max(n_rec, 1) * 4 + 3 * 4096
+    # FP32 models also hold rho_vx and mu_txx for the derived-coefficient fused kernel
+    coef = 7 if storage == "f32" else 5
+    return coef * stride * ces + fields * stride * fes + nt * max(n_rec, 1) * 4 + 3 * 4096
 
 
 def prepare(req: ShotRequest) -> Prepared:

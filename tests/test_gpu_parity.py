@@ -236,3 +236,72 @@ def test_non_ramp_weights_rejected(b):
     W = E.Wavefield(c.M.nx, c.M.nz, b)
     with pytest.raises(Exception, match="edge distance"):
         E.bind(b, W, c.M, c.H, c.a, c.p)
+
+
+def _perturbed_case(**kw):
+    """A case whose mu_txz is not init_medium's harmonic average in one
+    interior cell: the model keeps the five stored coefficient arrays."""
+    c = Case(**kw)
+    i, j = c.M.nx // 2, c.M.nz // 2
+    c.M.mu_txz = np.array(c.M.mu_txz, copy=True)
+    c.M.mu_txz[i, j] = np.nextafter(c.M.mu_txz[i, j], np.float32(np.inf))
+    return c
+
+
+@pytest.mark.parametrize("fs,src_type", [(True, "pressure"), (False, "force_z")])
+def test_stored_and_derived_coefficients_bit_exact(b, fs, src_type):
+    """Media with init_medium's averages run the fused kernel that forms the
+    coefficients from rho_vx, lam and mu_txx; any other medium keeps the five
+    stored arrays.  Both are bit-exact against the oracle on their own arrays."""
+    for perturbed in (False, True):
+        c = (_perturbed_case if perturbed else Case)(fs=fs, src_type=src_type, nt=120, water=fs)
+        print(f"case coefficients perturbed={perturbed} fs={fs} src={src_type}", flush=True)
+        res, W = gpu_shot(b, c, cpt=2)
+        b.set_tiling(1, 0, 16)
+        dm = E._model(b, c.M, c.H, c.a, c.p)
+        pitch = (c.M.nx + 272 + 255) // 256 * 256
+        block = 4 * ((c.M.nz + 36) * pitch + 512)  # one pitched FP32 array with guards
+        ramps = 4 * 3 * (c.H.nbc + 2)
+        blocks = (int(b.lib.fomo_model_device_bytes(dm.h)) - ramps) / block
+        # 5 + rho_vx, mu_txx for the derived form, 5 otherwise
+        assert blocks == (5 if perturbed else 7), blocks
+        r = c.oracle()
+        r.run(1, c.nt)
+        assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+        assert_same_state(W, r, f"perturbed={perturbed}")
+
+
+def test_pool_mixing_stored_and_derived_coefficients(b):
+    """A pool that starts with derived-coefficient members and then admits a
+    member with stored arrays recaptures its step and stays bit-exact."""
+    import ctypes
+    cases = [Case(nx_in=70, nz_in=50, nt=48, seed=1), _perturbed_case(nx_in=64, nz_in=58, nt=48, seed=2)]
+    print("case pool mixed coefficient forms", flush=True)
+    b.set_tiling(1, 0, 16)
+    b.set_step_mode(1)
+    solo = []
+    for c in cases:
+        W = E.Wavefield(c.M.nx, c.M.nz, b)
+        solo.append(E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p).gather)
+    lib = b.lib
+    hp = ctypes.c_void_p()
+    b.check(lib.fomo_pool_create(b.ctx, 8, 0, 4, ctypes.byref(hp)), "pool_create")
+    keep = []
+    try:
+        for q, c in enumerate(cases):
+            dm = E._model(b, c.M, c.H, c.a, c.p)
+            W = E.Wavefield(c.M.nx, c.M.nz, b)
+            W._bind(dm)
+            E.reset(b, W)
+            ds = E._DeviceShot(dm, c.src, c.rec, c.nt)
+            keep.append((W, ds))
+            b.check(lib.fomo_pool_add(hp, dm.h, W._h, ds.h), "pool_add")
+            if q == 0:
+                b.check(lib.fomo_pool_run(hp, 16), "pool_run")  # captured with the derived form
+        # member 0 is 16 steps ahead: run until both are done
+        b.check(lib.fomo_pool_run(hp, 48), "pool_run")
+        b.check(lib.fomo_pool_synchronize(hp), "pool_sync")
+        assert np.array_equal(keep[0][1].gather(), solo[0])
+        assert np.array_equal(keep[1][1].gather(), solo[1])
+    finally:
+        lib.fomo_pool_destroy(hp)
