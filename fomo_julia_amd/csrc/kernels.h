@@ -1273,6 +1273,9 @@ __device__ __forceinline__ HabcField habc_field(const ShotDev& sd, int stress, i
 }
 
 constexpr int HS = 65;  // LDS row stride: odd, so row and column walks are bank-conflict free
+// An LDS slot of the staging block that nothing reads (row 63, column 64):
+// recurrence steps of lanes with no cell store there instead of branching.
+constexpr int HDUMMY = 64 * HS - 1;
 
 // rows [r0, r0+nr) x cols [c0, c0+nc) of the field g and its backup go -> s, so
 // (s[r*HS + c], so[r*HS + c]), nc <= 64.  Loads of both arrays go out 64 rows
@@ -1503,13 +1506,15 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       __syncthreads();
       float up = act ? s_f[lane] : 0.f, ou = act ? s_o[lane] : 0.f;
       const int la = act ? lane : 0;  // idle lanes read a valid cell and never store
+      // edge distance of cell (1+la, r0+r) in this corner: min(1+la, n+1-r)
+      const int wl = min(1 + la, n + 1);
       float pf[HPF], po[HPF], pw[HPF];
 #pragma unroll
       for (int k = 0; k < HPF; ++k) {
         const int r = min(1 + k, n);
         pf[k] = s_f[r * HS + la];
         po[k] = s_o[r * HS + la];
-        pw[k] = W(c0 + la, r0 + r);
+        pw[k] = s_w[min(wl, n + 1 - r)];
       }
       for (int rg = 1; rg <= n; rg += HPF) {
         float nf[HPF], no[HPF], nw[HPF];
@@ -1518,7 +1523,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
           const int r = min(rg + HPF + k, n);
           nf[k] = s_f[r * HS + la];
           no[k] = s_o[r * HS + la];
-          nw[k] = W(c0 + la, r0 + r);
+          nw[k] = s_w[min(wl, n + 1 - r)];
         }
 #pragma unroll
         for (int k = 0; k < HPF; ++k) {
@@ -1530,7 +1535,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
             const float orr = from_upper_lane(ov);
             const float out =
                 (lane == n) ? x : q.corner(w, fv, q.sumx(fr, ov, orr), q.sumz(up, ov, ou));
-            if (act) s_f[(rg + k) * HS + lane] = out;
+            s_f[act ? (rg + k) * HS + lane : HDUMMY] = out;
             up = out;
             ou = ov;
           }
@@ -1548,16 +1553,20 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       // nx-n-2 (recurrence along +z), the rest the right-bottom corner.
       const int L = lane;
       float last = act ? s_f[L] : 0.f;  // lane 0 starts with cell (0,0), which is not updated
-      // operands of the cell this lane handles at step d (prefetched one step ahead)
+      // Operands of the cell this lane handles at step d, row rc = d - L
+      // clamped to the block (out-of-block steps read a valid cell and store
+      // nothing).  In this corner the edge distance of cell (c0+L, r0+rc) is
+      // min(n+1-L, n+1-rc), so the weight index needs no grid scalars.
+      const int wl = act ? n + 1 - L : 0;
+      const int lm = L > 0 ? 1 : 0;
       auto fetch = [&](int d, float& fv, float& ov, float& ol, float& ou, float& w) {
-        const int r = d - L;
-        const bool ok = act && r >= 0 && r <= n;
-        const int rr = ok ? r : 0;
-        fv = s_f[rr * HS + L];
-        ov = s_o[rr * HS + L];
-        ol = s_o[rr * HS + max(L - 1, 0)];
-        ou = s_o[max(rr - 1, 0) * HS + L];
-        w = W(c0 + L, r0 + rr);
+        const int rc = min(max(d - L, 0), n);
+        const int o = rc * HS + L;
+        fv = s_f[o];
+        ov = s_o[o];
+        ol = s_o[o - lm];
+        ou = s_o[rc > 0 ? o - HS : o];
+        w = s_w[min(wl, n + 1 - rc)];
       };
       float pf[HPF], po[HPF], pl[HPF], pu[HPF], pw[HPF];
 #pragma unroll
@@ -1581,10 +1590,9 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
             const float sz = (r == 0) ? 0.f : q.sumz(last, ov, ou);
             const float cw = (L == 0 || r == 0) ? (1.0f - w) : (1.0f - w) * 0.5f;
             const float out = fmaf_(cw, sx + sz, w * fv);
-            if (act && r >= 0 && r <= n && !(L == 0 && r == 0)) {
-              s_f[r * HS + L] = out;
-              last = out;
-            }
+            const bool valid = act && (unsigned)r <= (unsigned)n && !(L == 0 && r == 0);
+            s_f[valid ? r * HS + L : HDUMMY] = out;
+            last = valid ? out : last;
           }
         }
 #pragma unroll
