@@ -1278,10 +1278,12 @@ constexpr int HS = 65;  // LDS row stride: odd, so row and column walks are bank
 constexpr int HDUMMY = 64 * HS - 1;
 
 // rows [r0, r0+nr) x cols [c0, c0+nc) of the field g and its backup go -> s, so
-// (s[r*HS + c], so[r*HS + c]), nc <= 64.  Loads of both arrays go out 64 rows
-// at a time before any LDS store: a piece (at most 64 rows) costs one memory
-// latency.
-constexpr int STAGE_CHUNK = 64;
+// (s[r*HS + c], so[r*HS + c]), nc <= 64.  Loads of both arrays go out a chunk
+// of rows at a time before any LDS store, so a chunk costs one memory latency:
+// 32 rows for FP32 fields, a whole piece (64 rows) for 16-bit ones (measured
+// fastest on the SEAM shot and the 64-shot batch in each case).
+template <int K>
+constexpr int stage_chunk() { return sizeof(typename Store<K>::T) == 2 ? 64 : 32; }
 // Recurrence loops read their operands (pre-HABC values, which no earlier
 // step of the loop writes) this many steps ahead, so LDS latency stays off the
 // chain of dependent steps.
@@ -1290,6 +1292,7 @@ template <int K>
 __device__ __forceinline__ void lds_in2(float* s, float* so, const typename Store<K>::T* g,
                                         const typename Store<K>::T* go, float rs, int P, int r0,
                                         int nr, int c0, int nc, int lane) {
+  constexpr int STAGE_CHUNK = stage_chunk<K>();
   const bool on = lane < nc;
   for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
     float v[STAGE_CHUNK], w[STAGE_CHUNK];
@@ -1312,6 +1315,7 @@ __device__ __forceinline__ void lds_in2(float* s, float* so, const typename Stor
 template <int K>
 __device__ __forceinline__ void lds_out(const float* s, int sr, int sc, typename Store<K>::T* g,
                                         float ws, int P, int r0, int nr, int c0, int nc, int lane) {
+  constexpr int STAGE_CHUNK = stage_chunk<K>();
   const bool on = lane < nc;
   for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
     float v[STAGE_CHUNK];
