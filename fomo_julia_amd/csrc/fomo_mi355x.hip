@@ -118,6 +118,7 @@ struct fomo_batch {
   // every member's model holds rho_vx / mu_txx for the derived-coefficient
   // fused kernel (FP32, dtx == dtz)
   bool compact = false;
+  int habc_rows = 1;  // largest member nbc + 1: LDS rows of the HABC staging arrays
   std::vector<fomo_wavefield*> ws;
   std::vector<fomo_shot*> ss;
   std::vector<ShotDev> hs;  // [TBL_COUNT][n]
@@ -1161,7 +1162,10 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
     if (ms[i]->d.dtx != ms[i]->d.dtz) b->same = false;
   }
   b->compact = b->same && b->storage == FOMO_STORAGE_F32;
-  for (int i = 0; i < n; ++i) b->compact = b->compact && ms[i]->compact;
+  for (int i = 0; i < n; ++i) {
+    b->compact = b->compact && ms[i]->compact;
+    b->habc_rows = std::max(b->habc_rows, ms[i]->d.nbc + 1);
+  }
   if (b->tz == 0) {
     std::vector<int> tx, nzs;
     for (int i = 0; i < n; ++i) {
@@ -1332,12 +1336,14 @@ static void launch_habc(const fomo_batch* b, int table, bool stress, int k_off) 
   const int items = b->hprefix[(size_t)part * (b->n + 1) + b->n];
   const BatchArgs a = args_for(b, table, part, k_off);
   const dim3 g(b->grid_fixed[part] > 0 ? b->grid_fixed[part] : std::max(1, items));
+  const int rows = b->habc_rows;
+  const size_t lds = (size_t)2 * rows * HS * sizeof(float);
   if (b->storage == FOMO_STORAGE_BF16)
-    hipLaunchKernelGGL(k_habc<ST_BF16>, g, dim3(64), 0, lstream(b), a, stress ? 1 : 0);
+    hipLaunchKernelGGL(k_habc<ST_BF16>, g, dim3(64), lds, lstream(b), a, stress ? 1 : 0, rows);
   else if (b->storage == FOMO_STORAGE_F16)
-    hipLaunchKernelGGL(k_habc<ST_F16>, g, dim3(64), 0, lstream(b), a, stress ? 1 : 0);
+    hipLaunchKernelGGL(k_habc<ST_F16>, g, dim3(64), lds, lstream(b), a, stress ? 1 : 0, rows);
   else
-    hipLaunchKernelGGL(k_habc<ST_F32>, g, dim3(64), 0, lstream(b), a, stress ? 1 : 0);
+    hipLaunchKernelGGL(k_habc<ST_F32>, g, dim3(64), lds, lstream(b), a, stress ? 1 : 0, rows);
 }
 
 static void launch_fsrec(const fomo_batch* b, int table, int mode, int k_off) {
@@ -1864,9 +1870,12 @@ extern "C" int32_t fomo_pool_add(fomo_pool* p, fomo_model* m, fomo_wavefield* w,
   int rc = adopt_scales(w, s);
   if (rc) return rc;
   HIP_TRY(c, hipSetDevice(c->device));
-  if (p->b->compact && !m->compact) {
-    // this member has no rho_vx / mu_txx blocks: recapture with the kernel
-    // that reads the five stored coefficient arrays
+  // A member without rho_vx / mu_txx blocks needs the kernel that reads the
+  // five stored coefficient arrays; a wider HABC strip needs more LDS rows.
+  // Either way the captured step is rebuilt at the next run.
+  const bool to_stored = p->b->compact && !m->compact;
+  const bool wider = m->d.nbc + 1 > p->b->habc_rows;
+  if (to_stored || wider) {
     HIP_TRY(c, hipStreamSynchronize(p->stream));
     fomo_batch* b = p->b;
     for (int q = 0; q < 2; ++q) {
@@ -1875,7 +1884,8 @@ extern "C" int32_t fomo_pool_add(fomo_pool* p, fomo_model* m, fomo_wavefield* w,
       b->gexec[q] = nullptr;
       b->graph[q] = nullptr;
     }
-    b->compact = false;
+    if (to_stored) b->compact = false;
+    if (wider) b->habc_rows = m->d.nbc + 1;
   }
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, p->stream, s->dk, 1);
   p->ms.push_back(m);

@@ -1273,9 +1273,10 @@ __device__ __forceinline__ HabcField habc_field(const ShotDev& sd, int stress, i
 }
 
 constexpr int HS = 65;  // LDS row stride: odd, so row and column walks are bank-conflict free
-// An LDS slot of the staging block that nothing reads (row 63, column 64):
-// recurrence steps of lanes with no cell store there instead of branching.
-constexpr int HDUMMY = 64 * HS - 1;
+// The staging arrays hold `rows` = (largest nbc of the launch) + 1 LDS rows
+// of HS floats each (dynamic LDS, so small strips leave room for more blocks
+// per CU).  Column 64 of the last row is never read: recurrence steps of
+// lanes with no cell store there instead of branching.
 
 // rows [r0, r0+nr) x cols [c0, c0+nc) of the field g and its backup go -> s, so
 // (s[r*HS + c], so[r*HS + c]), nc <= 64.  Loads of both arrays go out a chunk
@@ -1328,15 +1329,60 @@ __device__ __forceinline__ void lds_out(const float* s, int sr, int sc, typename
   }
 }
 
+// Transposed staging for the side edges: rows [r0, r0+nr) x cols [c0, c0+nc)
+// of g and its backup -> (s[c*HS + r], so[c*HS + r]), nr <= 64, nc <= 64, so a
+// side piece takes nc (= nbc+1) LDS rows like every other piece.
+template <int K>
+__device__ __forceinline__ void lds_in2_t(float* s, float* so, const typename Store<K>::T* g,
+                                          const typename Store<K>::T* go, float rs, int P, int r0,
+                                          int nr, int c0, int nc, int lane) {
+  constexpr int STAGE_CHUNK = stage_chunk<K>();
+  const bool on = lane < nc;
+  for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
+    float v[STAGE_CHUNK], w[STAGE_CHUNK];
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r) {
+      const bool ok = on && rb + r < nr;
+      const ptrdiff_t o = (ptrdiff_t)(r0 + rb + r) * P + c0 + lane;
+      v[r] = ok ? ld1<K>(g + o, rs) : 0.f;
+      w[r] = ok ? ld1<K>(go + o, rs) : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r)
+      if (on && rb + r < nr) {
+        s[lane * HS + rb + r] = v[r];
+        so[lane * HS + rb + r] = w[r];
+      }
+  }
+}
+// s[(sc+c)*HS + r] -> rows [r0, r0+nr) x cols [c0, c0+nc) of g   (nr, nc <= 64)
+template <int K>
+__device__ __forceinline__ void lds_out_t(const float* s, int sc, typename Store<K>::T* g, float ws,
+                                          int P, int r0, int nr, int c0, int nc, int lane) {
+  constexpr int STAGE_CHUNK = stage_chunk<K>();
+  const bool on = lane < nc;
+  for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
+    float v[STAGE_CHUNK];
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r)
+      v[r] = (on && rb + r < nr) ? s[(sc + lane) * HS + rb + r] : 0.f;
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r)
+      if (on && rb + r < nr) st1<K>(g + (ptrdiff_t)(r0 + rb + r) * P + c0 + lane, v[r], ws);
+  }
+}
+
 // Items per field of the HABC partition and the first side-edge row.
 __device__ __forceinline__ int habc_side_r0(const ShotDev& sd) { return sd.fs ? sd.nbc : sd.nbc + 2; }
 
 template <int K>
-__global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
+__global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress, int rows) {
   constexpr int KF = Mode<K>::F;
   typedef typename Store<KF>::T T;
-  __shared__ float s_f[64 * HS];
-  __shared__ float s_o[64 * HS];
+  extern __shared__ float s_dyn[];  // 2 * rows * HS floats
+  float* const s_f = s_dyn;
+  float* const s_o = s_dyn + rows * HS;
+  const int hdummy = rows * HS - 1;
   __shared__ float s_w[MAX_NBC + 2];
   const int lane = threadIdx.x;
   const int nsh = *ba.n_shots_p;
@@ -1372,24 +1418,28 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       const bool right = p >= sd.n_l;
       const int j0 = ra + (right ? p - sd.n_l : p) * 64;
       const int nr = min(64, rb - j0);
-      const int c0 = right ? nx - n - 2 : 1;  // staged columns [c0, c0+n+1)
-      lds_in2<KF>(s_f, s_o, hf, hfo, hrs, P, j0, nr, c0, n + 1, lane);
+      const int c0 = right ? nx - n - 2 : 1;  // staged columns [c0, c0+n+1), transposed
+      lds_in2_t<KF>(s_f, s_o, hf, hfo, hrs, P, j0, nr, c0, n + 1, lane);
       __syncthreads();
       if (!right) {
         // left edge, i = 1..n: Jacobi on the row's pre-HABC values (rows independent)
 #pragma unroll 4
         for (int r = 0; r < nr; ++r) {
           if (lane < n) {
-            const float fv = s_f[r * HS + lane], fn = s_f[r * HS + lane + 1];
-            const float ov = s_o[r * HS + lane], on = s_o[r * HS + lane + 1];
+            const float fv = s_f[lane * HS + r], fn = s_f[(lane + 1) * HS + r];
+            const float ov = s_o[lane * HS + r], on = s_o[(lane + 1) * HS + r];
             const float out = q.edge(W(1 + lane, j0 + r), fv, q.sumx(fn, ov, on));
-            s_f[r * HS + lane] = out;
+            s_f[lane * HS + r] = out;
           }
         }
       } else if (lane < nr) {
         // right edge, one row per lane, recurrence along +x from column nx-n-2
-        float* fr = s_f + lane * HS;
-        const float* orow = s_o + lane * HS;
+        // (column c of this lane's row at [c*HS + lane])
+        struct Col {
+          float* p;
+          __device__ float& operator[](int c) const { return p[c * HS]; }
+        };
+        const Col fr{s_f + lane}, orow{s_o + lane};
         float prev = fr[0], op = orow[0];
         float pf[HPF], po[HPF], pw[HPF];
 #pragma unroll
@@ -1427,7 +1477,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
       }
       __syncthreads();
       // left: block columns 0..n-1 = columns 1..n; right: block columns 1..n
-      lds_out<KF>(s_f, 0, right ? 1 : 0, hf, hws, P, j0, nr, right ? c0 + 1 : 1, n, lane);
+      lds_out_t<KF>(s_f, right ? 1 : 0, hf, hws, P, j0, nr, right ? c0 + 1 : 1, n, lane);
       continue;
     }
     p -= sd.n_l + sd.n_r;
@@ -1539,7 +1589,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
             const float orr = from_upper_lane(ov);
             const float out =
                 (lane == n) ? x : q.corner(w, fv, q.sumx(fr, ov, orr), q.sumz(up, ov, ou));
-            s_f[act ? (rg + k) * HS + lane : HDUMMY] = out;
+            s_f[act ? (rg + k) * HS + lane : hdummy] = out;
             up = out;
             ou = ov;
           }
@@ -1595,7 +1645,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress) {
             const float cw = (L == 0 || r == 0) ? (1.0f - w) : (1.0f - w) * 0.5f;
             const float out = fmaf_(cw, sx + sz, w * fv);
             const bool valid = act && (unsigned)r <= (unsigned)n && !(L == 0 && r == 0);
-            s_f[valid ? r * HS + L : HDUMMY] = out;
+            s_f[valid ? r * HS + L : hdummy] = out;
             last = valid ? out : last;
           }
         }
