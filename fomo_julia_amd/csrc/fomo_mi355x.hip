@@ -1175,7 +1175,7 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
       nzs.push_back(ms[i]->d.nz);
     }
     b->tz = auto_tz(tx, nzs, stencil_slots(b->storage, b->M, b->cpt, c->cus, b->step_mode, b->compact), b->M,
-                    b->step_mode == 1 ? 2 * b->M + 2 : 0);
+                    b->step_mode == 1 ? fused_block(b->M) : 0);
   }
   b->hs.resize((size_t)TBL_COUNT * n);
   for (int t = 0; t < TBL_COUNT; ++t)
@@ -1807,8 +1807,8 @@ extern "C" int32_t fomo_pool_create(fomo_ctx* c, int32_t fd_order, int32_t stora
   b->graph_steps = c->graph_steps;
   b->step_mode = 1;
   b->step_mode_req = 1;
-  // rows per tile: about 32 with tz + 2M a whole number of march blocks (2M+2)
-  const int L = 2 * M + 2;
+  // rows per tile: about 32 with tz + 2M a whole number of march blocks
+  const int L = fused_block(M);
   b->tz = c->tz > 0 ? c->tz : std::max(8, round_up(32 + 2 * M, L) - 2 * M);
   b->tz_req = c->tz;
   // fixed persistent grids: one round of resident wavefronts for the stencil

@@ -724,6 +724,13 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
 // its first use.  The march runs in whole blocks of L rows; rows past the
 // tile's last needed row are computed from clamped (zero guard) rows and not
 // stored.
+// Rows the fused march loads ahead of their first use, and its ring length /
+// unroll (host and device: tile heights are chosen in whole march blocks).
+#ifndef FUSED_PD
+#define FUSED_PD 1
+#endif
+constexpr int fused_block(int M) { return 2 * M + 2 * FUSED_PD; }
+
 template <int M, int CPT, int WPB, int K, bool SAME, bool CMP>
 __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
   static_assert(!CMP || (K == ST_F32 && SAME), "derived coefficients: FP32, dtx == dtz");
@@ -733,7 +740,8 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
   constexpr int HL = ((M + CPT - 1) / CPT) * CPT;  // computed-not-written lane columns per side
   constexpr int LC = 64 * CPT;
   constexpr int LW = LC + 2 * M;
-  constexpr int L = 2 * M + 2;
+  constexpr int PD = FUSED_PD;
+  constexpr int L = fused_block(M);
   __shared__ float lds[WPB][4][LW];
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
@@ -823,47 +831,56 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
 
     V<CPT> rtxx[L], rtxz[L], rtzz[L];  // old stress rings
     V<CPT> qvx[L], qvz[L];             // new velocity rings
-    V<CPT> cvx[2], cvz[2], cb1[2], cb2[2], cb3[2], cb4[2], clam[2], cl2m[2], cmu[2];
+    // per-iteration operands, slot = iteration index mod L (the iteration of
+    // velocity row jv uses slot (jv - jb) mod L; stress coefficients there are
+    // those of row jv - M)
+    V<CPT> cvx[L], cvz[L], cb1[L], cb2[L], cb3[L], cb4[L], clam[L], cl2m[L], cmu[L];
     V<CPT> crho[L], cmux[L], imu;  // CMP: rho rows (velocity), mu rows (stress), 1/mu of row j
-    float htxx[2], htxz[2];
+    float htxx[L], htxz[L];
     const int jb = j0 - M;  // first velocity row; row r sits in ring slot (r - jb) mod L
-    // rows the march reads before its first row: txx jb-M..jb, txz jb-M..jb+M-1,
-    // tzz jb-M..jb+M (the source is added here to all but txx row jb and tzz
-    // row jb+M, which the first iteration adds like every later row)
+    // rows the march reads before its first row (PD rows of lookahead):
+    // txx jb-M..jb+PD-1, txz jb-M..jb+M+PD-2, tzz jb-M..jb+M+PD-1; the source is
+    // added here to txx rows below jb and tzz rows below jb+M, the later rows
+    // get it at their first use in the march
 #pragma unroll
-    for (int q = 0; q < 2 * M + 1; ++q) {
+    for (int q = 0; q < 2 * M + PD; ++q) {
       const int r = jb - M + q, sl = (q - M + L) % L;
-      if (q < 2 * M) rtxz[sl] = fld(F_TXZ, r, rxz);
+      if (q < 2 * M + PD - 1) rtxz[sl] = fld(F_TXZ, r, rxz);
       rtzz[sl] = fld(F_TZZ, r, rzz);
       if (q < 2 * M) add_src(rtzz[sl], r);
-      if (q <= M) {
+      if (q < M + PD) {
         rtxx[sl] = fld(F_TXX, r, rxx);
         if (q < M) add_src(rtxx[sl], r);
       }
     }
-    htxx[0] = hal(F_TXX, jb, rxx);
-    htxz[0] = hal(F_TXZ, jb, rxz);
-    cvx[0] = fld(F_VX, jb, rvx);
-    cvz[0] = fld(F_VZ, jb, rvz);
+#pragma unroll
+    for (int k = 0; k < PD; ++k) {
+      htxx[k] = hal(F_TXX, jb + k, rxx);
+      htxz[k] = hal(F_TXZ, jb + k, rxz);
+      cvx[k] = fld(F_VX, jb + k, rvx);
+      cvz[k] = fld(F_VZ, jb + k, rvz);
+      if constexpr (!CMP) {
+        cb1[k] = cof(QB1, jb + k, r1);
+        cb3[k] = cof(QB3, jb + k, r3);
+        if constexpr (!SAME) {
+          cb2[k] = cof(QB2, jb + k, r2);
+          cb4[k] = cof(QB4, jb + k, r4);
+        }
+        cl2m[k] = cof(QL2M, jb + k - M, rl2);
+        cmu[k] = cof(QMU, jb + k - M, rmu);
+      }
+      clam[k] = cof(QLAM, jb + k - M, rl);
+    }
     if constexpr (CMP) {
       // rho rows sit in ring slot (r - jb) mod L, mu rows likewise
-      crho[0] = cof(QRHO, jb, 1.f);
-      crho[1] = cof(QRHO, jb + 1, 1.f);
-      cmux[(L - M) % L] = cof(QMUX, jb - M, 1.f);
-      cmux[(L - M + 1) % L] = cof(QMUX, jb - M + 1, 1.f);
+#pragma unroll
+      for (int k = 0; k <= PD; ++k) {
+        crho[k] = cof(QRHO, jb + k, 1.f);
+        cmux[(k - M + L) % L] = cof(QMUX, jb - M + k, 1.f);
+      }
 #pragma unroll
       for (int c = 0; c < CPT; ++c) imu.v[c] = 1.0f / cmux[(L - M) % L].v[c];
-    } else {
-      cb1[0] = cof(QB1, jb, r1);
-      cb3[0] = cof(QB3, jb, r3);
-      if constexpr (!SAME) {
-        cb2[0] = cof(QB2, jb, r2);
-        cb4[0] = cof(QB4, jb, r4);
-      }
-      cl2m[0] = cof(QL2M, jb - M, rl2);
-      cmu[0] = cof(QMU, jb - M, rmu);
     }
-    clam[0] = cof(QLAM, jb - M, rl);
     const bool own_lane = lo >= HL && lo < LC - HL;
 
     const int jend = j1 + M;
@@ -871,19 +888,19 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
 #pragma unroll
       for (int u = 0; u < L; ++u) {
         const int jv = jv0 + u;
-        const int cu = u & 1, nu = cu ^ 1;  // L is even: parity of jv - jb
-        // ---- prefetch: velocity inputs of row jv+1, stress coefficients of row jv+1-M
-        const int jn = jv + 1;
+        const int cu = u % L, nu = (u + PD) % L;  // this iteration's slot, the prefetched one
+        // ---- prefetch: velocity inputs of row jv+PD, stress coefficients of row jv+PD-M
+        const int jn = jv + PD;
         htxx[nu] = hal(F_TXX, jn, rxx);
         htxz[nu] = hal(F_TXZ, jn, rxz);
-        rtxx[(u + 1) % L] = fld(F_TXX, jn, rxx);
-        rtxz[(u + M) % L] = fld(F_TXZ, jv + M, rxz);
-        rtzz[(u + M + 1) % L] = fld(F_TZZ, jv + M + 1, rzz);
+        rtxx[(u + PD) % L] = fld(F_TXX, jn, rxx);
+        rtxz[(u + M + PD - 1) % L] = fld(F_TXZ, jv + M + PD - 1, rxz);
+        rtzz[(u + M + PD) % L] = fld(F_TZZ, jv + M + PD, rzz);
         cvx[nu] = fld(F_VX, jn, rvx);
         cvz[nu] = fld(F_VZ, jn, rvz);
         if constexpr (CMP) {
-          crho[(u + 2) % L] = cof(QRHO, jv + 2, 1.f);
-          cmux[(u - M + 2 + L) % L] = cof(QMUX, jv - M + 2, 1.f);
+          crho[(u + 1 + PD) % L] = cof(QRHO, jv + 1 + PD, 1.f);
+          cmux[(u - M + 1 + PD + L) % L] = cof(QMUX, jv - M + 1 + PD, 1.f);
         } else {
           cb1[nu] = cof(QB1, jn, r1);
           cb3[nu] = cof(QB3, jn, r3);
@@ -896,7 +913,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
         }
         clam[nu] = cof(QLAM, jn - M, rl);
 
-        // ---- first uses of rows loaded one iteration ago: txx row jv, tzz row jv+M
+        // ---- first uses of rows loaded PD iterations ago: txx row jv, tzz row jv+M
         add_src(rtxx[u % L], jv);
         add_src(rtzz[(u + M) % L], jv + M);
         // ---- new velocity, row jv, all lane columns
