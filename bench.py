@@ -213,7 +213,7 @@ def main():
                     help="seam: the single SEAM-shaped shot (default); batch: 64 batched "
                          "Marmousi-shaped shots; serve: the served request stream")
     ap.add_argument("--shots", type=int, default=64, help="shots of the batch workload")
-    ap.add_argument("--requests", type=int, default=16,
+    ap.add_argument("--requests", type=int, default=40,
                     help="requests of the served sample (default run) or stream (serve)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served sample")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
