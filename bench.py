@@ -332,6 +332,20 @@ def main():
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = int(line.split(":")[1])
 
+    # the served sample runs with the SEAM shot's device objects released and
+    # before the CPU baseline (whose OpenMP threads would share the host cores
+    # with the server's worker threads)
+    del W, dm, ds
+    import gc
+    gc.collect()
+    b.synchronize()
+    served = None
+    if not args.no_serve and args.requests > 0:
+        try:
+            served = bench_serve(args.requests)
+        except Exception as e:  # reported, never fatal
+            served = {"error": str(e)}
+
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
@@ -345,13 +359,6 @@ def main():
         except Exception as e:  # reported, never fatal
             cpu = {"value": None, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
                    "sample": f"failed: {e}"}
-
-    served = None
-    if not args.no_serve and args.requests > 0:
-        try:
-            served = bench_serve(args.requests)
-        except Exception as e:  # reported, never fatal
-            served = {"error": str(e)}
 
     out = {
         "metric": METRIC,

@@ -87,9 +87,13 @@ class MI355XBackend:
 
     def set_tiling(self, cols_per_lane: int = 1, rows_per_tile: int = 0,
                    steps_per_graph: int = 16) -> None:
-        """Stencil tile shape (rows_per_tile 0 = automatic) and steps per captured graph (even)."""
+        """Stencil tile shape (rows_per_tile 0 = automatic) and steps per captured graph (even).
+        Batches and pools take the shape current when they are built."""
         self.check(self.lib.fomo_ctx_set_tiling(self.ctx, cols_per_lane, rows_per_tile,
                                                 steps_per_graph), "set_tiling")
+        self.tiling = (int(cols_per_lane), int(rows_per_tile), int(steps_per_graph))
+
+    tiling = (1, 0, 16)  # the library's default shape
 
     def set_step_mode(self, mode: int = 1) -> None:
         """1 = fused single-pass step on ping-pong buffers (default), 0 = two-pass in place."""

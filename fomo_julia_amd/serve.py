@@ -162,6 +162,32 @@ def prepare(req: ShotRequest) -> Prepared:
 
 
 # ---------------------------------------------------------------- pools
+# Tile shape (columns per lane, rows per tile) of the served pools and of the
+# serial baseline, per FD order.  One column per lane: two columns per lane
+# needs more registers than fd_order 12 leaves (3x slower on the 64-shot
+# Marmousi batch), and the served grids are small enough that one column per
+# lane still fills the GPU.
+SERVE_TILING = {4: (1, 0), 8: (1, 0), 12: (1, 0)}
+
+
+def _tiling_for(fd_order: int) -> tuple:
+    return SERVE_TILING.get(int(fd_order), (1, 0))
+
+
+class _Tiling:
+    """Sets the backend's tile shape for a block and restores it after."""
+
+    def __init__(self, b: E.MI355XBackend, fd_order: int):
+        self.b, self.shape = b, _tiling_for(fd_order)
+
+    def __enter__(self):
+        self.saved = self.b.tiling
+        self.b.set_tiling(self.shape[0], self.shape[1], self.saved[2])
+
+    def __exit__(self, *exc):
+        self.b.set_tiling(*self.saved)
+
+
 class Pool:
     """fomo_pool: continuous batch of one FD order and storage type."""
 
@@ -169,8 +195,9 @@ class Pool:
         self.b = b
         self.key = (fd_order, storage)
         h = ctypes.c_void_p()
-        b.check(b.lib.fomo_pool_create(b.ctx, fd_order, _lib.STORAGE[storage], capacity,
-                                       ctypes.byref(h)), "fomo_pool_create")
+        with _Tiling(b, fd_order):  # the pool keeps the shape current at its creation
+            b.check(b.lib.fomo_pool_create(b.ctx, fd_order, _lib.STORAGE[storage], capacity,
+                                           ctypes.byref(h)), "fomo_pool_create")
         self.h = h
         self.capacity = capacity
         self.members: dict = {}  # rid -> _Member
@@ -358,7 +385,8 @@ def serial(requests: list, b: Optional[E.MI355XBackend] = None, storage: Optiona
         ta = time.perf_counter() - t0
         prep = prepare(rr)
         W = E.Wavefield(prep.M.nx, prep.M.nz, b)
-        res = E.run_shot(b, W, prep.M, prep.H, prep.a, prep.src, prep.rec, prep.p)
+        with _Tiling(b, rr.fd_order):
+            res = E.run_shot(b, W, prep.M, prep.H, prep.a, prep.src, prep.rec, prep.p)
         out.append(Completed(r.rid, ta, ta, time.perf_counter() - t0, res.gather, rr.storage,
                              rr.fd_order, prep.M.nx * prep.M.nz * prep.p.nt))
         b._models.clear()

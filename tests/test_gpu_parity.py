@@ -273,9 +273,11 @@ def test_stored_and_derived_coefficients_bit_exact(b, fs, src_type):
 
 def test_pool_mixing_stored_and_derived_coefficients(b):
     """A pool that starts with derived-coefficient members and then admits a
-    member with stored arrays recaptures its step and stays bit-exact."""
+    member with stored arrays and a wider HABC strip (more LDS rows)
+    recaptures its step and stays bit-exact."""
     import ctypes
-    cases = [Case(nx_in=70, nz_in=50, nt=48, seed=1), _perturbed_case(nx_in=64, nz_in=58, nt=48, seed=2)]
+    cases = [Case(nx_in=70, nz_in=50, nt=48, seed=1),
+             _perturbed_case(nx_in=64, nz_in=58, nt=48, seed=2, nbc=20)]
     print("case pool mixed coefficient forms", flush=True)
     b.set_tiling(1, 0, 16)
     b.set_step_mode(1)
