@@ -1389,6 +1389,52 @@ __device__ __forceinline__ void lds_out_t(const float* s, int sc, typename Store
   }
 }
 
+// Interleaved staging for the bottom-right corner: cell (r, c) of rows
+// [r0, r0+nr) x cols [c0, c0+nc) -> s[2*((r+1)*HS + c + 1)] (field) and the
+// float after it (backup).  Row -1 and column -1 are padding: only lanes whose
+// result is discarded read them, and the first float is never read.  Every
+// operand of a corner cell is then a fixed offset from one per-lane index.
+template <int K>
+__device__ __forceinline__ void lds_in2_fo(float* s, const typename Store<K>::T* g,
+                                           const typename Store<K>::T* go, float rs, int P, int r0,
+                                           int nr, int c0, int nc, int lane) {
+  constexpr int STAGE_CHUNK = stage_chunk<K>();
+  const bool on = lane < nc;
+  for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
+    float v[STAGE_CHUNK], w[STAGE_CHUNK];
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r) {
+      const bool ok = on && rb + r < nr;
+      const ptrdiff_t o = (ptrdiff_t)(r0 + rb + r) * P + c0 + lane;
+      v[r] = ok ? ld1<K>(g + o, rs) : 0.f;
+      w[r] = ok ? ld1<K>(go + o, rs) : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r)
+      if (on && rb + r < nr) {
+        const int i = 2 * ((rb + r + 1) * HS + lane + 1);
+        s[i] = v[r];
+        s[i + 1] = w[r];
+      }
+  }
+}
+// fields of the interleaved block -> rows [r0, r0+nr) x cols [c0, c0+nc) of g
+template <int K>
+__device__ __forceinline__ void lds_out_fo(const float* s, typename Store<K>::T* g, float ws, int P,
+                                           int r0, int nr, int c0, int nc, int lane) {
+  constexpr int STAGE_CHUNK = stage_chunk<K>();
+  const bool on = lane < nc;
+  for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
+    float v[STAGE_CHUNK];
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r)
+      v[r] = (on && rb + r < nr) ? s[2 * ((rb + r + 1) * HS + lane + 1)] : 0.f;
+#pragma unroll
+    for (int r = 0; r < STAGE_CHUNK; ++r)
+      if (on && rb + r < nr) st1<K>(g + (ptrdiff_t)(r0 + rb + r) * P + c0 + lane, v[r], ws);
+  }
+}
+
 // Items per field of the HABC partition and the first side-edge row.
 __device__ __forceinline__ int habc_side_r0(const ShotDev& sd) { return sd.fs ? sd.nbc : sd.nbc + 2; }
 
@@ -1396,7 +1442,7 @@ template <int K>
 __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress, int rows) {
   constexpr int KF = Mode<K>::F;
   typedef typename Store<KF>::T T;
-  extern __shared__ float s_dyn[];  // 2 * rows * HS floats
+  extern __shared__ float s_dyn[];  // 2 * (rows + 1) * HS floats
   float* const s_f = s_dyn;
   float* const s_o = s_dyn + rows * HS;
   const int hdummy = rows * HS - 1;
@@ -1562,7 +1608,11 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress, int rows)
     const bool leftc = (pc % 2) == 0;
     const int r0 = bottom ? nz - n - 2 : 1;
     const int c0 = leftc ? 1 : nx - n - 2;
-    lds_in2<KF>(s_f, s_o, hf, hfo, hrs, P, r0, n + 1, c0, n + 1, lane);
+    const bool br = bottom && !leftc;  // staged interleaved (lds_in2_fo)
+    if (br)
+      lds_in2_fo<KF>(s_dyn, hf, hfo, hrs, P, r0, n + 1, c0, n + 1, lane);
+    else
+      lds_in2<KF>(s_f, s_o, hf, hfo, hrs, P, r0, n + 1, c0, n + 1, lane);
     __syncthreads();
     const bool act = lane <= n;
     if (bottom && leftc) {
@@ -1623,21 +1673,22 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress, int rows)
       // nz-n-2 (recurrence along +x), block column 0 the bottom-edge column
       // nx-n-2 (recurrence along +z), the rest the right-bottom corner.
       const int L = lane;
-      float last = act ? s_f[L] : 0.f;  // lane 0 starts with cell (0,0), which is not updated
+      // cell (0, L); lane 0 starts with cell (0,0), which is not updated
+      float last = s_dyn[2 * (HS + L + 1)];
       // Operands of the cell this lane handles at step d, row rc = d - L
-      // clamped to the block (out-of-block steps read a valid cell and store
-      // nothing).  In this corner the edge distance of cell (c0+L, r0+rc) is
-      // min(n+1-L, n+1-rc), so the weight index needs no grid scalars.
-      const int wl = act ? n + 1 - L : 0;
-      const int lm = L > 0 ? 1 : 0;
+      // clamped to the block (out-of-block steps store nothing), all at fixed
+      // offsets from the interleaved index of cell (rc-1, L-1).  In this
+      // corner the edge distance of cell (c0+L, r0+rc) is n+1 - max(L, rc);
+      // idle lanes use L = n+1 (weight 0's slot).
+      const int Lw = act ? L : n + 1;
       auto fetch = [&](int d, float& fv, float& ov, float& ol, float& ou, float& w) {
         const int rc = min(max(d - L, 0), n);
-        const int o = rc * HS + L;
-        fv = s_f[o];
-        ov = s_o[o];
-        ol = s_o[o - lm];
-        ou = s_o[rc > 0 ? o - HS : o];
-        w = s_w[min(wl, n + 1 - rc)];
+        const float* b = s_dyn + 2 * (rc * HS + L);
+        ou = b[3];           // backup of (rc-1, L)
+        ol = b[2 * HS + 1];  // backup of (rc, L-1)
+        fv = b[2 * HS + 2];  // field of (rc, L)
+        ov = b[2 * HS + 3];  // backup of (rc, L)
+        w = s_w[n + 1 - max(Lw, rc)];
       };
       float pf[HPF], po[HPF], pl[HPF], pu[HPF], pw[HPF];
 #pragma unroll
@@ -1662,7 +1713,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress, int rows)
             const float cw = (L == 0 || r == 0) ? (1.0f - w) : (1.0f - w) * 0.5f;
             const float out = fmaf_(cw, sx + sz, w * fv);
             const bool valid = act && (unsigned)r <= (unsigned)n && !(L == 0 && r == 0);
-            s_f[valid ? r * HS + L : hdummy] = out;
+            s_dyn[valid ? 2 * ((r + 1) * HS + L + 1) : 0] = out;
             last = valid ? out : last;
           }
         }
@@ -1741,7 +1792,10 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress, int rows)
       }
     }
     __syncthreads();
-    lds_out<KF>(s_f, 0, 0, hf, hws, P, r0, n + 1, c0, n + 1, lane);
+    if (br)
+      lds_out_fo<KF>(s_dyn, hf, hws, P, r0, n + 1, c0, n + 1, lane);
+    else
+      lds_out<KF>(s_f, 0, 0, hf, hws, P, r0, n + 1, c0, n + 1, lane);
   }
 }
 
