@@ -48,7 +48,9 @@ enum Part {
 // HABC.  Bands (0-based, half-open): left columns [0, b), right columns
 // [nx-b, nx), bottom rows [nz-b, nz), top rows [0, b) without a free surface,
 // with b = nbc + M + 1.
-constexpr int BAND_ROWS = 16;  // rows per band-kernel item
+// rows per band-kernel item: few for a single shot (more, shorter items
+// spread its thin bands over the GPU), more for batches (less halo overhead)
+constexpr int BAND_ROWS_SINGLE = 4, BAND_ROWS_BATCH = 8;
 
 // Everything a kernel needs to know about one shot.  Lives in device memory
 // (a table indexed by shot), so a captured graph stays valid when the host
@@ -61,6 +63,7 @@ struct ShotDev {
   int ftiles_x, ftiles_z;
   int band;             // band width b = nbc + M + 1
   int nb_side, nb_mid;  // band items: per side band, per bottom/top band
+  int band_rows;        // rows per band item
   int n_l, n_r, n_b, n_t;  // HABC side/bottom/top edge pieces per field
   float dtx, dtz;
   // Reduced-precision storage: a stored value is x * 2^e (exact), read back as

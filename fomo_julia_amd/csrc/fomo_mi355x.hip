@@ -919,7 +919,7 @@ extern "C" int32_t fomo_shot_gather(fomo_shot* s, float* out) {
 
 // ------------------------------------------------------------------ batches
 static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const fomo_shot* s,
-                            int cpt, int tz, int table) {
+                            int cpt, int tz, int table, int band_rows) {
   ShotDev sd;
   std::memset(&sd, 0, sizeof(sd));
   const fomo_model_desc& d = m->d;
@@ -964,8 +964,9 @@ static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const 
   sd.ftiles_x = (d.nx + sd.ftw - 1) / sd.ftw;
   sd.ftiles_z = (d.nz + tz - 1) / tz;
   sd.band = d.nbc + M + 1;
-  sd.nb_side = ((sd.band + 63) / 64) * ((d.nz + BAND_ROWS - 1) / BAND_ROWS);
-  sd.nb_mid = ((d.nx - 2 * sd.band + 63) / 64) * ((sd.band + BAND_ROWS - 1) / BAND_ROWS);
+  sd.band_rows = band_rows;
+  sd.nb_side = ((sd.band + 63) / 64) * ((d.nz + band_rows - 1) / band_rows);
+  sd.nb_mid = ((d.nx - 2 * sd.band + 63) / 64) * ((sd.band + band_rows - 1) / band_rows);
   sd.bvx_x = m->coef[0];
   sd.bvx_z = m->coef[1];
   sd.bvz_x = m->coef[2];
@@ -1180,7 +1181,8 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
   b->hs.resize((size_t)TBL_COUNT * n);
   for (int t = 0; t < TBL_COUNT; ++t)
     for (int i = 0; i < n; ++i)
-      b->hs[(size_t)t * n + i] = make_shotdev(ms[i], ws[i], ss[i], b->cpt, b->tz, t);
+      b->hs[(size_t)t * n + i] = make_shotdev(ms[i], ws[i], ss[i], b->cpt, b->tz, t,
+                                              n == 1 ? BAND_ROWS_SINGLE : BAND_ROWS_BATCH);
   b->hprefix.assign((size_t)PART_COUNT * (n + 1), 0);
   for (int p = 0; p < PART_COUNT; ++p) {
     int acc = 0;
@@ -1944,7 +1946,8 @@ static int pool_upload(fomo_pool* p) {
     for (int i = 0; i < n; ++i) {
       // table t of the member: reads buffer c0 ^ t, writes the other
       const int table = TBL_FUSED0 + (t ^ p->c0[i]);
-      p->h_rows[(size_t)t * C + i] = make_shotdev(p->ms[i], p->ws[i], p->ss[i], b->cpt, b->tz, table);
+      p->h_rows[(size_t)t * C + i] =
+          make_shotdev(p->ms[i], p->ws[i], p->ss[i], b->cpt, b->tz, table, BAND_ROWS_BATCH);
     }
   for (int q = 0; q < PART_COUNT; ++q) {
     int acc = 0;

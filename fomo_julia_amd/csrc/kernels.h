@@ -1060,7 +1060,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
 
 // Stress in the bands next to the HABC strips, after the velocity HABC:
 // new velocities from B (f[]), old stress from A (fo[]), output to B.  One
-// wavefront per (band, 64-column chunk, BAND_ROWS-row segment); the same
+// wavefront per (band, 64-column chunk, band_rows-row segment); the same
 // arithmetic as k_stress.
 template <int M, int WPB, int K>
 __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
@@ -1084,6 +1084,7 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
     int t = item - ba.prefix[s];
     const int nx = uniform(sd.nx), nz = uniform(sd.nz), P = uniform(sd.pitch);
     const int band = uniform(sd.band), nb_side = uniform(sd.nb_side), nb_mid = uniform(sd.nb_mid);
+    const int brows = uniform(sd.band_rows);
     // band rectangle [c0, c1) x [r0, r1) of this item
     int c0, c1, r0, r1;
     {
@@ -1095,8 +1096,8 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
         const int cc = u % side_chunks, rs = u / side_chunks;
         c0 = (right ? nx - band : 0) + cc * 64;
         c1 = min(c0 + 64, right ? nx : band);
-        r0 = rs * BAND_ROWS;
-        r1 = min(r0 + BAND_ROWS, nz);
+        r0 = rs * brows;
+        r1 = min(r0 + brows, nz);
       } else {
         t -= 2 * nb_side;
         const bool top = t >= nb_mid;
@@ -1104,8 +1105,8 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
         const int cc = u % mid_chunks, rs = u / mid_chunks;
         c0 = band + cc * 64;
         c1 = min(c0 + 64, nx - band);
-        r0 = (top ? 0 : nz - band) + rs * BAND_ROWS;
-        r1 = min(r0 + BAND_ROWS, top ? band : nz);
+        r0 = (top ? 0 : nz - band) + rs * brows;
+        r1 = min(r0 + brows, top ? band : nz);
       }
     }
     const int ib = c0 + lane;
