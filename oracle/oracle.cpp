@@ -1,7 +1,7 @@
 // ORACLE -- test infrastructure only.
 //
 // A CPU restatement of the reference's time-stepping path, used by tests/,
-// __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.  The
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg, to check results against.  The
 // product (fomo_julia_amd/) never links, loads or calls it.
 //
 // Arrays use the reference layout: (nx, nz) column-major, element (i, j)

@@ -2,7 +2,7 @@
 
 FP32 results are compared bit for bit with the oracle's FMA mode (the kernels
 and the oracle share one arithmetic contract); the oracle's unfused mode and
-its Float64 run are compared by relative L2 (north-star tolerance 1e-5 for
+its Float64 run are compared by relative L2 (tolerance 1e-5 for
 FP32 gathers)."""
 import os
 

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/oracle.cpp) before results are checked against it.
 
 - the reference's own output gather shot_1.bin (run.jl example);
 - stencils: exact derivatives of polynomials (offsets, signs, coefficients);
