@@ -231,6 +231,21 @@ __device__ __forceinline__ float ld1(const typename Store<K>::T* p, float s) {
   if constexpr (K == ST_F32) return ldg(p);
   else return from_bits<K>(*(const gu16*)p) * s;
 }
+// The stored bits of one element (16-bit kinds zero-extended) and their value:
+// staging loops load bits unconditionally and decode afterwards, so that all
+// loads of a chunk are in flight before the first wait (a conditional load
+// whose value is decoded in the same branch makes the compiler wait on it
+// right there).
+template <int K>
+__device__ __forceinline__ uint32_t ld_bits(const typename Store<K>::T* p) {
+  if constexpr (K == ST_F32) return __float_as_uint(ldg(p));
+  else return (uint32_t)*(const gu16*)p;
+}
+template <int K>
+__device__ __forceinline__ float from_stored(uint32_t b, float s) {
+  if constexpr (K == ST_F32) return __uint_as_float(b);
+  else return from_bits<K>(b) * s;
+}
 template <int K>
 __device__ __forceinline__ void st1(typename Store<K>::T* p, float x, float s) {
   if constexpr (K == ST_F32) stg(p, x);
@@ -1313,20 +1328,20 @@ __device__ __forceinline__ void lds_in2(float* s, float* so, const typename Stor
                                         int nr, int c0, int nc, int lane) {
   constexpr int STAGE_CHUNK = stage_chunk<K>();
   const bool on = lane < nc;
+  const int cl = min(lane, nc - 1);  // lanes past the block load a valid cell and store nothing
   for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
-    float v[STAGE_CHUNK], w[STAGE_CHUNK];
+    uint32_t v[STAGE_CHUNK], w[STAGE_CHUNK];
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r) {
-      const bool ok = on && rb + r < nr;
-      const ptrdiff_t o = (ptrdiff_t)(r0 + rb + r) * P + c0 + lane;
-      v[r] = ok ? ld1<K>(g + o, rs) : 0.f;
-      w[r] = ok ? ld1<K>(go + o, rs) : 0.f;
+      const ptrdiff_t o = (ptrdiff_t)(r0 + min(rb + r, nr - 1)) * P + c0 + cl;
+      v[r] = ld_bits<K>(g + o);
+      w[r] = ld_bits<K>(go + o);
     }
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r)
       if (on && rb + r < nr) {
-        s[(rb + r) * HS + lane] = v[r];
-        so[(rb + r) * HS + lane] = w[r];
+        s[(rb + r) * HS + lane] = from_stored<K>(v[r], rs);
+        so[(rb + r) * HS + lane] = from_stored<K>(w[r], rs);
       }
   }
 }
@@ -1356,20 +1371,20 @@ __device__ __forceinline__ void lds_in2_t(float* s, float* so, const typename St
                                           int nr, int c0, int nc, int lane) {
   constexpr int STAGE_CHUNK = stage_chunk<K>();
   const bool on = lane < nc;
+  const int cl = min(lane, nc - 1);
   for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
-    float v[STAGE_CHUNK], w[STAGE_CHUNK];
+    uint32_t v[STAGE_CHUNK], w[STAGE_CHUNK];
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r) {
-      const bool ok = on && rb + r < nr;
-      const ptrdiff_t o = (ptrdiff_t)(r0 + rb + r) * P + c0 + lane;
-      v[r] = ok ? ld1<K>(g + o, rs) : 0.f;
-      w[r] = ok ? ld1<K>(go + o, rs) : 0.f;
+      const ptrdiff_t o = (ptrdiff_t)(r0 + min(rb + r, nr - 1)) * P + c0 + cl;
+      v[r] = ld_bits<K>(g + o);
+      w[r] = ld_bits<K>(go + o);
     }
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r)
       if (on && rb + r < nr) {
-        s[lane * HS + rb + r] = v[r];
-        so[lane * HS + rb + r] = w[r];
+        s[lane * HS + rb + r] = from_stored<K>(v[r], rs);
+        so[lane * HS + rb + r] = from_stored<K>(w[r], rs);
       }
   }
 }
@@ -1401,21 +1416,21 @@ __device__ __forceinline__ void lds_in2_fo(float* s, const typename Store<K>::T*
                                            int nr, int c0, int nc, int lane) {
   constexpr int STAGE_CHUNK = stage_chunk<K>();
   const bool on = lane < nc;
+  const int cl = min(lane, nc - 1);
   for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
-    float v[STAGE_CHUNK], w[STAGE_CHUNK];
+    uint32_t v[STAGE_CHUNK], w[STAGE_CHUNK];
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r) {
-      const bool ok = on && rb + r < nr;
-      const ptrdiff_t o = (ptrdiff_t)(r0 + rb + r) * P + c0 + lane;
-      v[r] = ok ? ld1<K>(g + o, rs) : 0.f;
-      w[r] = ok ? ld1<K>(go + o, rs) : 0.f;
+      const ptrdiff_t o = (ptrdiff_t)(r0 + min(rb + r, nr - 1)) * P + c0 + cl;
+      v[r] = ld_bits<K>(g + o);
+      w[r] = ld_bits<K>(go + o);
     }
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r)
       if (on && rb + r < nr) {
         const int i = 2 * ((rb + r + 1) * HS + lane + 1);
-        s[i] = v[r];
-        s[i + 1] = w[r];
+        s[i] = from_stored<K>(v[r], rs);
+        s[i + 1] = from_stored<K>(w[r], rs);
       }
   }
 }
