@@ -65,7 +65,7 @@ def build_seam(nt: int, nbc=50, fd_order=8, with_weights=False, storage="f32"):
                    I.coord_to_index(F32(0.0), M.dz, M.pad), wav, "pressure")
     nx_in = vp.shape[0]
     rec = E.setup_receivers(np.arange(nx_in, dtype=F32) * dx, np.zeros(nx_in, F32), M, type="vz")
-    return dict(M=M, H=H, p=p, a=a, src=src, rec=rec, vp=vp)
+    return dict(M=M, H=H, p=p, a=a, src=src, rec=rec, vp=vp, velocity=(vp, vs, rho))
 
 
 def model_blocks(b, dm, M, H) -> int:
@@ -291,8 +291,19 @@ def main():
     gpu_ms = ev0.elapsed_time(ev1)
     ms_per_step = gpu_ms / max(args.steps, 1)
     value = cells * args.steps / (gpu_ms / 1e3) / 1e6
-    gather = ds.gather()
+    t_g = time.perf_counter()
+    gather = ds.gather()  # device -> host, [nt x n_rec]
+    gather_s = time.perf_counter() - t_g
     finite = bool(np.isfinite(gather).all())
+    # the boundary's host hand-over on either side of the timed region: the
+    # medium as three unpadded arrays (device-side setup), the gather back
+    t_u = time.perf_counter()
+    dm_up = E._DeviceModel(b, E.init_medium_device(*case["velocity"], M.dx, M.dz, H.nbc,
+                                                   p.fd_order, None, storage=args.storage),
+                           H, case["a"], p)
+    b.synchronize()
+    upload_s = time.perf_counter() - t_u
+    del dm_up
 
     # per-kernel mean durations (HIP events on the library's stream)
     def stage_ms(which: int, reps: int) -> float:
@@ -382,7 +393,12 @@ def main():
                    "padded_cells": cells, "physical_cells": 4444 * 3819,
                    "shot_seconds": round(gpu_ms / 1e3 * 11520 / max(args.steps, 1), 3),
                    "wall_seconds_timed": round(wall, 3), "setup_seconds": round(t_setup, 2),
-                   "gather_finite": finite},
+                   "gather_finite": finite,
+                   "pcie_inclusive": {
+                       "model_upload_and_device_setup_s": round(upload_s, 4),
+                       "gather_download_s": round(gather_s, 4),
+                       "shot_seconds": round(gpu_ms / 1e3 * 11520 / max(args.steps, 1)
+                                             + upload_s + gather_s, 3)}},
         "roofline": {"bound": "hbm", "achieved": round(main_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(main_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": main_name, "kernel_ms": round(main_ms, 5),
