@@ -905,15 +905,27 @@ extern "C" int32_t fomo_shot_reset(fomo_shot* s) {
 extern "C" int32_t fomo_shot_gather(fomo_shot* s, float* out) {
   if (!s || !out) return fail(s ? s->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   fomo_ctx* c = s->m->ctx;
-  if (s->n_rec == 0) return FOMO_OK;
-  std::vector<float> tmp((size_t)s->nt * s->n_rec);
-  HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpyAsync(tmp.data(), s->dgather, tmp.size() * sizeof(float),
-                            hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  // device [k][r]  ->  Julia [nt x n_rec] column-major: out[k + r*nt]
-  for (int k = 0; k < s->nt; ++k)
-    for (int r = 0; r < s->n_rec; ++r) out[(size_t)k + (size_t)r * s->nt] = tmp[(size_t)k * s->n_rec + r];
+  if (s->n_rec == 0 || s->nt == 0) return FOMO_OK;
+  // device [k][r] -> Julia [nt x n_rec] column-major (out[k + r*nt]), transposed
+  // on the device into a scratch buffer, then one copy
+  const size_t n = (size_t)s->nt * s->n_rec;
+  float* scratch = nullptr;
+  {
+    API_LOCK;  // allocations must not interleave with a graph capture on another thread
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMalloc(&scratch, n * sizeof(float)));
+  }
+  const dim3 g((unsigned)((s->n_rec + 63) / 64), (unsigned)((s->nt + 63) / 64));
+  hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, c->stream, s->dgather, scratch, s->nt, s->n_rec);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, scratch, n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  {
+    API_LOCK;
+    (void)hipFree(scratch);
+  }
+  if (e != hipSuccess) return fail(c, FOMO_ERR_HIP, std::string("shot_gather: ") + hipGetErrorString(e));
   return FOMO_OK;
 }
 

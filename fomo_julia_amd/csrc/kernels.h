@@ -1961,6 +1961,24 @@ __global__ void k_inject(BatchArgs ba) {
 }
 
 // One step counter set on the stream (a pool member joining between runs).
+// Gather layout change for the download: device [nt][n_rec] -> Julia's
+// [nt x n_rec] column-major (dst[k + r*nt] = src[k*n_rec + r]), through 64 x 64
+// LDS tiles so that reads and writes are both row-contiguous.
+__global__ __launch_bounds__(256) void k_transpose(const float* src, float* dst, int nt, int nrec) {
+  __shared__ float t[64][65];
+  const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4 threads
+  for (int i = ty; i < 64; i += 4) {
+    const int k = k0 + i, r = r0 + tx;
+    if (k < nt && r < nrec) t[i][tx] = src[(size_t)k * nrec + r];
+  }
+  __syncthreads();
+  for (int i = ty; i < 64; i += 4) {
+    const int r = r0 + i, k = k0 + tx;
+    if (k < nt && r < nrec) dst[(size_t)r * nt + k] = t[tx][i];
+  }
+}
+
 __global__ void k_set_int(int* p, int v) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
