@@ -206,3 +206,31 @@ def test_recorder_close_writes_frames(b, tmp_path):
     import json
     meta = json.loads((tmp_path / "shot_p.json").read_text())
     assert meta["steps"] == [4, 8, 12, 16, 20] and meta["frames"] == 5
+
+
+@gpu
+def test_run_shots_with_recorder_keeps_every_shot(b):
+    """run_shots(...; on_step=recorder) as run.jl calls it: one video over all
+    shots (frames of shot 1, then shot 2), gathers equal to plain runs."""
+    c = Case(nx_in=60, nz_in=44, nt=24, fd_order=8)
+    print("case frames run_shots", flush=True)
+    cfg = E.MultiShotConfig(np.array([100.0, 400.0], F32), np.array([20.0, 30.0], F32),
+                            c.src.wavelet)
+    rec = V.MultiFieldRecorder(c.M.nx, c.M.nz, c.dt, V.VideoConfig(fields=["p"], skip=6,
+                                                                   downsample=2))
+    W = E.Wavefield(c.M.nx, c.M.nz, b)
+    seen = []
+    res = E.run_shots(b, W, c.M, c.H, c.a, c.rec, cfg, c.p, on_step=rec,
+                      on_shot_complete=lambda r: seen.append(r.shot_id))
+    assert seen == [1, 2]
+    assert rec.recorders["p"].steps == [6, 12, 18, 24] * 2
+    plain = E.run_shots(b, E.Wavefield(c.M.nx, c.M.nz, b), c.M, c.H, c.a, c.rec, cfg, c.p, batch=1)
+    for r0, r1 in zip(res, plain):
+        assert np.array_equal(r0.gather, r1.gather)
+    # each shot's frames equal a recorder run on that shot alone
+    for q, shot in enumerate(cfg.shots):
+        src = E._create_source(c.M, shot, cfg.wavelet, cfg.source_type)
+        solo = V.MultiFieldRecorder(c.M.nx, c.M.nz, c.dt, V.VideoConfig(fields=["p"], skip=6,
+                                                                        downsample=2))
+        E.run_shot(b, E.Wavefield(c.M.nx, c.M.nz, b), c.M, c.H, c.a, src, c.rec, c.p, on_step=solo)
+        assert np.array_equal(rec.frames("p")[4 * q:4 * q + 4], solo.frames("p")), q
