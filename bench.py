@@ -197,6 +197,56 @@ def bench_serve(n_req: int, seed: int = 5, load: float = 0.8) -> dict:
                              for k, v in srv.stats.items()}}
 
 
+def pattern_probe(reps: int = 20) -> dict:
+    """Bandwidth of the fused step's access pattern with no stencil arithmetic
+    (csrc/pattern_probe.hip) and of plain streaming, on a 4608 x 3920 grid of
+    pitched rows (the SEAM shot's padded 4552 x 3927 rounded to whole tiles).
+    nread = 8 is the fused step's FP32 stream set (5 fields + rho_vx, lam,
+    mu_txx), 12 the set without derived coefficients; 5 fields are written.
+    An LDS reservation of 48 KiB per 4-wave block caps residency at 3 blocks
+    per CU = 3 waves per SIMD, the fused step's own occupancy (150 VGPRs)."""
+    import torch
+    from fomo_julia_amd import build as B
+    if not os.path.exists(B.PROBE_PATH):
+        raise RuntimeError(f"{B.PROBE_PATH} is not built (python -m fomo_julia_amd.build)")
+    lib = ctypes.CDLL(B.PROBE_PATH)
+    lib.fomo_probe_pattern.argtypes = [ctypes.c_int] * 9 + [ctypes.POINTER(ctypes.c_double)]
+    lib.fomo_probe_stream.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double)]
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    us = ctypes.c_double()
+    out = {"cus": cus, "stream": [], "pattern": []}
+    nbytes = 1 << 30
+    for mode, name, moved in ((0, "copy", 2), (1, "read", 1), (2, "write", 1)):
+        for grid in (cus * 8, cus * 32):
+            rc = lib.fomo_probe_stream(mode, nbytes, grid, reps, ctypes.byref(us))
+            if rc:
+                raise RuntimeError(f"fomo_probe_stream({name}, grid {grid}) returned {rc}")
+            gbs = moved * nbytes / (us.value * 1e3)
+            out["stream"].append({"kind": name, "grid": grid, "us": round(us.value, 1),
+                                  "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 3)})
+            print(json.dumps(out["stream"][-1]), file=sys.stderr, flush=True)
+    nx, nz, tz = 4608, 3920, 40
+    for nread in (8, 12):
+        for cpt in (1, 2, 4):
+            for lds, grid in ((0, cus * 8), (0, cus * 16), (49152, cus * 3), (49152, cus * 12)):
+                rc = lib.fomo_probe_pattern(nread, 5, cpt, nx, nz, tz, grid, lds, reps, ctypes.byref(us))
+                if rc:
+                    raise RuntimeError(f"fomo_probe_pattern({nread}, 5, {cpt}, grid {grid}, "
+                                       f"lds {lds}) returned {rc}")
+                gbs = (nread + 5) * 4 * nx * nz / (us.value * 1e3)
+                out["pattern"].append({"nread": nread, "nwrite": 5, "cols_per_lane": cpt, "grid": grid,
+                                       "lds": lds, "us": round(us.value, 1), "GB/s": round(gbs, 1),
+                                       "frac": round(gbs / HBM_PEAK_GBS, 3)})
+                print(json.dumps(out["pattern"][-1]), file=sys.stderr, flush=True)
+    for key in ("stream", "pattern"):
+        best = max(out[key], key=lambda r: r["GB/s"])
+        out[f"best_{key}"] = best
+    capped = [r for r in out["pattern"] if r["nread"] == 8 and r["lds"]]
+    out["best_pattern_8r5w_3_waves_per_simd"] = max(capped, key=lambda r: r["GB/s"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,7 +269,13 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
+    ap.add_argument("--pattern-probe", action="store_true",
+                    help="only measure the bandwidth ceiling of the fused step's access "
+                         "pattern and of plain streaming (libfomo_probe.so)")
     args = ap.parse_args()
+    if args.pattern_probe:
+        print(json.dumps({"probe": "hbm access-pattern ceiling", **pattern_probe()}))
+        return
     if args.copy_calibration:
         import torch
         n = args.copy_calibration << 18  # float32 elements

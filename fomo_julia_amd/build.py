@@ -1,4 +1,5 @@
-"""Builds libfomo_mi355x.so in-tree with hipcc for gfx950.
+"""Builds libfomo_mi355x.so in-tree with hipcc for gfx950, and beside it the
+bandwidth-probe library libfomo_probe.so that bench.py's --pattern-probe loads.
 
     python -m fomo_julia_amd.build
 
@@ -17,6 +18,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_NAME = "libfomo_mi355x.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 SOURCES = [os.path.join(HERE, "csrc", "fomo_mi355x.hip")]
+PROBE_NAME = "libfomo_probe.so"
+PROBE_PATH = os.path.join(HERE, PROBE_NAME)
+PROBE_SOURCES = [os.path.join(HERE, "csrc", "pattern_probe.hip")]
 DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("kernels.h", "fomo_device.h", "setup.h")] + [
     os.path.join(os.path.dirname(HERE), "include", "fomo_mi355x.h")]
 
@@ -28,19 +32,24 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def command(out: str) -> list[str]:
+def command(out: str, sources: list[str] | None = None) -> list[str]:
     return [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-Wno-unused-command-line-argument", "-o", out] + SOURCES
+            "-Wno-unused-command-line-argument", "-o", out] + (SOURCES if sources is None else sources)
 
 
-def build(verbose: bool = False) -> str:
-    tmp = LIB_PATH + ".tmp"
-    cmd = command(tmp)
+def _compile(path: str, sources: list[str], verbose: bool) -> None:
+    tmp = path + ".tmp"
+    cmd = command(tmp, sources)
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB_PATH)
+    os.replace(tmp, path)
+
+
+def build(verbose: bool = False) -> str:
+    _compile(LIB_PATH, SOURCES, verbose)
+    _compile(PROBE_PATH, PROBE_SOURCES, verbose)
     return LIB_PATH
 
 
