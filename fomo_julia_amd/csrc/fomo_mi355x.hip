@@ -1351,7 +1351,8 @@ static void launch_habc(const fomo_batch* b, int table, bool stress, int k_off) 
   const BatchArgs a = args_for(b, table, part, k_off);
   const dim3 g(b->grid_fixed[part] > 0 ? b->grid_fixed[part] : std::max(1, items));
   const int rows = b->habc_rows;
-  const size_t lds = (size_t)2 * (rows + 1) * HS * sizeof(float);  // + the corner's padding row
+  // + the corner's padding row and the bottom-right sweep's read-ahead
+  const size_t lds = ((size_t)2 * (rows + 1) * HS + habc_lds_slack(rows)) * sizeof(float);
   if (b->storage == FOMO_STORAGE_BF16)
     hipLaunchKernelGGL(k_habc<ST_BF16>, g, dim3(64), lds, lstream(b), a, stress ? 1 : 0, rows);
   else if (b->storage == FOMO_STORAGE_F16)

@@ -35,6 +35,11 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ float from_lower_lane(float v) {  // lane i <- lane i-1
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
 }
+// Same shift with the hardware's bound control filling lane 0 with 0 (no
+// old-value operand to initialise).
+__device__ __forceinline__ float from_lower_lane_z(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
+}
 __device__ __forceinline__ float from_upper_lane(float v) {  // lane i <- lane i+1
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
 }
@@ -1309,7 +1314,12 @@ constexpr int HS = 65;  // LDS row stride: odd, so row and column walks are bank
 // The staging arrays hold `rows` = (largest nbc of the launch) + 1 LDS rows
 // of HS floats each (dynamic LDS, so small strips leave room for more blocks
 // per CU).  Column 64 of the last row is never read: recurrence steps of
-// lanes with no cell store there instead of branching.
+// lanes with no cell store there instead of branching.  The bottom-right
+// sweep reads one group beyond its last step: up to float 132*nbc + 155 of the
+// interleaved block, past the 130*(nbc+2) floats of a launch with rows =
+// nbc+1 when nbc > 52, so the launch adds habc_lds_slack(rows) floats.
+__host__ __device__ constexpr int habc_lds_slack(int rows) { return 2 * (rows - 1) - 104 > 0 ? 2 * (rows - 1) - 104 : 0; }
+constexpr int HABC_WT_LEN = 64 + 2 * MAX_NBC + 12;  // weight table (s_w / s_rw in k_habc)
 
 // rows [r0, r0+nr) x cols [c0, c0+nc) of the field g and its backup go -> s, so
 // (s[r*HS + c], so[r*HS + c]), nc <= 64.  Loads of both arrays go out a chunk
@@ -1458,11 +1468,18 @@ template <int K>
 __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress, int rows) {
   constexpr int KF = Mode<K>::F;
   typedef typename Store<KF>::T T;
-  extern __shared__ float s_dyn[];  // 2 * (rows + 1) * HS floats
+  extern __shared__ float s_dyn[];  // 2 * (rows + 1) * HS + habc_lds_slack(rows) floats
   float* const s_f = s_dyn;
   float* const s_o = s_dyn + rows * HS;
   const int hdummy = rows * HS - 1;
-  __shared__ float s_w[MAX_NBC + 2];
+  // Weights of the current piece.  Most pieces: s_w[d] = ramp[d] by edge
+  // distance d in [0, n+1].  The bottom-right corner instead: s_rw[64 + j] =
+  // weight at corner distance j = max(row, column) of its block, j in [0,
+  // n+1]; its sweep also reads [2, 64) and (65 + n, 64 + 2n + 12) and stores
+  // into [0, 2*HPF), only on steps whose result is discarded.
+  __shared__ float s_wt[HABC_WT_LEN];
+  float* const s_w = s_wt;
+  float* const s_rw = s_wt;
   const int lane = threadIdx.x;
   const int nsh = *ba.n_shots_p;
   const int total = ba.prefix[nsh];
@@ -1490,7 +1507,10 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress, int rows)
     const int ra = habc_side_r0(sd), rb = nz - n - 2;  // side-edge rows [ra, rb)
     const int ca = n + 2, cb = nx - n - 2;             // bottom/top-edge columns [ca, cb)
     __syncthreads();  // LDS of the previous item is no longer read
-    if (lane < n + 2) s_w[lane] = ldg(h.ramp + lane);
+    if (lane < n + 2) {
+      if (is_corner && pc == 1) s_rw[64 + lane] = ldg(h.ramp + n + 1 - lane);
+      else s_w[lane] = ldg(h.ramp + lane);
+    }
     const HabcGeo W{nx, nz, n, s_w};
 
     if (!is_corner && p < sd.n_l + sd.n_r) {
@@ -1688,59 +1708,75 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress, int rows)
       // BR: one wavefront over the block; block row 0 is the right-edge row
       // nz-n-2 (recurrence along +x), block column 0 the bottom-edge column
       // nx-n-2 (recurrence along +z), the rest the right-bottom corner.
-      const int L = lane;
-      // cell (0, L); lane 0 starts with cell (0,0), which is not updated
-      float last = s_dyn[2 * (HS + L + 1)];
-      // Operands of the cell this lane handles at step d, row rc = d - L
-      // clamped to the block (out-of-block steps store nothing), all at fixed
-      // offsets from the interleaved index of cell (rc-1, L-1).  In this
-      // corner the edge distance of cell (c0+L, r0+rc) is n+1 - max(L, rc);
-      // idle lanes use L = n+1 (weight 0's slot).
-      const int Lw = act ? L : n + 1;
-      auto fetch = [&](int d, float& fv, float& ov, float& ol, float& ou, float& w) {
-        const int rc = min(max(d - L, 0), n);
-        const float* b = s_dyn + 2 * (rc * HS + L);
-        ou = b[3];           // backup of (rc-1, L)
-        ol = b[2 * HS + 1];  // backup of (rc, L-1)
-        fv = b[2 * HS + 2];  // field of (rc, L)
-        ov = b[2 * HS + 3];  // backup of (rc, L)
-        w = s_w[n + 1 - max(Lw, rc)];
+      // Lane r owns block row r and at step d updates cell (r, c = d - r):
+      // one anti-diagonal per step.  Its left neighbour (r, c-1) is the
+      // lane's own previous result and its upper neighbour (r-1, c) lane
+      // r-1's (DPP).  Every operand of the lane's cells is at a fixed offset
+      // from one per-lane LDS pointer that advances two floats per step, so a
+      // step costs no address arithmetic.  Steps whose cell lies outside the
+      // block (c < 0, c > n, idle lanes) compute a value nothing reads: their
+      // own `last` is next read as a left neighbour only after an edge-column
+      // step has replaced it by the x-term's exact 0, and by lane r+1 only on
+      // its own out-of-block steps; their store goes to the never-read float 0.
+      const int rl = min(lane, n);              // idle lanes shadow row n
+      const int off = act ? lane : (1 << 20);   // c = d - off: never in [0, n] when idle
+      // cell (r, c) is at interleaved index 2*((r+1)*HS + c + 1) (its backup
+      // at +1).  With pbi = 2*(rl*(HS-1) + 1), the step-d cell of this lane is
+      // at pbi + 2d + 2*HS; its left neighbour's backup one float before, and
+      // its upper neighbour's backup at pbi + 2d + 1.
+      const int pbi = 2 * (rl * (HS - 1) + 1);
+      const float* const pb = s_dyn + pbi;
+      const float* const pw = s_rw + 64 - rl;   // pw[d]: weight at distance c, for c >= rl
+      const float wl = s_rw[64 + rl];           // weight of the row's cells with c < rl
+      const bool row0 = lane == 0;
+      float last = s_dyn[2 * (HS + 1)];         // lane 0: cell (0,0), not updated, left of (0,1)
+      struct Op {
+        float fv, ov, ol, ou, w;
       };
-      float pf[HPF], po[HPF], pl[HPF], pu[HPF], pw[HPF];
+      auto fetch = [&](int d) {
+        const float* b = pb + 2 * d;
+        Op o;
+        o.fv = b[2 * HS];
+        o.ov = b[2 * HS + 1];
+        o.ol = b[2 * HS - 1];
+        o.ou = b[1];
+        o.w = d < 2 * rl ? wl : pw[d];
+        return o;
+      };
+      // step d = dg + k of a group whose cells start at s_dyn[qs]: the store
+      // goes to qs + 2k, or into s_rw[0..2*HPF) (read only by discarded steps)
+      auto step = [&](int d, int k, const Op& o, float* qs) {
+        const int c = d - off;
+        const float up = from_lower_lane_z(last);
+        float sx = q.sumx(last, o.ov, o.ol);
+        float sz = q.sumz(up, o.ov, o.ou);
+        // the bottom-edge column has no x-term, the right-edge row no z-term,
+        // and their weight factor is (1-w) instead of (1-w)/2; adding an
+        // exact 0 keeps each result identical to its own formula
+        sx = (c == 0) ? 0.f : sx;
+        sz = row0 ? 0.f : sz;
+        const float om = 1.0f - o.w;
+        const float cw = (c == 0 || row0) ? om : om * 0.5f;
+        const float out = fmaf_(cw, sx + sz, o.w * o.fv);
+        float* const dst = (unsigned)c <= (unsigned)n ? qs : s_rw;
+        dst[2 * k] = out;
+        last = out;
+      };
+      Op a[HPF], bq[HPF];
 #pragma unroll
-      for (int k = 0; k < HPF; ++k) fetch(1 + k, pf[k], po[k], pl[k], pu[k], pw[k]);
-      for (int dg = 1; dg <= 2 * n; dg += HPF) {
-        float nf[HPF], no[HPF], nl[HPF], nu[HPF], nw[HPF];
+      for (int k = 0; k < HPF; ++k) a[k] = fetch(1 + k);
+      // ping-pong over two groups of HPF steps, each group's operands read one
+      // group ahead; steps past 2n fall outside the block for every lane
+      for (int dg = 1; dg <= 2 * n; dg += 2 * HPF) {
+        float* const qa = s_dyn + pbi + 2 * HS + 2 * dg;
 #pragma unroll
-        for (int k = 0; k < HPF; ++k) fetch(dg + HPF + k, nf[k], no[k], nl[k], nu[k], nw[k]);
+        for (int k = 0; k < HPF; ++k) bq[k] = fetch(dg + HPF + k);
 #pragma unroll
-        for (int k = 0; k < HPF; ++k) {
-          const int d = dg + k;
-          if (d <= 2 * n) {
-            const float fv = pf[k], ov = po[k], ol = pl[k], ou = pu[k], w = pw[k];
-            const float left = from_lower_lane(last);
-            const int r = d - L;
-            // One formula for the three cell kinds: the bottom-edge column has
-            // no x-term, the right-edge row no z-term, and their weight factor
-            // is (1-w) instead of (1-w)/2; adding an exact 0 keeps each result
-            // identical to its own formula.
-            const float sx = (L == 0) ? 0.f : q.sumx(left, ov, ol);
-            const float sz = (r == 0) ? 0.f : q.sumz(last, ov, ou);
-            const float cw = (L == 0 || r == 0) ? (1.0f - w) : (1.0f - w) * 0.5f;
-            const float out = fmaf_(cw, sx + sz, w * fv);
-            const bool valid = act && (unsigned)r <= (unsigned)n && !(L == 0 && r == 0);
-            s_dyn[valid ? 2 * ((r + 1) * HS + L + 1) : 0] = out;
-            last = valid ? out : last;
-          }
-        }
+        for (int k = 0; k < HPF; ++k) step(dg + k, k, a[k], qa);
 #pragma unroll
-        for (int k = 0; k < HPF; ++k) {
-          pf[k] = nf[k];
-          po[k] = no[k];
-          pl[k] = nl[k];
-          pu[k] = nu[k];
-          pw[k] = nw[k];
-        }
+        for (int k = 0; k < HPF; ++k) a[k] = fetch(dg + 2 * HPF + k);
+#pragma unroll
+        for (int k = 0; k < HPF; ++k) step(dg + HPF + k, k, bq[k], qa + 2 * HPF);
       }
     } else if (leftc) {
       // TL (no free surface): block rows 0..n = rows 1..n+1, columns 0..n =
