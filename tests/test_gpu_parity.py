@@ -109,6 +109,41 @@ def test_each_stage_bit_exact_on_random_state(b, fs, name, fn):
     assert_same_state(W, r, name)
 
 
+@pytest.mark.parametrize("nbc", [1, 2, 33, 53, 57, 63])
+@pytest.mark.parametrize("fs", [True, False])
+@pytest.mark.parametrize("name", ["habc_v", "habc_s"])
+def test_habc_strip_widths_bit_exact_on_random_state(b, nbc, fs, name):
+    """Every HABC piece at strip widths from 1 to the widest one wavefront
+    spans (63), on a random state.  The bottom-right corner's sweep reads one
+    group of steps past its block, which for nbc > 52 runs into the LDS the
+    launch adds for it; 33 and 57 are odd widths (a partial last group)."""
+    c = Case(nx_in=30, nz_in=26, nbc=nbc, fs=fs, nt=4)
+    print(f"case stage={name} nbc={nbc} fs={fs} grid={c.M.nx}x{c.M.nz}", flush=True)
+    W = E.Wavefield(c.M.nx, c.M.nz, b)
+    E.bind(b, W, c.M, c.H, c.a, c.p)
+    r = c.oracle()
+    rng = np.random.default_rng(nbc)
+    for f in range(10):
+        v = rng.standard_normal((c.M.nx, c.M.nz)).astype(F32)
+        W.upload(f, v)
+        r.field(f)[:] = v
+    dict(STAGES)[name](b, W)
+    b.synchronize()
+    r.run(1, 1, O.STAGES[name])
+    assert_same_state(W, r, name)
+
+
+@pytest.mark.parametrize("nbc", [53, 63])
+def test_wide_strip_full_shot_bit_exact(b, nbc):
+    c = Case(nx_in=40, nz_in=36, nbc=nbc, nt=140)
+    print(f"case full shot nbc={nbc} grid={c.M.nx}x{c.M.nz}", flush=True)
+    res, W = gpu_shot(b, c)
+    r = c.oracle()
+    r.run(1, c.nt)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    assert_same_state(W, r)
+
+
 def test_inject_and_record_stages(b):
     c = Case(nt=12, src_type="pressure")
     W = E.Wavefield(c.M.nx, c.M.nz, b)
