@@ -309,6 +309,20 @@ __device__ __forceinline__ void bstore(Rsrc r, uint32_t vo, uint32_t so, float x
     __builtin_amdgcn_raw_buffer_store_b16((unsigned short)to_bits<K>(x * s), r, vo, so, 0);
 }
 
+// A lane's two adjacent elements at one offset: one 64-bit (FP32) or 32-bit
+// (16-bit kinds) store, the same bits as two bstore calls.
+template <int K>
+__device__ __forceinline__ void bstore2(Rsrc r, uint32_t vo, uint32_t so, float x0, float x1, float s) {
+  if constexpr (K == ST_F32) {
+    u2v w;
+    w.x = __float_as_uint(x0);
+    w.y = __float_as_uint(x1);
+    __builtin_amdgcn_raw_buffer_store_b64(w, r, vo, so, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b32(to_bits<K>(x0 * s) | (to_bits<K>(x1 * s) << 16), r, vo, so, 0);
+  }
+}
+
 // x as the storage holds it (stored with scale s, read back with is = 1/s).
 template <int K>
 __device__ __forceinline__ float as_stored(float x, float s, float is) {
@@ -902,6 +916,14 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
       for (int c = 0; c < CPT; ++c) imu.v[c] = 1.0f / cmux[(L - M) % L].v[c];
     }
     const bool own_lane = lo >= HL && lo < LC - HL;
+    // Two columns per lane: a lane's first column ib is even (tile widths and
+    // HL are even), so with nx even no column mask separates a lane's pair and
+    // each field row goes out as one paired store.  The stress mask is then
+    // widened to even bounds [band_lo, band_hi): the at most one extra column
+    // per side lies in the side bands, which the band kernel rewrites later
+    // in the step (from the HABC'd velocities) before anything reads it.
+    const bool pairs = CPT == 2 && (nx & 1) == 0;
+    const int band_lo = band & ~1, band_hi = (nx - band + 1) & ~1;
 
     const int jend = j1 + M;
     for (int jv0 = jb; jv0 < jend; jv0 += L) {
@@ -995,11 +1017,17 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
         }
         {
           const bool vrow = jv >= j0 && jv < j1;
+          if (CPT == 2 && pairs) {
+            const uint32_t vo = (vrow && own_lane && ib < nx) ? lvo : OOB;
+            bstore2<KF>(rB, vo, frow(F_VX, jv, ib0), nvx.v[0], nvx.v[CPT - 1], wvxs);
+            bstore2<KF>(rB, vo, frow(F_VZ, jv, ib0), nvz.v[0], nvz.v[CPT - 1], wvzs);
+          } else {
 #pragma unroll
-          for (int c = 0; c < CPT; ++c) {
-            const uint32_t vo = (vrow && own_lane && ib + c < nx) ? lvo + (uint32_t)(c * FES) : OOB;
-            bstore<KF>(rB, vo, frow(F_VX, jv, ib0), nvx.v[c], wvxs);
-            bstore<KF>(rB, vo, frow(F_VZ, jv, ib0), nvz.v[c], wvzs);
+            for (int c = 0; c < CPT; ++c) {
+              const uint32_t vo = (vrow && own_lane && ib + c < nx) ? lvo + (uint32_t)(c * FES) : OOB;
+              bstore<KF>(rB, vo, frow(F_VX, jv, ib0), nvx.v[c], wvxs);
+              bstore<KF>(rB, vo, frow(F_VZ, jv, ib0), nvz.v[c], wvzs);
+            }
           }
         }
         qvx[u % L] = nvx;
@@ -1063,13 +1091,20 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
         {
           const bool band_row = j >= nz - band || (!fs && j < band);
           const bool wrow = srow && !band_row;
+          if (CPT == 2 && pairs) {
+            const uint32_t vo = (wrow && own_lane && ib >= band_lo && ib < band_hi) ? lvo : OOB;
+            bstore2<KF>(rB, vo, frow(F_TXX, j, ib0), oxx.v[0], oxx.v[CPT - 1], wxx);
+            bstore2<KF>(rB, vo, frow(F_TZZ, j, ib0), ozz.v[0], ozz.v[CPT - 1], wzz);
+            bstore2<KF>(rB, vo, frow(F_TXZ, j, ib0), oxz.v[0], oxz.v[CPT - 1], wxz);
+          } else {
 #pragma unroll
-          for (int c = 0; c < CPT; ++c) {
-            const int i = ib + c;
-            const uint32_t vo = (wrow && own_lane && i >= band && i < nx - band) ? lvo + (uint32_t)(c * FES) : OOB;
-            bstore<KF>(rB, vo, frow(F_TXX, j, ib0), oxx.v[c], wxx);
-            bstore<KF>(rB, vo, frow(F_TZZ, j, ib0), ozz.v[c], wzz);
-            bstore<KF>(rB, vo, frow(F_TXZ, j, ib0), oxz.v[c], wxz);
+            for (int c = 0; c < CPT; ++c) {
+              const int i = ib + c;
+              const uint32_t vo = (wrow && own_lane && i >= band && i < nx - band) ? lvo + (uint32_t)(c * FES) : OOB;
+              bstore<KF>(rB, vo, frow(F_TXX, j, ib0), oxx.v[c], wxx);
+              bstore<KF>(rB, vo, frow(F_TZZ, j, ib0), ozz.v[c], wzz);
+              bstore<KF>(rB, vo, frow(F_TXZ, j, ib0), oxz.v[c], wxz);
+            }
           }
         }
         wave_sync();

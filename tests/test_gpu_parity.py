@@ -109,6 +109,24 @@ def test_each_stage_bit_exact_on_random_state(b, fs, name, fn):
     assert_same_state(W, r, name)
 
 
+@pytest.mark.parametrize("nx_in", [150, 151])
+@pytest.mark.parametrize("fd_order", [4, 8, 12])
+@pytest.mark.parametrize("nbc", [20, 21])
+def test_column_pair_stores_bit_exact(b, nx_in, fd_order, nbc):
+    """Two columns per lane: even grids store each lane's column pair with one
+    instruction and widen the fused kernel's stress mask to even bounds (the
+    band kernel rewrites the extra column); odd grids keep per-column stores.
+    Both grid parities, both band-width parities (nbc + M + 1), three orders."""
+    c = Case(nx_in=nx_in, nz_in=60, nbc=nbc, fd_order=fd_order, nt=90)
+    print(f"case pairs nx={c.M.nx} band={nbc + fd_order // 2 + 1} order={fd_order}", flush=True)
+    res, W = gpu_shot(b, c, cpt=2, tz=0, gsteps=8)
+    b.set_tiling(1, 0, 16)
+    r = c.oracle()
+    r.run(1, c.nt)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    assert_same_state(W, r)
+
+
 @pytest.mark.parametrize("nbc", [1, 2, 33, 53, 57, 63])
 @pytest.mark.parametrize("fs", [True, False])
 @pytest.mark.parametrize("name", ["habc_v", "habc_s"])
