@@ -1752,7 +1752,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress, int rows)
       // block (c < 0, c > n, idle lanes) compute a value nothing reads: their
       // own `last` is next read as a left neighbour only after an edge-column
       // step has replaced it by the x-term's exact 0, and by lane r+1 only on
-      // its own out-of-block steps; their store goes to the never-read float 0.
+      // its own out-of-block steps; their store goes to a dummy slot.
       const int rl = min(lane, n);              // idle lanes shadow row n
       const int off = act ? lane : (1 << 20);   // c = d - off: never in [0, n] when idle
       // cell (r, c) is at interleaved index 2*((r+1)*HS + c + 1) (its backup
