@@ -156,8 +156,16 @@ static thread_local std::string g_err;
 static std::recursive_mutex g_api_mu;
 #define API_LOCK std::lock_guard<std::recursive_mutex> api_lock_(g_api_mu)
 
+// A context's message is written by whichever thread fails on it (a serving
+// loop creates objects in worker threads while its main thread runs pools),
+// so the assignment is serialized.
+static std::mutex g_err_mu;
+
 static int fail(fomo_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg;
+  if (c) {
+    std::lock_guard<std::mutex> lock(g_err_mu);
+    c->err = msg;
+  }
   g_err = msg;
   return code;
 }
