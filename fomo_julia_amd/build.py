@@ -5,7 +5,12 @@ bandwidth-probe library libfomo_probe.so that bench.py's --pattern-probe loads.
 
 Flags: -ffp-contract=off keeps the kernels' arithmetic exactly as written
 (explicit fmaf only); packed-FP32 instructions are disabled; float division
-stays IEEE (hipcc's default correctly rounded divide).
+stays IEEE (hipcc's default correctly rounded divide).  The pragma unroll
+threshold is raised so that the fused kernel's z-march is fully unrolled at
+every order and tile width (its ring slots must be compile-time indices; at
+fd_order >= 10 with 2 or 4 columns per lane the default threshold left them
+in scratch memory).  Kernels the default threshold already unrolled compile
+to the same code.
 """
 from __future__ import annotations
 
@@ -35,6 +40,7 @@ def hipcc() -> str:
 def command(out: str, sources: list[str] | None = None) -> list[str]:
     return [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-mllvm", "-pragma-unroll-threshold=1000000",
             "-Wno-unused-command-line-argument", "-o", out] + (SOURCES if sources is None else sources)
 
 

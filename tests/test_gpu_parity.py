@@ -127,6 +127,21 @@ def test_column_pair_stores_bit_exact(b, nx_in, fd_order, nbc):
     assert_same_state(W, r)
 
 
+@pytest.mark.parametrize("cpt", [2, 4])
+@pytest.mark.parametrize("fd_order", [10, 12, 16])
+def test_high_order_wide_tiles_bit_exact(b, cpt, fd_order):
+    """Orders 10-16 with 2 and 4 columns per lane: the fused kernel's z-march
+    is fully unrolled there too (its ring windows stay in registers)."""
+    c = Case(nx_in=150, nz_in=70, nbc=20, fd_order=fd_order, nt=70)
+    print(f"case wide tiles order={fd_order} cpt={cpt} nx={c.M.nx}", flush=True)
+    res, W = gpu_shot(b, c, cpt=cpt, tz=0, gsteps=8)
+    b.set_tiling(1, 0, 16)
+    r = c.oracle()
+    r.run(1, c.nt)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    assert_same_state(W, r)
+
+
 @pytest.mark.parametrize("nbc", [1, 2, 33, 53, 57, 63])
 @pytest.mark.parametrize("fs", [True, False])
 @pytest.mark.parametrize("name", ["habc_v", "habc_s"])

@@ -152,6 +152,22 @@ def test_reduced_storage_tilings(b, storage, cpt, tz):
 
 
 @gpu
+@pytest.mark.parametrize("storage", ["bf16", "f16", "f16c"])
+@pytest.mark.parametrize("cpt", [2, 4])
+def test_reduced_storage_order12_wide_tiles(b, storage, cpt):
+    """fd_order 12 with 2 and 4 columns per lane in each reduced storage: the
+    fully unrolled z-march of the widest reduced-storage kernels."""
+    c = Case(nx_in=150, nz_in=70, nbc=20, fd_order=12, nt=60, storage=storage)
+    print(f"case storage={storage} order=12 cpt={cpt}", flush=True)
+    b.set_tiling(cpt, 0, 6)
+    try:
+        res, W, r, _ = _run_and_emulate(b, c, storage)
+    finally:
+        b.set_tiling(1, 0, 16)
+    _assert_same(res, W, r, f"{storage} order 12 cpt={cpt}")
+
+
+@gpu
 @pytest.mark.parametrize("storage", ["f32", "f16c", "f16", "bf16"])
 def test_c1_accuracy_vs_float64(b, storage):
     """C1 with each storage: bit-exact against the emulation and within the
