@@ -1,10 +1,12 @@
 """Parity at the benchmark's full size: the SEAM-shaped grid (4444 x 3819,
 nbc 50, order 8, padded 4552 x 3927), the benchmark's kernel and tile shape
-(two columns per lane, coefficients formed in the kernel, paired stores),
-replayed HIP graphs.  Every field starts from a random state, so every HABC
-piece, corner, band item and free-surface row is active from the first step;
-after 20 steps the five fields and the gather equal the oracle's bit for bit
-(the oracle runs the same 20 steps on the host, about a second)."""
+(two columns per lane, coefficients formed in the kernel for FP32, paired
+stores), replayed HIP graphs.  Every field starts from a random state, so
+every HABC piece, corner, band item and free-surface row is active from the
+first step; after 20 steps the five fields and the gather equal the oracle's
+bit for bit (the oracle runs the same 20 steps on the host, about a second;
+for BF16 storage it emulates the storage's rounding at every field write and
+its dithered coefficients)."""
 import numpy as np
 import pytest
 
@@ -18,13 +20,21 @@ F32 = np.float32
 NT = 20
 
 
-def test_seam_random_state_bit_exact():
+def _bf16_rne(v):
+    """v rounded to BF16 (nearest, ties to even), as FP32 values."""
+    u = v.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(F32)
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_seam_random_state_bit_exact(storage):
     import oracle as O
     dx = F32(12.5)
     vp, vs, rho = media.seam_like()
     dt = media.cfl_dt(vp, dx, 0.4)
     nbc, order = 50, 8
-    M = E.init_medium(vp, vs, rho, dx, dx, nbc, order, None, free_surface=True)
+    M = E.init_medium(vp, vs, rho, dx, dx, nbc, order, None, free_surface=True, storage=storage)
     H = E.init_habc(M.nx, M.nz, nbc, dt, dx, dx, I.julia_mean_f32(vp), None)
     p = E.SimParams(dt, NT, dx, dx, order)
     a = I.get_fd_coefficients(order)
@@ -34,7 +44,8 @@ def test_seam_random_state_bit_exact():
     xr = np.arange(0, vp.shape[0], 7, dtype=F32) * dx
     types = [("vz", "vx", "p")[k % 3] for k in range(len(xr))]
     rec = E.setup_receivers(xr, np.full(len(xr), 2 * dx, F32), M, type=types)
-    print(f"case SEAM random state {M.nx}x{M.nz}, {NT} steps, {len(xr)} receivers", flush=True)
+    print(f"case SEAM random state {storage} {M.nx}x{M.nz}, {NT} steps, {len(xr)} receivers",
+          flush=True)
 
     b = E.backend("mi355x")
     b.set_tiling(2, 0, 16)
@@ -46,12 +57,13 @@ def test_seam_random_state_bit_exact():
                     rho_vx=M.rho_vx, rho_vz=M.rho_vz, lam=M.lam, mu_txx=M.mu_txx,
                     mu_txz=M.mu_txz, w_vx=H.w_vx, w_vz=H.w_vz, w_tau=H.w_tau, src_i=src.i,
                     src_j=src.j, src_type=SRC_CODES[src.type], wavelet=src.wavelet, rec_i=rec.i,
-                    rec_j=rec.j, rec_type=rec.type_codes(), nt=NT, dtype=F32, flags=1)
+                    rec_j=rec.j, rec_type=rec.type_codes(), nt=NT, dtype=F32, flags=1,
+                    storage=storage)
     rng = np.random.default_rng(2026)
     for f in range(5):
         v = rng.standard_normal((M.nx, M.nz), dtype=F32)
         W.upload(f, v)
-        r.field(f)[:] = v
+        r.field(f)[:] = _bf16_rne(v) if storage == "bf16" else v
         del v
     E.run_time_loop(b, W, M, H, a, src, rec, p)
     b.set_tiling(1, 0, 16)
