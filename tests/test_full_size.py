@@ -76,3 +76,76 @@ def test_seam_random_state_bit_exact(storage):
             i, j = bad[0]
             raise AssertionError(f"field {f}: {len(bad)} cells differ, first (i={i}, j={j}) "
                                  f"got {got[i, j]!r} want {want[i, j]!r}")
+
+
+def test_marmousi_batch_of_64_random_states_bit_exact():
+    """The batched workload at its full size: 64 Marmousi-shaped shots (1700 x
+    350, nbc 40, order 12, padded 1792 x 442) in one batch, two columns per
+    lane (the batch's autotuned shape), each from its own random state and
+    source position; after 20 steps every shot's fields and gather equal its
+    own oracle run."""
+    import ctypes
+    import oracle as O
+    dx = F32(10.0)
+    vp, vs, rho = media.marmousi_like()
+    dt = media.cfl_dt(vp, dx, 0.4)
+    nbc, order, n_shots = 40, 12, 64
+    M = E.init_medium(vp, vs, rho, dx, dx, nbc, order, None, free_surface=True)
+    H = E.init_habc(M.nx, M.nz, nbc, dt, dx, dx, I.julia_mean_f32(vp), None)
+    p = E.SimParams(dt, NT, dx, dx, order)
+    a = I.get_fd_coefficients(order)
+    wav = I.ricker_wavelet(F32(25.0), dt, NT)
+    nx_in = vp.shape[0]
+    rec = E.setup_receivers(np.arange(nx_in, dtype=F32) * dx, np.zeros(nx_in, F32), M, type="vz")
+    print(f"case Marmousi batch {n_shots} x {M.nx}x{M.nz}, order {order}, {NT} steps", flush=True)
+
+    b = E.backend("mi355x")
+    b.set_tiling(2, 0, 16)
+    b.set_step_mode(1)
+    W0 = E.Wavefield(M.nx, M.nz, b)
+    dm = E.bind(b, W0, M, H, a, p)
+    xs = ((np.arange(n_shots) + 0.5) * (float(M.x_max) / n_shots)).astype(F32)
+    srcs, wfs, dss, states = [], [], [], []
+    rng = np.random.default_rng(7)
+    for x in xs:
+        src = E.Source(I.coord_to_index(F32(x), M.dx, M.pad), I.coord_to_index(F32(0.0), M.dz, M.pad),
+                       wav, "pressure")
+        w = E.Wavefield(M.nx, M.nz, b)
+        w._bind(dm)
+        st = [rng.standard_normal((M.nx, M.nz), dtype=F32) for _ in range(5)]
+        for f in range(5):
+            w.upload(f, st[f])
+        srcs.append(src)
+        wfs.append(w)
+        states.append(st)
+        dss.append(E._DeviceShot(dm, src, rec, NT))
+    arr = lambda hs: (ctypes.c_void_p * n_shots)(*[h.value for h in hs])
+    bh = ctypes.c_void_p()
+    b.check(b.lib.fomo_batch_create(b.ctx, n_shots, arr([dm.h] * n_shots), arr([w._h for w in wfs]),
+                                    arr([d.h for d in dss]), ctypes.byref(bh)), "batch_create")
+    try:
+        b.check(b.lib.fomo_batch_run(bh, 1, NT), "batch_run")
+        b.synchronize()
+    finally:
+        b.lib.fomo_batch_destroy(bh)
+        b.set_tiling(1, 0, 16)
+    for s in range(n_shots):
+        r = O.OracleRun(nx=M.nx, nz=M.nz, pad=M.pad, nbc=nbc, fd_order=order, free_surface=True,
+                        dt=p.dt, dtx=p.dtx, dtz=p.dtz, q=(H.qx, H.qz, H.qt_x, H.qt_z, H.qxt), a=a,
+                        rho_vx=M.rho_vx, rho_vz=M.rho_vz, lam=M.lam, mu_txx=M.mu_txx,
+                        mu_txz=M.mu_txz, w_vx=H.w_vx, w_vz=H.w_vz, w_tau=H.w_tau, src_i=srcs[s].i,
+                        src_j=srcs[s].j, src_type=SRC_CODES["pressure"], wavelet=wav,
+                        rec_i=rec.i, rec_j=rec.j, rec_type=rec.type_codes(), nt=NT, dtype=F32,
+                        flags=1)
+        for f in range(5):
+            r.field(f)[:] = states[s][f]
+        r.run(1, NT)
+        assert np.array_equal(dss[s].gather(), r.gather_nt_by_nrec()), f"shot {s}: gather"
+        for f in range(5):
+            got, want = wfs[s].download(f), r.field(f)
+            if not np.array_equal(got, want):
+                bad = np.argwhere(got != want)
+                i, j = bad[0]
+                raise AssertionError(f"shot {s} field {f}: {len(bad)} cells differ, first (i={i}, "
+                                     f"j={j}) got {got[i, j]!r} want {want[i, j]!r}")
+        states[s] = None
