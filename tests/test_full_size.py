@@ -149,3 +149,65 @@ def test_marmousi_batch_of_64_random_states_bit_exact():
                 raise AssertionError(f"shot {s} field {f}: {len(bad)} cells differ, first (i={i}, "
                                      f"j={j}) got {got[i, j]!r} want {want[i, j]!r}")
         states[s] = None
+
+
+def test_grid_beyond_32_bit_buffer_offsets_bit_exact():
+    """A grid whose coefficient blocks exceed the fused kernel's 2 GiB buffer
+    range (8600 x 8600 padded, 74 M cells; 7 blocks of 310 MB): the fused step
+    is requested, the library takes the two-pass step with 64-bit addressing,
+    and 4 steps from a random state equal the oracle's bit for bit."""
+    import oracle as O
+    nbc, order = 20, 8
+    pad = nbc + order // 2
+    nx_in = nz_in = 8600 - 2 * pad
+    rng = np.random.default_rng(5)
+    vp = np.full((nx_in, nz_in), 3000.0, F32)
+    vp[:, nz_in // 2:] = 3600.0
+    vp += rng.uniform(-50, 50, vp.shape).astype(F32)
+    vs = (vp / F32(1.73)).astype(F32)
+    rho = np.full((nx_in, nz_in), 2200.0, F32)
+    dx = F32(10.0)
+    dt = media.cfl_dt(vp, dx, 0.4)
+    M = E.init_medium(vp, vs, rho, dx, dx, nbc, order, None, free_surface=True)
+    del vs, rho
+    H = E.init_habc(M.nx, M.nz, nbc, dt, dx, dx, I.julia_mean_f32(vp), None)
+    del vp
+    p = E.SimParams(dt, 4, dx, dx, order)
+    a = I.get_fd_coefficients(order)
+    src = E.Source(I.coord_to_index(F32(M.x_max / F32(2)), M.dx, M.pad),
+                   I.coord_to_index(F32(0.0), M.dz, M.pad), I.ricker_wavelet(F32(15.0), dt, 4),
+                   "pressure")
+    xr = np.arange(0, nx_in, 13, dtype=F32) * dx
+    rec = E.setup_receivers(xr, np.full(len(xr), 2 * dx, F32), M, type="p")
+    print(f"case large grid {M.nx}x{M.nz} ({M.nx * M.nz / 1e6:.0f} M cells), 4 steps", flush=True)
+
+    b = E.backend("mi355x")
+    b.set_step_mode(1)
+    b.set_tiling(2, 0, 16)
+    W = E.Wavefield(M.nx, M.nz, b)
+    E.bind(b, W, M, H, a, p)
+    r = O.OracleRun(nx=M.nx, nz=M.nz, pad=M.pad, nbc=nbc, fd_order=order, free_surface=True,
+                    dt=p.dt, dtx=p.dtx, dtz=p.dtz, q=(H.qx, H.qz, H.qt_x, H.qt_z, H.qxt), a=a,
+                    rho_vx=M.rho_vx, rho_vz=M.rho_vz, lam=M.lam, mu_txx=M.mu_txx,
+                    mu_txz=M.mu_txz, w_vx=H.w_vx, w_vz=H.w_vz, w_tau=H.w_tau, src_i=src.i,
+                    src_j=src.j, src_type=SRC_CODES[src.type], wavelet=src.wavelet, rec_i=rec.i,
+                    rec_j=rec.j, rec_type=rec.type_codes(), nt=4, dtype=F32, flags=1)
+    for f in range(5):
+        v = rng.standard_normal((M.nx, M.nz), dtype=F32)
+        W.upload(f, v)
+        r.field(f)[:] = v
+        del v
+    try:
+        E.run_time_loop(b, W, M, H, a, src, rec, p)
+    finally:
+        b.set_tiling(1, 0, 16)
+    r.run(1, 4)
+    assert np.array_equal(rec.data, r.gather_nt_by_nrec())
+    for f in range(5):
+        got, want = W.download(f), r.field(f)
+        if not np.array_equal(got, want):
+            bad = np.argwhere(got != want)
+            i, j = bad[0]
+            raise AssertionError(f"field {f}: {len(bad)} cells differ, first (i={i}, j={j}) "
+                                 f"got {got[i, j]!r} want {want[i, j]!r}")
+        del got
