@@ -177,14 +177,27 @@ __device__ __forceinline__ float from_bits(uint32_t b) {
   if constexpr (K == ST_BF16) return __uint_as_float(b << 16);
   else return (float)__builtin_bit_cast(_Float16, (uint16_t)b);
 }
+// FP32 -> 16-bit bits, rounded to nearest (ties to even).  Both use the
+// hardware conversions (v_cvt_pk_bf16_f32, v_cvt_f16_f32); for every non-NaN
+// input the BF16 result equals the integer rounding u + 0x7FFF + ((u >> 16) & 1)
+// taken >> 16, which the oracle and the host encoder use.
 template <int K>
 __device__ __forceinline__ uint32_t to_bits(float x) {
+  if constexpr (K == ST_BF16) return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)x);
+  else return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x);
+}
+typedef float f2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
+// Two values packed low | high << 16 (one v_cvt_pk_bf16_f32 for BF16).
+template <int K>
+__device__ __forceinline__ uint32_t to_bits2(float x0, float x1) {
   if constexpr (K == ST_BF16) {
-    uint32_t u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return u >> 16;
+    f2_t v;
+    v.x = x0;
+    v.y = x1;
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf2_t));
   } else {
-    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x);
+    return to_bits<K>(x0) | (to_bits<K>(x1) << 16);
   }
 }
 
@@ -223,11 +236,11 @@ __device__ __forceinline__ void stf(typename Store<K>::T* p, const V<N>& a, floa
   } else if constexpr (N == 1) {
     *(gu16*)p = (uint16_t)to_bits<K>(a.v[0] * s);
   } else if constexpr (N == 2) {
-    *(gu32*)p = to_bits<K>(a.v[0] * s) | (to_bits<K>(a.v[1] * s) << 16);
+    *(gu32*)p = to_bits2<K>(a.v[0] * s, a.v[1] * s);
   } else {
     u2v w;
-    w.x = to_bits<K>(a.v[0] * s) | (to_bits<K>(a.v[1] * s) << 16);
-    w.y = to_bits<K>(a.v[2] * s) | (to_bits<K>(a.v[3] * s) << 16);
+    w.x = to_bits2<K>(a.v[0] * s, a.v[1] * s);
+    w.y = to_bits2<K>(a.v[2] * s, a.v[3] * s);
     *(gu2v*)p = w;
   }
 }
@@ -319,7 +332,7 @@ __device__ __forceinline__ void bstore2(Rsrc r, uint32_t vo, uint32_t so, float 
     w.y = __float_as_uint(x1);
     __builtin_amdgcn_raw_buffer_store_b64(w, r, vo, so, 0);
   } else {
-    __builtin_amdgcn_raw_buffer_store_b32(to_bits<K>(x0 * s) | (to_bits<K>(x1 * s) << 16), r, vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(to_bits2<K>(x0 * s, x1 * s), r, vo, so, 0);
   }
 }
 
