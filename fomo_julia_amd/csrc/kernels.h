@@ -206,6 +206,14 @@ typedef __attribute__((address_space(1))) uint32_t gu32;
 typedef uint32_t u2v __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) u2v gu2v;
 
+// x * s for a stored kind: BF16 arrays always carry scale 1 (they keep FP32's
+// range), so the product is dropped there; x * 1 is x exactly.
+template <int K>
+__device__ __forceinline__ float scl(float x, float s) {
+  if constexpr (K == ST_BF16) return x;
+  else return x * s;
+}
+
 // N consecutive stored values at p, as floats (x stored * s).
 template <int K, int N>
 __device__ __forceinline__ V<N> ldf(const typename Store<K>::T* p, float s) {
@@ -214,17 +222,17 @@ __device__ __forceinline__ V<N> ldf(const typename Store<K>::T* p, float s) {
   } else {
     V<N> r;
     if constexpr (N == 1) {
-      r.v[0] = from_bits<K>(*(const gu16*)p) * s;
+      r.v[0] = scl<K>(from_bits<K>(*(const gu16*)p), s);
     } else if constexpr (N == 2) {
       const uint32_t w = *(const gu32*)p;
-      r.v[0] = from_bits<K>(w & 0xFFFFu) * s;
-      r.v[1] = from_bits<K>(w >> 16) * s;
+      r.v[0] = scl<K>(from_bits<K>(w & 0xFFFFu), s);
+      r.v[1] = scl<K>(from_bits<K>(w >> 16), s);
     } else {
       const u2v w = *(const gu2v*)p;
-      r.v[0] = from_bits<K>(w.x & 0xFFFFu) * s;
-      r.v[1] = from_bits<K>(w.x >> 16) * s;
-      r.v[2] = from_bits<K>(w.y & 0xFFFFu) * s;
-      r.v[3] = from_bits<K>(w.y >> 16) * s;
+      r.v[0] = scl<K>(from_bits<K>(w.x & 0xFFFFu), s);
+      r.v[1] = scl<K>(from_bits<K>(w.x >> 16), s);
+      r.v[2] = scl<K>(from_bits<K>(w.y & 0xFFFFu), s);
+      r.v[3] = scl<K>(from_bits<K>(w.y >> 16), s);
     }
     return r;
   }
@@ -234,20 +242,20 @@ __device__ __forceinline__ void stf(typename Store<K>::T* p, const V<N>& a, floa
   if constexpr (K == ST_F32) {
     stv<N>(p, a);
   } else if constexpr (N == 1) {
-    *(gu16*)p = (uint16_t)to_bits<K>(a.v[0] * s);
+    *(gu16*)p = (uint16_t)to_bits<K>(scl<K>(a.v[0], s));
   } else if constexpr (N == 2) {
-    *(gu32*)p = to_bits2<K>(a.v[0] * s, a.v[1] * s);
+    *(gu32*)p = to_bits2<K>(scl<K>(a.v[0], s), scl<K>(a.v[1], s));
   } else {
     u2v w;
-    w.x = to_bits2<K>(a.v[0] * s, a.v[1] * s);
-    w.y = to_bits2<K>(a.v[2] * s, a.v[3] * s);
+    w.x = to_bits2<K>(scl<K>(a.v[0], s), scl<K>(a.v[1], s));
+    w.y = to_bits2<K>(scl<K>(a.v[2], s), scl<K>(a.v[3], s));
     *(gu2v*)p = w;
   }
 }
 template <int K>
 __device__ __forceinline__ float ld1(const typename Store<K>::T* p, float s) {
   if constexpr (K == ST_F32) return ldg(p);
-  else return from_bits<K>(*(const gu16*)p) * s;
+  else return scl<K>(from_bits<K>(*(const gu16*)p), s);
 }
 // The stored bits of one element (16-bit kinds zero-extended) and their value:
 // staging loops load bits unconditionally and decode afterwards, so that all
@@ -262,12 +270,12 @@ __device__ __forceinline__ uint32_t ld_bits(const typename Store<K>::T* p) {
 template <int K>
 __device__ __forceinline__ float from_stored(uint32_t b, float s) {
   if constexpr (K == ST_F32) return __uint_as_float(b);
-  else return from_bits<K>(b) * s;
+  else return scl<K>(from_bits<K>(b), s);
 }
 template <int K>
 __device__ __forceinline__ void st1(typename Store<K>::T* p, float x, float s) {
   if constexpr (K == ST_F32) stg(p, x);
-  else *(gu16*)p = (uint16_t)to_bits<K>(x * s);
+  else *(gu16*)p = (uint16_t)to_bits<K>(scl<K>(x, s));
 }
 
 // ---- buffer addressing: a descriptor (base + length in SGPRs), a per-lane
@@ -299,17 +307,17 @@ __device__ __forceinline__ V<N> bload(Rsrc r, uint32_t vo, uint32_t so, float s)
     }
   } else {
     if constexpr (N == 1) {
-      x.v[0] = from_bits<K>(__builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0)) * s;
+      x.v[0] = scl<K>(from_bits<K>(__builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0)), s);
     } else if constexpr (N == 2) {
       const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0);
-      x.v[0] = from_bits<K>(w & 0xFFFFu) * s;
-      x.v[1] = from_bits<K>(w >> 16) * s;
+      x.v[0] = scl<K>(from_bits<K>(w & 0xFFFFu), s);
+      x.v[1] = scl<K>(from_bits<K>(w >> 16), s);
     } else {
       const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0);
-      x.v[0] = from_bits<K>(w[0] & 0xFFFFu) * s;
-      x.v[1] = from_bits<K>(w[0] >> 16) * s;
-      x.v[2] = from_bits<K>(w[1] & 0xFFFFu) * s;
-      x.v[3] = from_bits<K>(w[1] >> 16) * s;
+      x.v[0] = scl<K>(from_bits<K>(w[0] & 0xFFFFu), s);
+      x.v[1] = scl<K>(from_bits<K>(w[0] >> 16), s);
+      x.v[2] = scl<K>(from_bits<K>(w[1] & 0xFFFFu), s);
+      x.v[3] = scl<K>(from_bits<K>(w[1] >> 16), s);
     }
   }
   return x;
@@ -319,7 +327,7 @@ __device__ __forceinline__ void bstore(Rsrc r, uint32_t vo, uint32_t so, float x
   if constexpr (K == ST_F32)
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, vo, so, 0);
   else
-    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)to_bits<K>(x * s), r, vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)to_bits<K>(scl<K>(x, s)), r, vo, so, 0);
 }
 
 // A lane's two adjacent elements at one offset: one 64-bit (FP32) or 32-bit
@@ -332,7 +340,7 @@ __device__ __forceinline__ void bstore2(Rsrc r, uint32_t vo, uint32_t so, float 
     w.y = __float_as_uint(x1);
     __builtin_amdgcn_raw_buffer_store_b64(w, r, vo, so, 0);
   } else {
-    __builtin_amdgcn_raw_buffer_store_b32(to_bits2<K>(x0 * s, x1 * s), r, vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(to_bits2<K>(scl<K>(x0, s), scl<K>(x1, s)), r, vo, so, 0);
   }
 }
 
@@ -340,7 +348,7 @@ __device__ __forceinline__ void bstore2(Rsrc r, uint32_t vo, uint32_t so, float 
 template <int K>
 __device__ __forceinline__ float as_stored(float x, float s, float is) {
   if constexpr (K == ST_F32) return x;
-  else return from_bits<K>(to_bits<K>(x * s)) * is;
+  else return scl<K>(from_bits<K>(to_bits<K>(scl<K>(x, s))), is);
 }
 
 template <class T>
