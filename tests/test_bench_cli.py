@@ -1,0 +1,55 @@
+"""bench.py's contract pieces that need no GPU, and the consistency of the
+measurement files it and the profiles hold:
+- more than one GPU is refused with a message, before anything touches a device;
+- the committed HBM traffic figure parses and lies near the fused kernel's
+  algorithmic 52 bytes per cell of the SEAM-shaped grid;
+- the committed batch lines agree with themselves (shots/h from ms per step)."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402  (module import only: no device work at import time)
+
+SEAM_CELLS = 4552 * 3927
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120,
+                       env={**os.environ, "HIP_VISIBLE_DEVICES": ""})
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
+
+
+def test_committed_traffic_is_near_the_algorithmic_bytes():
+    lines = [l for l in open(bench.TRAFFIC_FILE) if l.startswith("traffic_bytes_per_launch:")]
+    assert len(lines) == 1
+    traffic = int(lines[0].split(":")[1])
+    per_cell = traffic / SEAM_CELLS
+    # the counters see what the caches let through: a little under the 52
+    # algorithmic bytes (halo rows hit in L2), never a re-read of whole fields
+    assert 0.85 * bench.BYTES_FUSED_DERIVED < per_cell < 1.1 * bench.BYTES_FUSED_DERIVED
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_batch_line_is_self_consistent(storage):
+    d = json.load(open(os.path.join(ROOT, "profiles", "batch_marmousi64_bench.json")))[storage]
+    shots = int(d["config"]["workload"].split()[0])
+    seconds = d["ms_per_step"] * d["steps"] / 1e3
+    assert d["value"] == pytest.approx(shots / seconds * 3600.0, rel=2e-3)
+    assert d["mcell_updates_per_s"] == pytest.approx(
+        d["config"]["padded_cells_per_shot"] * shots / (d["ms_per_step"] / 1e3) / 1e6, rel=2e-3)
+    assert d["storage"] == storage and d["gathers_finite"]
+
+
+def test_storage_byte_counts_follow_the_array_accesses():
+    # 15 array accesses per cell (read 5 fields + 5 coefficients, write 5 fields)
+    width = {"f32": (4, 4), "bf16": (2, 2), "f16": (2, 2), "f16c": (4, 2)}  # (field, coefficient) bytes
+    for st, (fb, cb) in width.items():
+        assert bench.BYTES_FUSED_STORAGE[st] == 10 * fb + 5 * cb
+    assert bench.BYTES_FUSED_DERIVED == 10 * 4 + 3 * 4
