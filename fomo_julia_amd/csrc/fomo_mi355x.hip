@@ -1593,7 +1593,7 @@ extern "C" int64_t fomo_frames_device_bytes(const fomo_frames* f) { return f ? f
 
 extern "C" int32_t fomo_frames_download(fomo_frames* f, int32_t first, int32_t count, float* host) {
   if (!f || !host) return fail(f ? f->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
-  if (first < 0 || count < 0 || first + count > f->a.capacity)
+  if (first < 0 || count < 0 || (int64_t)first + count > f->a.capacity)
     return fail(f->ctx, FOMO_ERR_ARG, "frame range outside the store");
   fomo_ctx* c = f->ctx;
   const size_t per = (size_t)f->a.nfields * f->a.nx_d * f->a.nz_d;

@@ -234,3 +234,17 @@ def test_run_shots_with_recorder_keeps_every_shot(b):
                                                                         downsample=2))
         E.run_shot(b, E.Wavefield(c.M.nx, c.M.nz, b), c.M, c.H, c.a, src, c.rec, c.p, on_step=solo)
         assert np.array_equal(rec.frames("p")[4 * q:4 * q + 4], solo.frames("p")), q
+
+
+@gpu
+def test_frame_download_range_is_checked(b):
+    """Ranges outside the ring are refused before any copy, including one whose
+    end overflows 32 bits."""
+    import ctypes
+    print("case frames download range", flush=True)
+    st = V._FrameStore(b, 32, 24, [V.FIELDS["p"]], 1, 1, 3)
+    buf = np.zeros(1, F32)
+    ptr = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    for first, count in [(-1, 1), (0, -1), (3, 1), (2, 2), (2**31 - 1, 1), (1, 2**31 - 1)]:
+        assert b.lib.fomo_frames_download(st.h, first, count, ptr) != 0, (first, count)
+    assert st.download(b, 1, 2).shape == (2, 1, 24, 32)
