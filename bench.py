@@ -167,6 +167,93 @@ def bench_batch(args) -> dict:
             "gathers_finite": bool(all(np.isfinite(g).all() for g in gathers))}
 
 
+def bench_edges(args) -> dict:
+    """Two edges of the path on the SEAM-shaped FP32 shot (4444 x 3819, nbc 50,
+    fd_order 8):
+    - medium setup, wall time to a ready device model (synchronized, best of
+      3): init_medium on the host plus the upload of its padded coefficient
+      arrays, against init_medium_device (upload of the unpadded vp, vs, rho;
+      k_setup pads them and forms the coefficients on the device);
+    - on-step frames: the replayed loop writing a 4x-decimated pressure frame
+      every 10 steps into a device ring (fomo_run_time_loop_frames), against
+      the plain loop; HIP-event time per step on the library's stream, both
+      from step 1 of a reset shot, 2 columns per lane.  Draining the ring to
+      the host is not timed."""
+    import torch
+    from fomo_julia_amd import elastic2d as E
+    from fomo_julia_amd import init as I
+    from fomo_julia_amd import media
+    from fomo_julia_amd import video as V
+    F32 = np.float32
+    steps = args.steps if args.steps != 11520 else 2000
+    skip, dsamp = 10, 4
+    b = E.backend("mi355x")
+    lib = b.lib
+    vp, vs, rho = media.seam_like()
+    dx, nbc, order = F32(12.5), 50, 8
+    dt = media.cfl_dt(vp, dx, 0.4)
+    v_ref = I.julia_mean_f32(vp)
+    p = E.SimParams(dt, steps, dx, dx, order)
+    a = I.get_fd_coefficients(order)
+    setup = {}
+    for name, init in (("host", E.init_medium), ("device", E.init_medium_device)):
+        best = None
+        for _ in range(3):
+            b._models.clear()
+            b.synchronize()
+            t0 = time.perf_counter()
+            M = init(vp, vs, rho, dx, dx, nbc, order, None, free_surface=True)
+            H = E.init_habc(M.nx, M.nz, nbc, dt, dx, dx, v_ref, None, weights=False)
+            E._model(b, M, H, a, p)
+            b.synchronize()
+            t = time.perf_counter() - t0
+            best = t if best is None else min(best, t)
+        setup[name] = round(best, 3)
+    # the device-built model (the last one created) runs the shot, at the
+    # tiling the SEAM autotune picks (2 columns per lane, automatic rows)
+    b.set_tiling(2, 0, 16)
+    W = E.Wavefield(M.nx, M.nz, b)
+    dm = E.bind(b, W, M, H, a, p)
+    wav = I.ricker_wavelet(F32(10.0), dt, steps)
+    src = E.Source(I.coord_to_index(F32(M.x_max / F32(2)), M.dx, M.pad),
+                   I.coord_to_index(F32(0.0), M.dz, M.pad), wav, "pressure")
+    rec = E.setup_receivers(np.arange(vp.shape[0], dtype=F32) * dx, np.zeros(vp.shape[0], F32), M,
+                            type="vz")
+    ds = E._shot_for(dm, src, rec, steps)
+    store = V._FrameStore(b, M.nx, M.nz, [V.FIELDS["p"]], skip, dsamp, steps // skip)
+    stream = torch.cuda.ExternalStream(b.stream_handle())
+
+    def run(frames: bool, k0: int, n: int):
+        if frames:
+            b.check(lib.fomo_run_time_loop_frames(dm.h, W._h, ds.h, k0, n, store.h), "loop with frames")
+        else:
+            b.check(lib.fomo_run_time_loop(dm.h, W._h, ds.h, k0, n), "loop")
+
+    per_step = {}
+    for frames in (False, True):
+        run(frames, 1, 64)  # warm-up: graph capture
+        E.reset(b, W)
+        b.check(lib.fomo_shot_reset(ds.h), "shot reset")
+        b.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        run(frames, 1, steps)
+        e1.record(stream)
+        b.synchronize()
+        per_step["frames" if frames else "plain"] = e0.elapsed_time(e1) / steps
+    frame = store.download(b, steps // skip - 1, 1)
+    return {"metric": "edges of the path, SEAM-shaped FP32 shot, 1 GPU",
+            "setup_seconds": setup,
+            "setup_speedup": round(setup["host"] / setup["device"], 2),
+            "frames": {"field": "p", "skip": skip, "downsample": dsamp, "steps": steps,
+                       "frame_shape": list(store.shape[1:]),
+                       "ms_per_step_plain": round(per_step["plain"], 4),
+                       "ms_per_step_with_frames": round(per_step["frames"], 4),
+                       "overhead": round(per_step["frames"] / per_step["plain"] - 1.0, 4),
+                       "last_frame_finite_nonzero": bool(np.isfinite(frame).all() and np.abs(frame).max() > 0)},
+            "data": "synthetic (seeded SEAM-like medium)"}
+
+
 def bench_serve(n_req: int, seed: int = 5, load: float = 0.8) -> dict:
     """Served stream (requests with grids nx 300-2000, nz 300-1200, nt
     1000-6000, fd_order 4/8/12, FP32 and BF16 storage alternating): serial
@@ -259,9 +346,10 @@ def main():
     ap.add_argument("--two-pass", action="store_true", help="time the two-pass in-place step instead")
     ap.add_argument("--storage", default="f32", choices=sorted(BYTES_FUSED_STORAGE),
                     help="device storage of coefficients/fields (arithmetic stays FP32)")
-    ap.add_argument("--workload", default="seam", choices=("seam", "batch", "serve"),
+    ap.add_argument("--workload", default="seam", choices=("seam", "batch", "serve", "edges"),
                     help="seam: the single SEAM-shaped shot (default); batch: 64 batched "
-                         "Marmousi-shaped shots; serve: the served request stream")
+                         "Marmousi-shaped shots; serve: the served request stream; edges: "
+                         "device-side setup and on-step frames on the SEAM-shaped shot")
     ap.add_argument("--shots", type=int, default=64, help="shots of the batch workload")
     ap.add_argument("--requests", type=int, default=40,
                     help="requests of the served sample (default run) or stream (serve)")
@@ -290,6 +378,9 @@ def main():
         print(f"bench.py: this benchmark runs on one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "edges":
+        print(json.dumps(bench_edges(args)))
+        return
     if args.workload == "batch":
         print(json.dumps(bench_batch(args)))
         return

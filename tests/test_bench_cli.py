@@ -3,7 +3,7 @@ measurement files it and the profiles hold:
 - more than one GPU is refused with a message, before anything touches a device;
 - the committed HBM traffic figure parses and lies near the fused kernel's
   algorithmic 52 bytes per cell of the SEAM-shaped grid;
-- the committed batch lines agree with themselves (shots/h from ms per step)."""
+- the committed batch and edges lines agree with themselves."""
 import json
 import os
 import subprocess
@@ -53,3 +53,13 @@ def test_storage_byte_counts_follow_the_array_accesses():
     for st, (fb, cb) in width.items():
         assert bench.BYTES_FUSED_STORAGE[st] == 10 * fb + 5 * cb
     assert bench.BYTES_FUSED_DERIVED == 10 * 4 + 3 * 4
+
+
+def test_edges_line_is_self_consistent():
+    d = json.load(open(os.path.join(ROOT, "profiles", "edges_seam_bench.json")))
+    s = d["setup_seconds"]
+    assert d["setup_speedup"] == pytest.approx(s["host"] / s["device"], rel=2e-2)
+    f = d["frames"]
+    assert f["overhead"] == pytest.approx(f["ms_per_step_with_frames"] / f["ms_per_step_plain"] - 1, abs=2e-3)
+    assert f["frame_shape"] == [-(-3927 // f["downsample"]), -(-4552 // f["downsample"])]
+    assert f["last_frame_finite_nonzero"]
