@@ -35,7 +35,10 @@ class OracleDesc(ctypes.Structure):
                 ("flags", ctypes.c_int32), ("threads", ctypes.c_int32),
                 ("storage", ctypes.c_int32), ("fscale", ctypes.c_float * 5)]
 
-STORAGE = {"f32": 0, "bf16": 1, "f16": 2, "f16c": 3}
+# "bfp16" (block floating point fields, FP32 coefficients) and "bfp16c" (the
+# same with FP16 coefficients) are a design study of a 16-bit field format:
+# the oracle emulates them, no kernel implements them.
+STORAGE = {"f32": 0, "bf16": 1, "f16": 2, "f16c": 3, "bfp16": 4, "bfp16c": 5}
 
 
 def build() -> str:
@@ -64,12 +67,21 @@ def lib():
         _lib.oracle_max_threads.restype = ctypes.c_int
         _lib.oracle_round_storage.restype = ctypes.c_float
         _lib.oracle_round_storage.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float]
+        _lib.oracle_round_block.restype = None
+        _lib.oracle_round_block.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int]
     return _lib
 
 
 def round_storage(storage: str, x: float, scale: float = 1.0) -> float:
     """x rounded to the storage format (at the FP16 scale), as the oracle rounds fields."""
     return lib().oracle_round_storage(STORAGE[storage], float(x), float(scale))
+
+
+def round_block(x) -> np.ndarray:
+    """One block floating-point segment (storage "bfp16") as the oracle rounds it."""
+    y = np.ascontiguousarray(np.asarray(x, dtype=np.float32)).copy()
+    lib().oracle_round_block(y.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(y))
+    return y
 
 
 def max_threads() -> int:
@@ -84,8 +96,10 @@ class OracleRun:
     ``dtype`` np.float32 is the reference's FP32 path; np.float64 evaluates the
     same algorithm in double precision (inputs converted from the given
     arrays, which the caller may have computed in Float64).  ``storage``
-    "bf16" / "f16" (FP32 only) emulates reduced-precision storage of the
-    coefficients and fields; ``fscale`` are the FP16 field store scales.
+    "bf16" / "f16" / "f16c" (FP32 only) emulates reduced-precision storage of
+    the coefficients and fields; ``fscale`` are the FP16 field store scales.
+    "bfp16" emulates the block floating-point field format of the design study
+    (FP32 coefficients).
     """
 
     def __init__(self, *, nx, nz, pad, nbc, fd_order, free_surface, dt, dtx, dtz,

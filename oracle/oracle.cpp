@@ -48,6 +48,14 @@
 // (spacing 2^(e-p+1) for a p-bit significand, the subnormal spacing below
 // the smallest normal exponent).
 //
+// Storage 4 (block floating point) is a design study of a 16-bit field
+// format: no kernel implements it.  Coefficients stay FP32.  After each
+// stage, every row of each field it writes is cut into segments of 64 cells
+// from i = 0.  A segment keeps one exponent e, with its largest magnitude in
+// [2^(e-1), 2^e), and per cell a signed 16-bit mantissa on the spacing
+// 2^(e-15), rounded to nearest (ties to even) and clamped to +-32767.
+// Storage 5 is the same field format with storage 3's FP16 coefficients.
+//
 // Pinning: see tests/test_oracle.py (polynomial-exactness of the stencils,
 // closed-form HABC cases, free surface, symmetry, 2^k linearity, energy decay,
 // the reference's shot_1.bin gather).
@@ -72,7 +80,7 @@ typedef struct {
   double qx, qz, qt_x, qt_z, qxt;
   int32_t flags;
   int32_t threads;
-  int32_t storage;   // 0 = FP32 (no rounding), 1 = BF16, 2 = FP16 (see above)
+  int32_t storage;   // 0 = FP32 (no rounding), 1 = BF16, 2 = FP16, 3, 4 (see above)
   float fscale[5];   // FP16 store scales of vx vz txx tzz txz
 } oracle_desc;
 }
@@ -90,6 +98,7 @@ struct Model {
   const T *w_vx, *w_vz, *w_tau;
   T* f[10];  // vx vz txx tzz txz, then the *_old of each
   int storage = 0;
+  bool cq_on = false;    // stages read the rounded coefficients cq
   std::vector<T> cq[7];  // rounded coefficients (reduced-precision storage)
 };
 
@@ -133,8 +142,29 @@ static float dither_coef(int storage, float x, float scale, uint32_t i, uint32_t
   const double q = std::copysign((a - fl > u ? fl + 1 : fl) * spacing, v);
   return (float)(q / scale);
 }
+// One block floating-point segment of n cells, rounded in place (storage 4).
+template <typename T>
+static void round_block(T* x, int n) {
+  double mx = 0;
+  for (int t = 0; t < n; ++t) mx = std::max(mx, std::fabs((double)x[t]));
+  if (!(mx > 0) || !std::isfinite(mx)) return;
+  int e;
+  std::frexp(mx, &e);  // mx in [2^(e-1), 2^e)
+  const double spacing = std::ldexp(1.0, e - 15);
+  for (int t = 0; t < n; ++t) {
+    const double q = std::min(32767.0, std::max(-32767.0, std::nearbyint((double)x[t] / spacing)));
+    x[t] = (T)(float)(q * spacing);
+  }
+}
 template <typename T>
 static void round_fields(Model<T>& m, int c0, int c1) {
+  if (m.storage == 4 || m.storage == 5) {
+    for (int c = c0; c < c1; ++c)
+      for (int j = 0; j < m.nz; ++j)
+        for (int i0 = 0; i0 < m.nx; i0 += 64)
+          round_block(m.f[c] + (size_t)j * m.nx + i0, std::min(64, m.nx - i0));
+    return;
+  }
   if (m.storage != 1 && m.storage != 2) return;
   const size_t n = (size_t)m.nx * m.nz;
   for (int c = c0; c < c1; ++c) {
@@ -216,7 +246,7 @@ static void stage_velocity(Model<T>& m) {
       }
       size_t o = (size_t)i + (size_t)j * nx;
       T bx = m.dtx / m.rho_vx[o], bz = m.dtz / m.rho_vx[o];
-      if (m.storage) {
+      if (m.cq_on) {
         bx = m.cq[0][o];
         bz = m.cq[1][o];
       }
@@ -224,7 +254,7 @@ static void stage_velocity(Model<T>& m) {
       vx[o] = vx[o] + inc;
       bx = m.dtx / m.rho_vz[o];
       bz = m.dtz / m.rho_vz[o];
-      if (m.storage) {
+      if (m.cq_on) {
         bx = m.cq[2][o];
         bz = m.cq[3][o];
       }
@@ -257,7 +287,7 @@ static void stage_stress(Model<T>& m) {
       T l_val = m.lam[o], m_val = m.mu_txx[o];
       T l2m = l_val + (T)2 * m_val;
       T mu_xz = m.mu_txz[o];
-      if (m.storage) {
+      if (m.cq_on) {
         l_val = m.cq[4][o];
         l2m = m.cq[5][o];
         mu_xz = m.cq[6][o];
@@ -442,7 +472,8 @@ static Model<T> make_model(const oracle_desc* d, const T* a, const T* const* med
   size_t n = (size_t)d->nx * d->nz;
   for (int c = 0; c < 10; ++c) m.f[c] = fields + c * n;
   m.storage = d->storage;
-  if (m.storage) {
+  m.cq_on = (m.storage >= 1 && m.storage <= 3) || m.storage == 5;
+  if (m.cq_on) {
     std::vector<float> c(n);
     for (int k = 0; k < 7; ++k) {
       for (size_t o = 0; o < n; ++o) {
@@ -530,6 +561,8 @@ int oracle_run_f64(const oracle_desc* d, const double* a, const double* const* m
                            n_rec, nt, k_first, n_steps, gather, stage_mask);
 }
 
+// One block floating-point segment rounded in place (storage 4's own tests).
+void oracle_round_block(float* x, int n) { round_block(x, n); }
 // One value rounded as the storage holds it (for the rounding's own tests).
 float oracle_round_storage(int storage, float x, float scale) {
   return round_storage(storage, x, scale);

@@ -65,21 +65,65 @@ def test_bf16_rounding_matches_torch_and_bits():
     assert np.array_equal(got, want)
 
 
-def test_emulated_storage_accuracy_c1_cpu():
-    """The oracle's storage emulation on the homogeneous 200 x 200 case: FP16
-    coefficients with FP32 fields stay within 1e-3 of Float64 because the
-    coefficients are dithered (rounding them to nearest instead gives a
-    systematic ~3e-3 error in a homogeneous medium)."""
+@pytest.fixture(scope="module")
+def c1_f64():
     O.build()
     c = C1Case()
     r64 = c.oracle(np.float64)
     r64.run(1, c.nt)
-    g64 = c.split(r64.gather_nt_by_nrec())
+    return c, c.split(r64.gather_nt_by_nrec())
+
+
+def test_emulated_storage_accuracy_c1_cpu(c1_f64):
+    """The oracle's storage emulation on the homogeneous 200 x 200 case: FP16
+    coefficients with FP32 fields stay within 1e-3 of Float64 because the
+    coefficients are dithered (rounding them to nearest instead gives a
+    systematic ~3e-3 error in a homogeneous medium)."""
+    c, g64 = c1_f64
     r = c.oracle(storage="f16c")
     r.run(1, c.nt)
     g = c.split(r.gather_nt_by_nrec())
     for t in g:
         assert rel_l2(g[t], g64[t]) < C1_BOUNDS["f16c"], t
+
+
+def test_block_format_rounding():
+    """The block floating-point segment (storage "bfp16", a design study that
+    no kernel implements) against hand-computed values: the largest magnitude
+    3 puts the spacing at 2^-13, so 2^-20 vanishes, ties go to even, and a
+    mantissa that would round to 2^15 is clamped to 32767."""
+    O.build()
+    F = np.float32
+    assert np.array_equal(O.round_block([3.0, 1.0, 2.0**-20, -1.5]), np.array([3.0, 1.0, 0.0, -1.5], F))
+    # 1 + 2^-14 is 8192.5 spacings (tie: to 8192), 1 + 3 * 2^-14 is 8193.5 (to 8194)
+    got = O.round_block([3.0, 1 + 2.0**-14, 1 + 3 * 2.0**-14, -(1 + 2.0**-14)])
+    assert np.array_equal(got, np.array([3.0, 1.0, 1 + 2.0**-12, -1.0], F))
+    assert O.round_block([4 - 2.0**-20])[0] == F(4 - 2.0**-13)  # 32768 clamped to 32767
+    assert np.array_equal(O.round_block(np.zeros(64)), np.zeros(64, F))
+    rng = np.random.default_rng(11)
+    for _ in range(50):
+        x = (rng.standard_normal(64) * 10.0 ** rng.uniform(-8, 8)).astype(F)
+        y = O.round_block(x)
+        e = np.frexp(np.abs(x.astype(np.float64)).max())[1]
+        spacing = 2.0 ** (e - 15)
+        assert np.abs(y.astype(np.float64) - x).max() <= spacing / 2
+        assert np.array_equal(np.rint(y / spacing) * spacing, y)  # on the segment's grid
+        assert np.array_equal(O.round_block(y), y)                # idempotent
+
+
+@pytest.mark.parametrize("storage", ["bfp16", "bfp16c"])
+def test_block_format_accuracy_c1_cpu(c1_f64, storage):
+    """The design study's 16-bit field format meets 1e-3 of Float64 on the
+    homogeneous 200 x 200 case, where FP16 fields give 2.7e-3 and BF16 2.2e-2:
+    about 5.1e-4 / 3.5e-4 / 6.0e-4 for vx / vz / p with FP32 coefficients
+    ("bfp16"), 9.6e-4 / 6.5e-4 / 8.1e-4 with dithered FP16 coefficients
+    ("bfp16c")."""
+    c, g64 = c1_f64
+    r = c.oracle(storage=storage)
+    r.run(1, c.nt)
+    g = c.split(r.gather_nt_by_nrec())
+    for t in g:
+        assert rel_l2(g[t], g64[t]) < 1e-3, t
 
 
 # ---------------------------------------------------------------- GPU
