@@ -1834,9 +1834,13 @@ extern "C" int32_t fomo_pool_create(fomo_ctx* c, int32_t fd_order, int32_t stora
   b->graph_steps = c->graph_steps;
   b->step_mode = 1;
   b->step_mode_req = 1;
-  // rows per tile: about 32 with tz + 2M a whole number of march blocks
+  // rows per tile: about 12 with tz + 2M a whole number of march blocks (14,
+  // 12, 16 rows at orders 4, 8, 12).  The served grids are small, and short
+  // tiles give more items to fill the GPU: on a 40-request stream of mixed
+  // orders this served 16 % more shots per hour than about 32 rows (20-30
+  // rows: 11 %; 60-90 rows: 17-30 % fewer), with lower latency.
   const int L = fused_block(M);
-  b->tz = c->tz > 0 ? c->tz : std::max(8, round_up(32 + 2 * M, L) - 2 * M);
+  b->tz = c->tz > 0 ? c->tz : std::max(8, round_up(12 + 2 * M, L) - 2 * M);
   b->tz_req = c->tz;
   // fixed persistent grids: one round of resident wavefronts for the stencil
   // kernels, a few blocks per CU for the strip kernels
