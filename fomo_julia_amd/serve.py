@@ -164,10 +164,11 @@ def prepare(req: ShotRequest) -> Prepared:
 # ---------------------------------------------------------------- pools
 # Tile shape (columns per lane, rows per tile) of the served pools and of the
 # serial baseline, per FD order, fixed so that a pool does not inherit whatever
-# shape an earlier autotune left on the context.  On the 40-request stream,
-# two columns per lane for orders 8 and 12 measured within 1 % of one column,
-# and two columns for every order 4 % slower: the served grids are small, and
-# one column per lane gives more tiles to fill the GPU.
+# shape an earlier autotune left on the context.  On the 40-request stream
+# (pool tiles of 12-16 rows), two columns per lane for orders 8 and 12 served
+# 0.6 % fewer shots per hour than one column, and two columns for every order
+# 1.1 % fewer: the served grids are small, and one column per lane gives more
+# tiles to fill the GPU.
 SERVE_TILING = {4: (1, 0), 8: (1, 0), 12: (1, 0)}
 
 
