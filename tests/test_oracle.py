@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from fomo_julia_amd import elastic2d as E
 from tests.helpers import Case, rel_l2, run_jl_case
 
 F32 = np.float32
@@ -232,3 +233,101 @@ def test_fp64_energy_bounded_and_fp32_agrees():
     rsep = c.oracle(flags=0)
     rsep.run(1, 400)
     assert rel_l2(rsep.gather, r32.gather) < 1e-5
+
+
+def _random_state(r, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    r.fields[:] = (scale * rng.standard_normal(r.fields.shape)).astype(r.fields.dtype)
+    return r.fields.copy()
+
+
+@pytest.mark.parametrize("src_type,comp,rho_idx", [("force_z", 1, 1), ("force_x", 0, 0)])
+def test_force_injection_changes_one_cell_by_w_dt_over_rho(src_type, comp, rho_idx):
+    """backend/cpu.jl:180-183: vz[si,sj] += wav*(dt/rho_vz[si,sj]) (force_z),
+    vx[si,sj] += wav*(dt/rho_vx[si,sj]) (force_x), Float32 in that order.
+    The layered, jittered medium makes rho_vx and rho_vz differ at the source
+    cell and dx = 10 makes dt differ from dt/dx, so a wrong density array or a
+    missing/extra 1/dx fails here."""
+    c = Case(nx_in=40, nz_in=36, nbc=6, fd_order=4, fs=False, nt=8, src_type=src_type,
+             src_xz=(F32(200.0), F32(170.0)))
+    k = 3
+    r = c.oracle()
+    # a state of the increment's magnitude, so that the increment shows in Float32
+    before = _random_state(r, 11, scale=1e-7)
+    r.run(k, 1, O.STAGES["inject"])
+    si, sj = c.src.i - 1, c.src.j - 1
+    rho = (c.M.rho_vx, c.M.rho_vz)[rho_idx][si, sj]
+    other = (c.M.rho_vx, c.M.rho_vz)[1 - rho_idx][si, sj]
+    assert rho != other and c.p.dtx != c.p.dt
+    w = c.src.wavelet[k - 1]
+    assert w != 0
+    want = F32(before[comp].T[si, sj] + F32(w * F32(c.p.dt / rho)))
+    assert r.field(comp)[si, sj] == want
+    changed = np.argwhere(r.fields != before)
+    assert changed.tolist() == [[comp, sj, si]]
+
+
+@pytest.mark.parametrize("src_type,comp,rho_idx", [("force_z", 1, 1), ("force_x", 0, 0)])
+def test_force_source_adds_momentum_w_dt_per_step(src_type, comp, rho_idx):
+    """Momentum balance of the staggered scheme, independent of how the source
+    term is written: the stress-divergence stencil telescopes, so before the
+    wave reaches the edges sum(rho_c * v_c) over the grid grows by exactly
+    dt * w[k] per step of a force on component c (FP64 oracle).  A source
+    scaled by the wrong density, by dt/dx, or applied to the other component
+    breaks the balance."""
+    nt = 24
+    # the source sits on the last row above the layer interface, where rho_vz
+    # (a 4-cell average across it) differs from rho_vx
+    c = Case(nx_in=90, nz_in=90, nbc=10, fd_order=8, fs=False, nt=nt, src_type=src_type,
+             src_xz=(F32(450.0), F32(440.0)), layered=True, f0=30.0)
+    si, sj = c.src.i - 1, c.src.j - 1
+    assert c.M.rho_vx[si, sj] != c.M.rho_vz[si, sj]
+    r = c.oracle(dtype=np.float64)
+    rho = np.asarray((c.M.rho_vx, c.M.rho_vz)[rho_idx], dtype=np.float64)
+    w = c.src.wavelet.astype(np.float64)
+    for k in range(1, nt + 1):
+        r.run(k, 1)
+        mom = float((rho * r.field(comp)).sum())
+        want = float(c.p.dt) * float(w[:k].sum())
+        assert mom == pytest.approx(want, rel=1e-9, abs=1e-9 * float(np.abs(w).sum()) * float(c.p.dt))
+        # the other component gains no net momentum
+        other = np.asarray((c.M.rho_vx, c.M.rho_vz)[1 - rho_idx], dtype=np.float64)
+        assert abs(float((other * r.field(1 - comp)).sum())) < 1e-9 * max(abs(want), 1e-30) + 1e-18
+    # the wave is still clear of the HABC strips (the balance would not hold
+    # there): only the stencil's numerical tail reaches them
+    n = c.nbc + 2
+    peak = np.abs(r.fields[:5]).max()
+    assert np.abs(r.fields[:5, :, :n]).max() < 1e-12 * peak
+    assert np.abs(r.fields[:5, :n, :]).max() < 1e-12 * peak
+
+
+def test_receivers_record_vz_vx_and_half_pressure():
+    """kernels/source_receiver.jl:65-77 (CPU path): data[k, r] = vz, vx or
+    (txx + tzz) * 0.5f0 at (i_r, j_r), Float32, after the step."""
+    c = Case(nx_in=40, nz_in=36, nbc=6, fd_order=4, fs=False, nt=6)
+    rng = np.random.default_rng(5)
+    n = 24
+    ri = rng.integers(1, c.M.nx + 1, n)
+    rj = rng.integers(1, c.M.nz + 1, n)
+    types = np.array([0, 1, 2] * (n // 3), dtype=np.int32)
+    c.rec = E.Receivers(ri, rj, np.zeros((1, n), F32), type=[("vz", "vx", "p")[t] for t in types])
+    r = c.oracle()
+    st = _random_state(r, 9)
+    k = 4
+    r.run(k, 1, O.STAGES["record"])
+    g = r.gather_nt_by_nrec()
+    f = lambda comp, q: st[comp].T[ri[q] - 1, rj[q] - 1]
+    for q in range(n):
+        if types[q] == 0:
+            want = f(1, q)
+        elif types[q] == 1:
+            want = f(0, q)
+        else:
+            want = F32(F32(f(2, q) + f(3, q)) * F32(0.5))
+        assert g[k - 1, q] == want, q
+    assert np.all(np.delete(g, k - 1, axis=0) == 0)
+    # recording changes no field
+    assert np.array_equal(r.fields, st)
+    # p is the mean of txx and tzz, not their sum or vz: the three types differ here
+    q = int(np.argmax(types == 2))
+    assert g[k - 1, q] != f(1, q) and g[k - 1, q] != F32(f(2, q) + f(3, q))
