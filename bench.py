@@ -7,17 +7,19 @@ dx=dz=12.5 m, free surface, synthetic SEAM-like medium (seed 2), dt from CFL
 0.4, 10 Hz Ricker pressure source at the surface centre, 4444 vz receivers
 on the surface.  A step is one time step of the whole grid (velocity, HABC,
 stress, HABC, free surface, receivers); the default K = 11520 steps is the
-full shot.  W warm-up steps run first (graph capture, clocks), then the
-wavefield and gather are reset and exactly K steps are timed from step 1.
+full shot.  W warm-up steps run first (clocks, first launches), then the
+wavefield and gather are reset, the step graphs are captured (no step runs),
+and exactly K steps are timed from step 1.
 
 The step is the fused single-pass form (one velocity+stress kernel over
 ping-pong buffers, then velocity HABC, band stress, stress HABC, free
 surface + receivers).  Prints one JSON line: Mcell-updates/s over the padded
 grid, the fused kernel's HBM roofline (algorithmic 52 B per cell-update: read
 5 fields + rho_vx, lam, mu_txx, write 5 fields; over
-its mean duration measured with HIP events on the library's stream; HBM
-traffic per launch from the committed rocprofv3 counter summary), and the
-CPU oracle timed on a bounded sample of the same workload.
+its mean duration measured with HIP events on the library's stream, at a step
+inside a 64-step timing shot; HBM traffic per launch from the committed
+rocprofv3 counter summary), and the CPU oracle timed on a bounded sample of
+the same workload on the CPUs this process may use.
 """
 from __future__ import annotations
 
@@ -46,6 +48,52 @@ BYTES_VELOCITY = 36  # two-pass velocity: read txx tzz txz, 2 buoyancies; read+w
 BYTES_STRESS = 44    # two-pass stress: read vx vz lam lam+2mu mu_txz; read+write txx tzz txz
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fused_seam_traffic.txt")
 METRIC = "Mcell-updates/s & %HBM roofline, 1 GPU; shots/hr + p99 latency (served)"
+# The autotuner's trials and the per-kernel timings run on a timing shot of
+# TIMING_NT steps (same medium, source cell and receivers), so that every
+# timed launch does a step's full work whatever K the timed loop has: a
+# kernel launched at a step k > nt of its shot returns at once.
+TIMING_NT = 64
+STAGE_STEP = 32
+
+
+def stage_step(nt: int) -> int:
+    """The step the per-kernel launches are timed at: inside [1, nt] of the shot they run on."""
+    if nt < 1:
+        raise ValueError("a shot has at least one step")
+    return max(1, min(STAGE_STEP, nt))
+
+
+def cpu_share() -> dict:
+    """The host CPUs this process may use: its affinity set, capped by the
+    cgroup's CPU quota when one is set (cgroup v2 cpu.max, else v1)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        try:
+            q = int(open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us").read())
+            period = int(open("/sys/fs/cgroup/cpu/cpu.cfs_period_us").read())
+            if q > 0:
+                quota = q / period
+        except (OSError, ValueError):
+            pass
+    usable = aff if quota is None else max(1, min(aff, int(quota + 0.999)))
+    return {"usable": usable, "host_cpus": os.cpu_count(), "affinity": aff,
+            "cgroup_quota": None if quota is None else round(quota, 2)}
+
+
+def check_roofline(frac: float, kernel_ms: float, ms_per_step: float) -> None:
+    """A roofline line is only printed for a kernel that did its work: its
+    fraction of peak lies in (0, 1] and one launch is no longer than a step."""
+    if not (0.0 < frac <= 1.0):
+        raise SystemExit(f"bench.py: roofline fraction {frac:.4f} is outside (0, 1]: "
+                         "the timed kernel did not do a step's work")
+    if not (0.0 < kernel_ms <= ms_per_step):
+        raise SystemExit(f"bench.py: one kernel launch ({kernel_ms:.4f} ms) is not within "
+                         f"a step ({ms_per_step:.4f} ms)")
 
 
 def build_seam(nt: int, nbc=50, fd_order=8, with_weights=False, storage="f32"):
@@ -265,8 +313,10 @@ def bench_serve(n_req: int, seed: int = 5, load: float = 0.8) -> dict:
     b = E.backend("mi355x")
     reqs = S.c5_stream(n_req, seed=seed)
     t = time.perf_counter()
-    ser = S.serial(reqs, b, storage="f32")
+    prep_times = []
+    ser = S.serial(reqs, b, storage="f32", prep_times=prep_times)
     w_ser = time.perf_counter() - t
+    w_ser_gpu = w_ser - sum(prep_times)  # the serial loop without its host preparation
     srv = S.Server(b, chunk=64, capacity=64)
     t = time.perf_counter()
     sat = srv.serve(reqs, realtime=False)
@@ -277,11 +327,16 @@ def bench_serve(n_req: int, seed: int = 5, load: float = 0.8) -> dict:
     poi = srv2.serve(S.with_arrivals(reqs, rate), realtime=True)
     w_poi = time.perf_counter() - t
     s_ser, s_sat, s_poi = S.summarize(ser, w_ser), S.summarize(sat, w_sat), S.summarize(poi, w_poi)
-    return {"requests": n_req, "serial_f32": s_ser, "served_capacity": s_sat,
+    return {"requests": n_req, "serial_f32": {**s_ser, "prepare_s": round(sum(prep_times), 3)},
+            "served_capacity": s_sat,
             "served_poisson": {**s_poi, "arrival_rate_per_s": round(rate, 3), "load": load},
             "speedup_shots_per_hour": round(s_sat["shots_per_hour"] / s_ser["shots_per_hour"], 3),
+            # against the serial loop with its host preparation taken out (the
+            # server prepares on worker threads, overlapped with the GPU)
+            "speedup_vs_serial_without_prepare": round(w_ser_gpu / w_sat, 3),
             "server_stats": {k: (round(v, 3) if isinstance(v, float) else v)
-                             for k, v in srv.stats.items()}}
+                             for k, v in srv.stats.items()},
+            "failed_requests": [f.__dict__ for f in srv.failed + srv2.failed]}
 
 
 def pattern_probe(reps: int = 20) -> dict:
@@ -295,7 +350,7 @@ def pattern_probe(reps: int = 20) -> dict:
     import torch
     from fomo_julia_amd import build as B
     if not os.path.exists(B.PROBE_PATH):
-        raise RuntimeError(f"{B.PROBE_PATH} is not built (python -m fomo_julia_amd.build)")
+        B.build_probe()
     lib = ctypes.CDLL(B.PROBE_PATH)
     lib.fomo_probe_pattern.argtypes = [ctypes.c_int] * 9 + [ctypes.POINTER(ctypes.c_double)]
     lib.fomo_probe_stream.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
@@ -407,6 +462,10 @@ def main():
     W = E.Wavefield(M.nx, M.nz, b)
     dm = E.bind(b, W, M, H, case["a"], p)
     ds = E._shot_for(dm, src, rec, p.nt)
+    # the timing shot: TIMING_NT steps of the same source and receivers
+    from fomo_julia_amd import init as I
+    src_t = E.Source(src.i, src.j, I.ricker_wavelet(np.float32(10.0), p.dt, TIMING_NT), src.type)
+    ds_t = E._DeviceShot(dm, src_t, rec, TIMING_NT)
     lib = b.lib
     stream = torch.cuda.ExternalStream(b.stream_handle())
     tuning = None
@@ -415,14 +474,16 @@ def main():
         b.set_tiling(cpt, tz, gs)
     elif fused:
         from fomo_julia_amd import tune as T
-        tuning = T.autotune_batch(b, [dm], [W], [ds])
+        tuning = T.autotune_batch(b, [dm], [W], [ds_t])
 
-    # warm-up: graph capture, clocks
+    # warm-up (clocks, first launches), then the step graphs of the timed
+    # run are captured before timing starts
     if args.warmup:
         b.check(lib.fomo_run_time_loop(dm.h, W._h, ds.h, 1, args.warmup), "warmup")
     b.synchronize()
     E.reset(b, W)
     b.check(lib.fomo_shot_reset(ds.h), "shot_reset")
+    b.check(lib.fomo_run_time_loop_prepare(dm.h, W._h, ds.h), "graph capture")
     b.synchronize()
     torch.cuda.synchronize()
 
@@ -452,14 +513,17 @@ def main():
     upload_s = time.perf_counter() - t_u
     del dm_up
 
-    # per-kernel mean durations (HIP events on the library's stream)
+    # per-kernel mean durations (HIP events on the library's stream), on the
+    # timing shot at a step inside its nt
+    k_stage = stage_step(TIMING_NT)
+
     def stage_ms(which: int, reps: int) -> float:
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
-        b.check(lib.fomo_launch_stage(dm.h, W._h, ds.h, which, 200, 2), "stage warm")
+        b.check(lib.fomo_launch_stage(dm.h, W._h, ds_t.h, which, k_stage, 2), "stage warm")
         b.synchronize()
         e0.record(stream)
-        b.check(lib.fomo_launch_stage(dm.h, W._h, ds.h, which, 200, reps), "stage")
+        b.check(lib.fomo_launch_stage(dm.h, W._h, ds_t.h, which, k_stage, reps), "stage")
         e1.record(stream)
         b.synchronize()
         return e0.elapsed_time(e1) / reps
@@ -484,6 +548,7 @@ def main():
     main_gbs = main_bytes * cells / (main_ms / 1e3) / 1e9
     step_bytes = main_bytes if fused else BYTES_VELOCITY + BYTES_STRESS
     step_gbs = step_bytes * cells / (ms_per_step / 1e3) / 1e9
+    check_roofline(main_gbs / HBM_PEAK_GBS, main_ms, ms_per_step)
     traffic = None
     if fused and args.storage == "f32" and os.path.exists(TRAFFIC_FILE):
         for line in open(TRAFFIC_FILE):
@@ -493,7 +558,7 @@ def main():
     # the served sample runs with the SEAM shot's device objects released and
     # before the CPU baseline (whose OpenMP threads would share the host cores
     # with the server's worker threads)
-    del W, dm, ds
+    del W, dm, ds, ds_t
     import gc
     gc.collect()
     b.synchronize()
@@ -506,17 +571,20 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
+        share = cpu_share()
+        threads = share["usable"]
         try:
             v, el = cpu_baseline(case, args.cpu_steps, threads)
             cpu = {"value": round(v, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+                   "host_cpus": share,
                    "sample": f"first {args.cpu_steps} steps of the same SEAM-shaped shot "
                              f"({M.nx}x{M.nz} padded cells/step), C++ oracle, OpenMP stencils, "
                              f"sequential HABC, {el:.1f} s; full shot extrapolated "
-                             f"{el / args.cpu_steps * 11520 / 60:.1f} min"}
+                             f"{el / args.cpu_steps * 11520 / 60:.1f} min; OpenMP threads = "
+                             "the CPUs this process may use (affinity, cgroup quota)"}
         except Exception as e:  # reported, never fatal
             cpu = {"value": None, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
-                   "sample": f"failed: {e}"}
+                   "host_cpus": share, "sample": f"failed: {e}"}
 
     out = {
         "metric": METRIC,
@@ -548,7 +616,12 @@ def main():
                                              + upload_s + gather_s, 3)}},
         "roofline": {"bound": "hbm", "achieved": round(main_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(main_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": ("profiles/k_fused_seam_traffic.txt: rocprofv3 --pmc "
+                                        "FETCH_SIZE and WRITE_SIZE per launch, scaled by a plain "
+                                        "copy's counts (not measured in this run)") if traffic else None,
                      "kernel": main_name, "kernel_ms": round(main_ms, 5),
+                     "kernel_timing": f"HIP events around {reps} back-to-back launches on the "
+                                      f"library's stream, step {k_stage} of a {TIMING_NT}-step shot",
                      "algorithmic_bytes_per_launch": main_bytes * cells},
         "kernels_ms": {k: round(v, 5) for k, v in kernels.items()},
         "tiling": ({"cols_per_lane": tuning["tiling"][0], "rows_per_tile": tuning["tiling"][1],

@@ -18,6 +18,7 @@ c_void_p = ctypes.c_void_p
 P_f32 = ctypes.POINTER(ctypes.c_float)
 P_i32 = ctypes.POINTER(ctypes.c_int32)
 
+FOMO_OK, FOMO_ERR_ARG, FOMO_ERR_HIP, FOMO_ERR_UNSUPPORTED, FOMO_ERR_STATE = 0, -1, -2, -3, -4
 SRC_TYPES = {"pressure": 0, "force_z": 1, "force_x": 2}
 REC_TYPES = {"vz": 0, "vx": 1, "p": 2}
 STORAGE = {"f32": 0, "bf16": 1, "f16": 2, "f16c": 3}
@@ -64,6 +65,7 @@ SIGNATURES = {
     "fomo_shot_gather": (c_int32, [c_void_p, P_f32]),
     "fomo_time_step": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32]),
     "fomo_run_time_loop": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32]),
+    "fomo_run_time_loop_prepare": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "fomo_frames_create": (c_int32, [c_void_p, c_int32, c_int32, ctypes.POINTER(c_int32), c_int32,
                                      c_int32, c_int32, c_int32, ctypes.POINTER(c_void_p)]),
     "fomo_frames_destroy": (c_int32, [c_void_p]),

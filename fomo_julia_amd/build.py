@@ -1,5 +1,6 @@
 """Builds libfomo_mi355x.so in-tree with hipcc for gfx950, and beside it the
-bandwidth-probe library libfomo_probe.so that bench.py's --pattern-probe loads.
+bandwidth-probe library libfomo_probe.so that bench.py's --pattern-probe loads
+(measurement only: its failure to build does not fail the product build).
 
     python -m fomo_julia_amd.build
 
@@ -54,9 +55,20 @@ def _compile(path: str, sources: list[str], verbose: bool) -> None:
 
 
 def build(verbose: bool = False) -> str:
+    """Compiles the product library.  The bandwidth probe is measurement code:
+    it is compiled afterwards, and a failure there is reported, not raised."""
     _compile(LIB_PATH, SOURCES, verbose)
-    _compile(PROBE_PATH, PROBE_SOURCES, verbose)
+    try:
+        build_probe(verbose)
+    except (subprocess.CalledProcessError, OSError) as e:
+        print(f"warning: {PROBE_NAME} (bench.py --pattern-probe only) did not build: {e}",
+              file=sys.stderr, flush=True)
     return LIB_PATH
+
+
+def build_probe(verbose: bool = False) -> str:
+    _compile(PROBE_PATH, PROBE_SOURCES, verbose)
+    return PROBE_PATH
 
 
 if __name__ == "__main__":

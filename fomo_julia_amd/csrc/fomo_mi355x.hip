@@ -63,6 +63,7 @@ struct fomo_model {
   int nxi = 0, nzi = 0;
   float a[MAX_M] = {};
   int64_t bytes = 0;
+  int pool_refs = 0;  // pool members on this model (destroy is refused while > 0)
 };
 
 struct fomo_shot;
@@ -86,6 +87,7 @@ struct fomo_wavefield {
   // one-shot batches cached per wavefield: [0] placeholder, [1] last real shot
   fomo_batch* single[2] = {nullptr, nullptr};
   uint64_t single_id[2] = {0, 0};
+  int pool_refs = 0;  // 1 while in a pool (destroy is refused)
 };
 
 struct fomo_shot {
@@ -99,6 +101,7 @@ struct fomo_shot {
   float src_scale = 0.f;
   float fsc[5] = {1, 1, 1, 1, 1};  // field store scales 2^e (FP16 storage)
   uint64_t id = 0;  // unique per shot object, keys the cached batches
+  int pool_refs = 0;  // 1 while in a pool (destroy is refused)
 };
 
 static uint64_t g_next_shot_id = 1;
@@ -647,6 +650,8 @@ extern "C" int32_t fomo_model_create_from_velocity(fomo_ctx* c, const fomo_model
 extern "C" int32_t fomo_model_destroy(fomo_model* m) {
   API_LOCK;
   if (!m) return FOMO_OK;
+  if (m->pool_refs > 0)
+    return fail(m->ctx, FOMO_ERR_STATE, "model still has pool members: remove them first");
   (void)hipSetDevice(m->ctx->device);
   (void)hipStreamSynchronize(m->ctx->stream);
   (void)hipFree(m->dcoef);
@@ -674,7 +679,9 @@ extern "C" int32_t fomo_wavefield_create(fomo_model* m, fomo_wavefield** out) {
   const size_t bytes = (f32 ? 15 : 10) * m->fstride * m->fes;
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipMalloc(&w->dbuf, bytes));
-  HIP_TRY(c, hipMemset(w->dbuf, 0, bytes));
+  // zeroed on the context's stream: single-shot runs and batches use that
+  // stream, and a pool orders its stream after it when the member joins
+  HIP_TRY(c, hipMemsetAsync(w->dbuf, 0, bytes, c->stream));
   for (int k = 0; k < 5; ++k) {
     w->buf[0][k] = block_origin(w->dbuf, m->fstride, k, m->pitch, m->fes);
     w->buf[1][k] = block_origin(w->dbuf, m->fstride, 5 + k, m->pitch, m->fes);
@@ -689,6 +696,8 @@ extern "C" int32_t fomo_wavefield_create(fomo_model* m, fomo_wavefield** out) {
 extern "C" int32_t fomo_wavefield_destroy(fomo_wavefield* w) {
   API_LOCK;
   if (!w) return FOMO_OK;
+  if (w->pool_refs > 0)
+    return fail(w->m->ctx, FOMO_ERR_STATE, "wavefield is in a pool: remove its shot first");
   (void)hipSetDevice(w->m->ctx->device);
   (void)hipStreamSynchronize(w->m->ctx->stream);
   drop_single(w);
@@ -875,9 +884,9 @@ extern "C" int32_t fomo_shot_create(fomo_model* m, int32_t src_i, int32_t src_j,
   HIP_TRY(c, hipMemcpy(s->drec, rec.data(), rec.size() * sizeof(int), hipMemcpyHostToDevice));
   const size_t gbytes = (size_t)nt * std::max(n_rec, 1) * sizeof(float);
   HIP_TRY(c, hipMalloc(&s->dgather, gbytes));
-  HIP_TRY(c, hipMemset(s->dgather, 0, gbytes));
+  HIP_TRY(c, hipMemsetAsync(s->dgather, 0, gbytes, c->stream));
   HIP_TRY(c, hipMalloc(&s->dk, sizeof(int)));
-  HIP_TRY(c, hipMemset(s->dk, 0, sizeof(int)));
+  HIP_TRY(c, hipMemsetAsync(s->dk, 0, sizeof(int), c->stream));
   *out = s.release();
   return FOMO_OK;
 }
@@ -891,6 +900,7 @@ extern "C" int32_t fomo_shot_field_scales(const fomo_shot* s, float* out) {
 extern "C" int32_t fomo_shot_destroy(fomo_shot* s) {
   API_LOCK;
   if (!s) return FOMO_OK;
+  if (s->pool_refs > 0) return fail(s->m->ctx, FOMO_ERR_STATE, "shot is in a pool: remove it first");
   (void)hipSetDevice(s->m->ctx->device);
   (void)hipStreamSynchronize(s->m->ctx->stream);
   (void)hipFree(s->dwav);
@@ -1243,6 +1253,7 @@ static void drop_single(fomo_wavefield* w) {
 // The one-row batch for (m, w, s); s == NULL selects an idle placeholder shot
 // (no source, no receivers, nt = 1) for the stages that take no shot.
 static int single_batch(fomo_model* m, fomo_wavefield* w, fomo_shot* s, fomo_batch** out) {
+  API_LOCK;  // may allocate (a new batch, the placeholder shot)
   fomo_ctx* c = m->ctx;
   if (w->m != m || (s && s->m != m)) return fail(c, FOMO_ERR_STATE, "wavefield/shot of another model");
   if (!s) {
@@ -1540,6 +1551,27 @@ extern "C" int32_t fomo_run_time_loop(fomo_model* m, fomo_wavefield* w, fomo_sho
   return run_batch(b, k_first, n_steps);
 }
 
+// Captures the step graphs fomo_run_time_loop replays for (m, w, s), for
+// both buffer parities of the fused step, without running a step: the first
+// run then starts replaying at once instead of capturing inside it.
+extern "C" int32_t fomo_run_time_loop_prepare(fomo_model* m, fomo_wavefield* w, fomo_shot* s) {
+  if (!m || !w || !s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_batch* b = nullptr;
+  int rc = single_batch(m, w, s, &b);
+  if (rc) return rc;
+  set_frames(b, nullptr);
+  HIP_TRY(m->ctx, hipSetDevice(m->ctx->device));
+  const int parities = b->step_mode == 1 ? 2 : 1;
+  for (int q = 0; q < parities; ++q) {
+    const int par = (w->cur + q) & 1;
+    if (!b->gexec[par]) {
+      rc = capture_chunk(b, par);
+      if (rc) return rc;
+    }
+  }
+  return FOMO_OK;
+}
+
 // ------------------------------------------------------------------ frames
 extern "C" int32_t fomo_frames_create(fomo_ctx* c, int32_t nx, int32_t nz, const int32_t* fields,
                                       int32_t n_fields, int32_t skip, int32_t downsample,
@@ -1569,7 +1601,7 @@ extern "C" int32_t fomo_frames_create(fomo_ctx* c, int32_t nx, int32_t nz, const
   const size_t bytes = (size_t)capacity * n_fields * a.nx_d * a.nz_d * sizeof(float);
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipMalloc(&a.out, bytes));
-  HIP_TRY(c, hipMemset(a.out, 0, bytes));
+  HIP_TRY(c, hipMemsetAsync(a.out, 0, bytes, c->stream));
   f->bytes = (int64_t)bytes;
   f->id = g_next_frames_id++;
   *out = f.release();
@@ -1792,17 +1824,24 @@ struct fomo_pool {
   ShotDev* h_rows = nullptr;  // pinned staging: [2][capacity] fused-table rows
   int* h_prefix = nullptr;    // pinned staging: [PART_COUNT][capacity+1], then the live count
   hipEvent_t uploaded = nullptr;
+  hipEvent_t joined = nullptr;  // the context's stream at the last join
   bool dirty = false;
 };
 
 static void pool_free(fomo_pool* p) {
   if (!p) return;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (size_t i = 0; i < p->ss.size(); ++i) {  // members still in the pool leave it
+    --p->ms[i]->pool_refs;
+    --p->ws[i]->pool_refs;
+    --p->ss[i]->pool_refs;
+  }
   if (p->b) destroy_batch(p->b);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   if (p->h_rows) (void)hipHostFree(p->h_rows);
   if (p->h_prefix) (void)hipHostFree(p->h_prefix);
   if (p->uploaded) (void)hipEventDestroy(p->uploaded);
+  if (p->joined) (void)hipEventDestroy(p->joined);
   delete p;
 }
 
@@ -1852,17 +1891,18 @@ extern "C" int32_t fomo_pool_create(fomo_ctx* c, int32_t fd_order, int32_t stora
   b->grid_fixed[PART_FSREC] = cus;
   b->hs.assign((size_t)TBL_COUNT * capacity, ShotDev{});
   b->hprefix.assign((size_t)PART_COUNT * (capacity + 1), 0);
-  HIP_TRY(c, hipMalloc(&b->ds, b->hs.size() * sizeof(ShotDev)));
-  HIP_TRY(c, hipMemset(b->ds, 0, b->hs.size() * sizeof(ShotDev)));
-  HIP_TRY(c, hipMalloc(&b->dn, sizeof(int)));
-  HIP_TRY(c, hipMemset(b->dn, 0, sizeof(int)));
-  HIP_TRY(c, hipMalloc(&b->dprefix, b->hprefix.size() * sizeof(int)));
-  HIP_TRY(c, hipMemset(b->dprefix, 0, b->hprefix.size() * sizeof(int)));
-  HIP_TRY(c, hipHostMalloc((void**)&p->h_rows, 2 * (size_t)capacity * sizeof(ShotDev)));
-  HIP_TRY(c, hipHostMalloc((void**)&p->h_prefix, (b->hprefix.size() + 1) * sizeof(int)));
   HIP_TRY(c, hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
   b->ls = p->stream;
+  HIP_TRY(c, hipMalloc(&b->ds, b->hs.size() * sizeof(ShotDev)));
+  HIP_TRY(c, hipMemsetAsync(b->ds, 0, b->hs.size() * sizeof(ShotDev), p->stream));
+  HIP_TRY(c, hipMalloc(&b->dn, sizeof(int)));
+  HIP_TRY(c, hipMemsetAsync(b->dn, 0, sizeof(int), p->stream));
+  HIP_TRY(c, hipMalloc(&b->dprefix, b->hprefix.size() * sizeof(int)));
+  HIP_TRY(c, hipMemsetAsync(b->dprefix, 0, b->hprefix.size() * sizeof(int), p->stream));
+  HIP_TRY(c, hipHostMalloc((void**)&p->h_rows, 2 * (size_t)capacity * sizeof(ShotDev)));
+  HIP_TRY(c, hipHostMalloc((void**)&p->h_prefix, (b->hprefix.size() + 1) * sizeof(int)));
   HIP_TRY(c, hipEventCreateWithFlags(&p->uploaded, hipEventDisableTiming));
+  HIP_TRY(c, hipEventCreateWithFlags(&p->joined, hipEventDisableTiming));
   HIP_TRY(c, hipEventRecord(p->uploaded, p->stream));
   *out = p.release();
   return FOMO_OK;
@@ -1884,6 +1924,7 @@ static int pool_find(const fomo_pool* p, const fomo_shot* s) {
 }
 
 extern "C" int32_t fomo_pool_add(fomo_pool* p, fomo_model* m, fomo_wavefield* w, fomo_shot* s) {
+  API_LOCK;
   if (!p || !m || !w || !s) return fail(p ? p->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   fomo_ctx* c = p->ctx;
   if (m->ctx != c || w->m != m || s->m != m)
@@ -1895,9 +1936,8 @@ extern "C" int32_t fomo_pool_add(fomo_pool* p, fomo_model* m, fomo_wavefield* w,
   if (fb >= 0x7FFFFFFF || cb >= 0x7FFFFFFF)
     return fail(c, FOMO_ERR_UNSUPPORTED, "grid too large for the fused step");
   if ((int)p->ss.size() >= p->capacity) return fail(c, FOMO_ERR_ARG, "pool is full");
-  for (size_t i = 0; i < p->ss.size(); ++i)
-    if (p->ws[i] == w || p->ss[i] == s)
-      return fail(c, FOMO_ERR_ARG, "wavefield or shot already in the pool");
+  if (w->pool_refs > 0 || s->pool_refs > 0)
+    return fail(c, FOMO_ERR_ARG, "wavefield or shot already in a pool");
   int rc = adopt_scales(w, s);
   if (rc) return rc;
   HIP_TRY(c, hipSetDevice(c->device));
@@ -1918,7 +1958,15 @@ extern "C" int32_t fomo_pool_add(fomo_pool* p, fomo_model* m, fomo_wavefield* w,
     if (to_stored) b->compact = false;
     if (wider) b->habc_rows = m->d.nbc + 1;
   }
+  // The member's zeroing (creation, fomo_wavefield_reset, fomo_shot_reset)
+  // and any earlier run were queued on the context's stream: the pool's
+  // stream waits for them before the member's first step.
+  HIP_TRY(c, hipEventRecord(p->joined, c->stream));
+  HIP_TRY(c, hipStreamWaitEvent(p->stream, p->joined, 0));
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, p->stream, s->dk, 1);
+  ++m->pool_refs;
+  ++w->pool_refs;
+  ++s->pool_refs;
   p->ms.push_back(m);
   p->ws.push_back(w);
   p->ss.push_back(s);
@@ -1929,6 +1977,7 @@ extern "C" int32_t fomo_pool_add(fomo_pool* p, fomo_model* m, fomo_wavefield* w,
 }
 
 extern "C" int32_t fomo_pool_remove(fomo_pool* p, fomo_shot* s) {
+  API_LOCK;
   if (!p || !s) return fail(p ? p->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   const int i = pool_find(p, s);
   if (i < 0) return fail(p->ctx, FOMO_ERR_ARG, "shot is not in the pool");
@@ -1938,6 +1987,9 @@ extern "C" int32_t fomo_pool_remove(fomo_pool* p, fomo_shot* s) {
   fomo_wavefield* w = p->ws[i];
   w->cur = p->c0[i] ^ (done & 1);
   w->backup_is_prev = done > 0;
+  --p->ms[i]->pool_refs;
+  --w->pool_refs;
+  --s->pool_refs;
   p->ms.erase(p->ms.begin() + i);
   p->ws.erase(p->ws.begin() + i);
   p->ss.erase(p->ss.begin() + i);
@@ -1947,7 +1999,10 @@ extern "C" int32_t fomo_pool_remove(fomo_pool* p, fomo_shot* s) {
   return FOMO_OK;
 }
 
-extern "C" int32_t fomo_pool_size(const fomo_pool* p) { return p ? (int32_t)p->ss.size() : 0; }
+extern "C" int32_t fomo_pool_size(const fomo_pool* p) {
+  API_LOCK;
+  return p ? (int32_t)p->ss.size() : 0;
+}
 
 extern "C" int32_t fomo_pool_synchronize(fomo_pool* p) {
   if (!p) return fail(nullptr, FOMO_ERR_ARG, "NULL pool");
@@ -1957,6 +2012,7 @@ extern "C" int32_t fomo_pool_synchronize(fomo_pool* p) {
 }
 
 extern "C" int32_t fomo_pool_next_step(const fomo_pool* p, const fomo_shot* s, int32_t* k_next) {
+  API_LOCK;
   if (!p || !s || !k_next) return fail(p ? p->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
   const int i = pool_find(p, s);
   if (i < 0) return fail(p->ctx, FOMO_ERR_ARG, "shot is not in the pool");
@@ -2010,6 +2066,7 @@ static int pool_upload(fomo_pool* p) {
 }
 
 extern "C" int32_t fomo_pool_run(fomo_pool* p, int32_t n_steps) {
+  API_LOCK;
   if (!p) return fail(nullptr, FOMO_ERR_ARG, "NULL pool");
   fomo_ctx* c = p->ctx;
   fomo_batch* b = p->b;

@@ -365,10 +365,15 @@ class _DeviceModel:
         b.check(rc, "fomo_model_create")
         self.h = h
 
+    def close(self) -> None:
+        """Frees the device model now (refused while a pool member uses it)."""
+        if getattr(self, "h", None):
+            _lib.check(_lib.load().fomo_model_destroy(self.h), self.b._ctx, "fomo_model_destroy")
+            self.h = None
+
     def __del__(self):
         try:
-            if getattr(self, "h", None):
-                _lib.load().fomo_model_destroy(self.h)
+            self.close()
         except Exception:
             pass
 
@@ -413,10 +418,15 @@ class _DeviceShot:
             self.dm.b.check(self.dm.b.lib.fomo_shot_gather(self.h, _f32ptr(out)), "fomo_shot_gather")
         return out
 
+    def close(self) -> None:
+        """Frees the device shot now (refused while it is in a pool)."""
+        if getattr(self, "h", None):
+            _lib.check(_lib.load().fomo_shot_destroy(self.h), self.dm.b._ctx, "fomo_shot_destroy")
+            self.h = None
+
     def __del__(self):
         try:
-            if getattr(self, "h", None):
-                _lib.load().fomo_shot_destroy(self.h)
+            self.close()
         except Exception:
             pass
 
@@ -455,9 +465,14 @@ class Wavefield:
 
     def _free(self):
         if self._h is not None:
-            _lib.load().fomo_wavefield_destroy(self._h)
+            _lib.check(_lib.load().fomo_wavefield_destroy(self._h), self.b._ctx,
+                       "fomo_wavefield_destroy")
             self._h = None
             self._dm = None
+
+    def close(self) -> None:
+        """Frees the device state now (refused while the wavefield is in a pool)."""
+        self._free()
 
     def __del__(self):
         try:

@@ -17,14 +17,21 @@ the same per-shot objects (medium, HABC, source, receivers, gather):
 
 Every shot's arithmetic is the same as alone (the kernels treat each shot
 of a pool independently), so a result is identical alone and in the mix.
+
+Threads: worker threads prepare requests and create their device objects;
+the thread that calls ``Server.serve`` owns the pools.  Every pool call, and
+the freeing of a finished shot's device objects, happens on that thread
+(``Pool`` refuses calls from any other).  A request whose preparation or
+device objects fail is reported in ``Server.failed`` and the loop goes on.
 """
 from __future__ import annotations
 
 import ctypes
 import math
+import threading
 import time
 from collections import deque
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
 from typing import Optional
 
@@ -70,6 +77,14 @@ class Prepared:
     rec: E.Receivers
     bytes: int
     prep_s: float
+
+
+@dataclass
+class Failed:
+    rid: int
+    arrival: float
+    stage: str   # "prepare" or "admit"
+    error: str
 
 
 @dataclass
@@ -191,10 +206,12 @@ class _Tiling:
 
 
 class Pool:
-    """fomo_pool: continuous batch of one FD order and storage type."""
+    """fomo_pool: continuous batch of one FD order and storage type.  Its
+    owner is the thread that creates it; calls from any other thread raise."""
 
     def __init__(self, b: E.MI355XBackend, fd_order: int, storage: str, capacity: int):
         self.b = b
+        self.owner = threading.get_ident()
         self.key = (fd_order, storage)
         h = ctypes.c_void_p()
         with _Tiling(b, fd_order):  # the pool keeps the shape current at its creation
@@ -204,24 +221,40 @@ class Pool:
         self.capacity = capacity
         self.members: dict = {}  # rid -> _Member
 
+    def _own(self, what: str) -> None:
+        if threading.get_ident() != self.owner:
+            raise RuntimeError(f"Pool.{what} called from a thread that does not own the pool")
+
     def add(self, m) -> None:
+        self._own("add")
         self.b.check(self.b.lib.fomo_pool_add(self.h, m.dm.h, m.W._h, m.ds.h), "fomo_pool_add")
         self.members[m.rid] = m
 
     def remove(self, m) -> None:
+        self._own("remove")
         self.b.check(self.b.lib.fomo_pool_remove(self.h, m.ds.h), "fomo_pool_remove")
         del self.members[m.rid]
 
     def run(self, n_steps: int) -> None:
+        self._own("run")
         self.b.check(self.b.lib.fomo_pool_run(self.h, n_steps), "fomo_pool_run")
 
     def synchronize(self) -> None:
+        self._own("synchronize")
         self.b.check(self.b.lib.fomo_pool_synchronize(self.h), "fomo_pool_synchronize")
 
     def next_step(self, m) -> int:
+        self._own("next_step")
         k = ctypes.c_int32()
         self.b.check(self.b.lib.fomo_pool_next_step(self.h, m.ds.h, ctypes.byref(k)), "next_step")
         return k.value
+
+    def close(self) -> None:
+        """Destroys the pool (its members leave it; their objects stay)."""
+        self._own("close")
+        if getattr(self, "h", None):
+            _lib.load().fomo_pool_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
@@ -232,15 +265,29 @@ class Pool:
 
 
 class _Member:
+    """A request's device objects: model, wavefield and shot.  Built on a
+    worker thread; freed with close() on the pools' owner thread."""
+
     def __init__(self, b: E.MI355XBackend, prep: Prepared, admitted: float):
         self.rid = prep.req.rid
         self.prep = prep
         self.admitted = admitted
-        self.dm = E._DeviceModel(b, prep.M, prep.H, prep.a, prep.p)
-        self.W = E.Wavefield(prep.M.nx, prep.M.nz, b)
-        self.W._bind(self.dm)
-        self.ds = E._DeviceShot(self.dm, prep.src, prep.rec, prep.p.nt)
+        self.dm = self.W = self.ds = None
+        try:
+            self.dm = E._DeviceModel(b, prep.M, prep.H, prep.a, prep.p)
+            self.W = E.Wavefield(prep.M.nx, prep.M.nz, b)
+            self.W._bind(self.dm)
+            self.ds = E._DeviceShot(self.dm, prep.src, prep.rec, prep.p.nt)
+        except Exception:
+            self.close()
+            raise
         self.nt = prep.p.nt
+
+    def close(self) -> None:
+        for obj in (self.ds, self.W, self.dm):
+            if obj is not None:
+                obj.close()
+        self.dm = self.W = self.ds = None
 
 
 class Server:
@@ -259,6 +306,7 @@ class Server:
         self.capacity = capacity
         self.workers = workers
         self.pools: dict = {}
+        self.failed: list = []  # Failed records of requests that could not run
         # seconds spent: preparing requests (worker threads, summed), creating
         # device objects, waiting for the GPU; chunks launched
         self.stats = {"prepare_s": 0.0, "admit_s": 0.0, "gpu_wait_s": 0.0, "chunks": 0,
@@ -300,7 +348,11 @@ class Server:
             while waiting:
                 r, fut = waiting.popleft()
                 if fut.done():
-                    prep = fut.result()
+                    try:
+                        prep = fut.result()
+                    except Exception as e:  # this request cannot run; the others go on
+                        self.failed.append(Failed(r.rid, r.arrival, "prepare", repr(e)))
+                        continue
                     key = (r.fd_order, r.storage)
                     pool = self._pool(key)
                     fits = used + prep.bytes <= self.budget or not (active or creating)
@@ -308,21 +360,27 @@ class Server:
                         used += prep.bytes
                         slots[key] = slots.get(key, 0) + 1
                         st["prepare_s"] += prep.prep_s
-                        creating.append((pool, ex.submit(make_member, prep, now())))
+                        creating.append((pool, prep, ex.submit(make_member, prep, now())))
                         continue
                 keep.append((r, fut))
             waiting = keep
             # members whose device objects are ready join their pools
             still = deque()
             while creating:
-                pool, fut = creating.popleft()
+                pool, prep, fut = creating.popleft()
                 if fut.done():
-                    m = fut.result()
+                    try:
+                        m = fut.result()
+                    except Exception as e:  # device objects failed: release the reservation
+                        used -= prep.bytes
+                        slots[pool.key] -= 1
+                        self.failed.append(Failed(prep.req.rid, prep.req.arrival, "admit", repr(e)))
+                        continue
                     st["admit_s"] += m.create_s
                     pool.add(m)
                     active[m.rid] = (pool, m)
                 else:
-                    still.append((pool, fut))
+                    still.append((pool, prep, fut))
             creating.extend(still)
             st["max_resident"] = max(st["max_resident"], len(active))
 
@@ -339,9 +397,9 @@ class Server:
                     admit(ex)
                     if not active:
                         if creating:
-                            creating[0][1].result()  # wait for device objects
+                            wait([creating[0][2]])  # device objects being built
                         elif waiting:
-                            waiting[0][1].result()  # wait for a preparation
+                            wait([waiting[0][1]])  # a preparation
                         elif pending and realtime:
                             time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
                         continue
@@ -363,6 +421,7 @@ class Server:
                 for pool, m in finished:
                     pool.remove(m)
                     g = m.ds.gather()
+                    m.close()  # the device objects are freed here, on the pools' thread
                     c = Completed(m.rid, m.prep.req.arrival, m.admitted, t_done, g,
                                   m.prep.req.storage, m.prep.req.fd_order,
                                   m.prep.M.nx * m.prep.M.nz * m.nt)
@@ -376,9 +435,12 @@ class Server:
         return done
 
 
-def serial(requests: list, b: Optional[E.MI355XBackend] = None, storage: Optional[str] = "f32") -> list:
+def serial(requests: list, b: Optional[E.MI355XBackend] = None, storage: Optional[str] = "f32",
+           prep_times: Optional[list] = None) -> list:
     """The same requests one at a time with the plain single-shot path
-    (``run_shot``), storage forced to `storage` (None keeps each request's)."""
+    (``run_shot``), storage forced to `storage` (None keeps each request's).
+    Each request's host preparation runs in the loop, as a client's would;
+    its seconds are appended to `prep_times` when given."""
     b = b or E.backend("mi355x")
     out = []
     t0 = time.perf_counter()
@@ -386,6 +448,8 @@ def serial(requests: list, b: Optional[E.MI355XBackend] = None, storage: Optiona
         rr = ShotRequest(**{**r.__dict__, "storage": storage or r.storage})
         ta = time.perf_counter() - t0
         prep = prepare(rr)
+        if prep_times is not None:
+            prep_times.append(prep.prep_s)
         W = E.Wavefield(prep.M.nx, prep.M.nz, b)
         with _Tiling(b, rr.fd_order):
             res = E.run_shot(b, W, prep.M, prep.H, prep.a, prep.src, prep.rec, prep.p)

@@ -27,8 +27,19 @@ def _key(dms, n: int, storage: str) -> tuple:
     return (dms[0].p.fd_order, storage, n, round(cells, -5))
 
 
+def trial_steps(nt: int, steps: int, warm: int) -> int:
+    """Timed steps of a trial on members of `nt` steps: the warm-up and timed
+    steps (1 .. warm + steps) stay inside every member's nt, since a kernel
+    launched past a shot's last step returns at once; even, at least 2."""
+    steps = min(steps, nt - warm) // 2 * 2
+    if steps < 2:
+        raise ValueError(f"members of {nt} steps are too short to time {warm} + 2 steps")
+    return steps
+
+
 def time_batch(b: E.MI355XBackend, dms, wfs, dss, steps: int, warm: int = 4) -> float:
     """ms per step of one batch over `steps` steps (HIP events on the library's stream)."""
+    steps = trial_steps(min(int(d.nt) for d in dss), steps, warm)
     lib = b.lib
     n = len(wfs)
     arr = lambda hs: (ctypes.c_void_p * n)(*[h.value for h in hs])

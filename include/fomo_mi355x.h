@@ -15,7 +15,10 @@
  *  - Grid indices passed in (source, receivers) are 1-based, as in Julia.
  *  - Host pointers are borrowed for the duration of the call only.
  *  - A context owns one HIP stream; all work is queued on it in call order.
- *    A context and its objects are used by one host thread at a time.
+ *    Runs on one context are driven by one host thread at a time.  Calls that
+ *    allocate, free or capture graphs, and the pool calls, hold one library
+ *    lock, so other threads may create and destroy models, wavefields and
+ *    shots while that thread runs (the serving loop does).
  *  - Device memory: one model (medium) is shared by any number of
  *    wavefields and shots; a shot holds its wavelet, receivers and gather.
  */
@@ -174,6 +177,11 @@ int32_t fomo_time_step(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t k
  * fomo_ctx_synchronize). */
 int32_t fomo_run_time_loop(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t k_first,
                            int32_t n_steps);
+/* Captures the graphs fomo_run_time_loop replays for (m, w, s) without
+ * running a step, so that a later run (a timed one, or the first of a
+ * serving loop) does not capture inside it.  Part of run_time_loop!'s job
+ * (time_stepper.jl:97-128); the reference has no separate call for it. */
+int32_t fomo_run_time_loop_prepare(fomo_model* m, fomo_wavefield* w, fomo_shot* s);
 
 /* ---- wavefield frames: the on_step video callback without leaving the graphs
  * MultiFieldRecorder (video_recorder.jl:113-148,207-228) called as on_step
@@ -247,7 +255,13 @@ int32_t fomo_batch_run(fomo_batch* b, int32_t k_first, int32_t n_steps);
  * number of steps; one captured graph per pool serves every membership.
  * Each pool has its own stream, so the runs of different pools (other FD
  * orders or storage types) proceed side by side on the GPU; wait for a pool
- * with fomo_pool_synchronize (fomo_ctx_synchronize does not). */
+ * with fomo_pool_synchronize (fomo_ctx_synchronize does not).
+ * Ordering and threads: a joining member's zeroing and earlier work, queued
+ * on the context's stream, complete before its first pool step.  Pool calls
+ * hold the library lock that every allocating or freeing call takes, so
+ * other threads may create and destroy objects meanwhile; a model, wavefield
+ * or shot that is in a pool cannot be destroyed (FOMO_ERR_STATE) until it is
+ * removed or the pool is destroyed. */
 typedef struct fomo_pool fomo_pool;
 int32_t fomo_pool_create(fomo_ctx* ctx, int32_t fd_order, int32_t storage, int32_t capacity,
                          fomo_pool** out);

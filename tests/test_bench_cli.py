@@ -63,3 +63,44 @@ def test_edges_line_is_self_consistent():
     assert f["overhead"] == pytest.approx(f["ms_per_step_with_frames"] / f["ms_per_step_plain"] - 1, abs=2e-3)
     assert f["frame_shape"] == [-(-3927 // f["downsample"]), -(-4552 // f["downsample"])]
     assert f["last_frame_finite_nonzero"]
+
+
+def test_timed_kernels_run_at_a_step_inside_their_shot():
+    """Every per-kernel timing launch runs at a step k with 1 <= k <= nt of
+    the shot it runs on (a kernel launched past its shot's last step returns
+    at once), and the autotuner's warm-up and timed steps fit the timing shot."""
+    from fomo_julia_amd import tune as T
+    for nt in (1, 2, 5, 20, 31, 32, 33, 64, 11520):
+        k = bench.stage_step(nt)
+        assert 1 <= k <= nt
+    assert bench.stage_step(bench.TIMING_NT) == bench.STAGE_STEP
+    with pytest.raises(ValueError):
+        bench.stage_step(0)
+    # the autotuner's defaults (4 warm-up steps, 32 timed) inside the timing shot
+    steps = T.trial_steps(bench.TIMING_NT, 32, 4)
+    assert steps == 32 and 4 + steps <= bench.TIMING_NT
+    # short members: the trial shrinks to what the shots hold, or is refused
+    assert T.trial_steps(20, 32, 4) == 16
+    assert T.trial_steps(9, 32, 4) == 4
+    for nt in (1, 4, 5):
+        with pytest.raises(ValueError):
+            T.trial_steps(nt, 32, 4)
+    for nt in range(6, 200):
+        s = T.trial_steps(nt, 32, 4)
+        assert s >= 2 and s % 2 == 0 and 4 + s <= nt
+
+
+def test_roofline_check_refuses_idle_kernels():
+    bench.check_roofline(0.5, 0.233, 0.2955)
+    bench.check_roofline(1.0, 0.2, 0.2)
+    for frac, kms, step in ((47.5, 0.00245, 0.3018), (0.0, 0.2, 0.3), (-0.1, 0.2, 0.3),
+                            (0.5, 0.31, 0.30), (0.5, 0.0, 0.3)):
+        with pytest.raises(SystemExit):
+            bench.check_roofline(frac, kms, step)
+
+
+def test_cpu_share_is_the_usable_cpus():
+    s = bench.cpu_share()
+    assert 1 <= s["usable"] <= s["affinity"] <= (s["host_cpus"] or s["affinity"])
+    if s["cgroup_quota"] is not None:
+        assert s["usable"] <= max(1, int(s["cgroup_quota"] + 0.999))

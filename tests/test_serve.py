@@ -89,3 +89,98 @@ def test_device_bytes_estimate_matches_library(b):
         got = lib.fomo_model_device_bytes(dm.h) + lib.fomo_wavefield_device_bytes(W._h)
         gather = r.nt * len(p.rec.i) * 4
         assert abs(p.bytes - (got + gather)) <= 3 * 4096 + 4 * (p.M.nz + 36 + 8), storage
+
+
+def _member_objects(b, r):
+    from fomo_julia_amd import elastic2d as E
+    p = S.prepare(r)
+    dm = E._DeviceModel(b, p.M, p.H, p.a, p.p)
+    W = E.Wavefield(p.M.nx, p.M.nz, b)
+    W._bind(dm)
+    ds = E._DeviceShot(dm, p.src, p.rec, p.p.nt)
+    return p, dm, W, ds
+
+
+@gpu
+def test_reused_objects_join_after_reset_on_the_context_stream(b):
+    """run alone, reset (queued on the context's stream), add to a pool, run:
+    the pool's first steps wait for the zeroing, so the gather equals the
+    solo run's bit for bit."""
+    from fomo_julia_amd import _lib
+    r = S.c5_stream(1, seed=41, orders=(8,), storages=("f32",), **SMALL)[0]
+    print("case reset-then-join", r, flush=True)
+    p, dm, W, ds = _member_objects(b, r)
+    lib = b.lib
+    with S._Tiling(b, r.fd_order):
+        b.check(lib.fomo_run_time_loop(dm.h, W._h, ds.h, 1, p.p.nt), "solo")
+        b.synchronize()
+        solo = ds.gather()
+        pool = S.Pool(b, r.fd_order, r.storage, 4)
+    for _ in range(2):
+        b.check(lib.fomo_wavefield_reset(W._h), "reset")
+        b.check(lib.fomo_shot_reset(ds.h), "shot reset")
+        b.check(lib.fomo_pool_add(pool.h, dm.h, W._h, ds.h), "add")
+        steps = (p.p.nt + 1) // 2 * 2
+        pool.run(steps)
+        pool.synchronize()
+        b.check(lib.fomo_pool_remove(pool.h, ds.h), "remove")
+        assert np.array_equal(ds.gather(), solo)
+    pool.close()
+    for obj in (ds, W, dm):
+        obj.close()
+
+
+@gpu
+def test_pool_runs_while_other_threads_create_and_free_objects(b):
+    """Worker threads create and destroy models, wavefields and shots while
+    the owner thread runs a pool: the library lock keeps allocations and
+    frees out of the pool's graph capture, a pool member cannot be destroyed
+    while in the pool, and every served gather equals its solo run."""
+    import threading
+    from fomo_julia_amd import _lib
+    reqs = S.c5_stream(3, seed=52, orders=(8,), storages=("f32",), **SMALL)
+    print("case frees from worker threads", [(r.nx, r.nz, r.nt) for r in reqs], flush=True)
+    members = [_member_objects(b, r) for r in reqs]
+    with S._Tiling(b, 8):
+        pool = S.Pool(b, 8, "f32", 4)
+    for p, dm, W, ds in members:
+        b.check(b.lib.fomo_pool_add(pool.h, dm.h, W._h, ds.h), "add")
+    # a member in the pool cannot be destroyed
+    assert b.lib.fomo_shot_destroy(members[0][3].h) == _lib.FOMO_ERR_STATE
+    assert b.lib.fomo_wavefield_destroy(members[0][2]._h) == _lib.FOMO_ERR_STATE
+    assert b.lib.fomo_model_destroy(members[0][1].h) == _lib.FOMO_ERR_STATE
+    stop = threading.Event()
+    churn = {"n": 0, "err": None}
+    spare = S.c5_stream(1, seed=60, orders=(4,), storages=("f32",), **SMALL)[0]
+
+    def worker():
+        try:
+            while not stop.is_set():
+                objs = _member_objects(b, spare)
+                for obj in (objs[3], objs[2], objs[1]):
+                    obj.close()
+                churn["n"] += 1
+        except Exception as e:  # reported to the owner thread
+            churn["err"] = e
+
+    th = [threading.Thread(target=worker) for _ in range(2)]
+    for t in th:
+        t.start()
+    nt = max(r.nt for r in reqs)
+    k = 0
+    while k < nt:
+        pool.run(16)
+        pool.synchronize()
+        k += 16
+    stop.set()
+    for t in th:
+        t.join()
+    assert churn["err"] is None, churn["err"]
+    assert churn["n"] > 0
+    for (p, dm, W, ds), r in zip(members, reqs):
+        b.check(b.lib.fomo_pool_remove(pool.h, ds.h), "remove")
+        assert np.array_equal(ds.gather(), _alone(b, r)), r.rid
+    pool.close()
+    for p, dm, W, ds in members:
+        for obj in (ds, W, dm):
+            obj.close()

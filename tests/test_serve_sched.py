@@ -7,7 +7,12 @@ checked on its own:
 - device memory of the resident shots stays within the budget, except for a
   lone request larger than the budget (admitted when nothing else runs);
 - requests of different (fd_order, storage) go to different pools;
-- with real-time arrivals nothing is admitted before it arrives."""
+- with real-time arrivals nothing is admitted before it arrives;
+- threads: members are built on worker threads while the pools run, but
+  every pool call and every free of a member's device objects happens on
+  the serving thread; a request whose preparation or device objects fail is
+  reported and releases its budget, and the others complete."""
+import threading
 import time
 from types import SimpleNamespace
 
@@ -19,6 +24,11 @@ from fomo_julia_amd import serve as S
 
 class _World:
     def __init__(self):
+        self.pool_threads = set()     # threads that made pool calls
+        self.create_threads = set()   # threads that built members
+        self.close_threads = []       # (rid, thread) of every member close
+        self.fail_prepare = set()     # rids whose preparation raises
+        self.fail_create = set()      # rids whose device objects raise
         self.resident = {}   # rid -> bytes
         self.max_resident_bytes = 0
         self.pools = []
@@ -36,6 +46,7 @@ def _install(monkeypatch, world):
             world.pools.append(self)
 
         def add(self, m):
+            world.pool_threads.add(threading.get_ident())
             assert len(self.members) < self.capacity, "pool over capacity"
             assert m.prep.req.fd_order == self.key[0] and m.prep.req.storage == self.key[1]
             self.members[m.rid] = m
@@ -46,23 +57,30 @@ def _install(monkeypatch, world):
             world.steps_run[m.rid] = 0
 
         def remove(self, m):
+            world.pool_threads.add(threading.get_ident())
             del self.members[m.rid]
             del world.resident[m.rid]
 
         def run(self, n):
+            world.pool_threads.add(threading.get_ident())
             assert n > 0 and n % 2 == 0
             for rid, m in self.members.items():
                 self.k[rid] = min(self.k[rid] + n, m.nt + 1)
                 world.steps_run[rid] += n
 
         def synchronize(self):
-            pass
+            world.pool_threads.add(threading.get_ident())
 
         def next_step(self, m):
+            world.pool_threads.add(threading.get_ident())
             return self.k[m.rid]
 
     class FakeMember:
         def __init__(self, b, prep, admitted):
+            world.create_threads.add(threading.get_ident())
+            time.sleep(0.002)  # device setup takes a while, overlapping the pools' runs
+            if prep.req.rid in world.fail_create:
+                raise RuntimeError("fomo_model_create_from_velocity failed (-2): out of memory")
             self.rid = prep.req.rid
             self.prep = prep
             self.admitted = admitted
@@ -70,7 +88,12 @@ def _install(monkeypatch, world):
             self.ds = SimpleNamespace(gather=lambda rid=self.rid, nt=self.nt:
                                       np.full((nt, 1), rid, np.float32))
 
+        def close(self):
+            world.close_threads.append((self.rid, threading.get_ident()))
+
     def fake_prepare(req):
+        if req.rid in world.fail_prepare:
+            raise ValueError("medium generation failed")
         nb = req.nbc + req.fd_order // 2
         M = SimpleNamespace(nx=req.nx + 2 * nb, nz=req.nz + 2 * nb)
         return SimpleNamespace(req=req, M=M, p=SimpleNamespace(nt=req.nt),
@@ -137,3 +160,47 @@ def test_realtime_arrivals_are_not_admitted_early(monkeypatch):
         assert c.admitted >= c.arrival - 1e-3
     s = S.summarize(done, 1.0)
     assert s["shots"] == 8 and s["latency_p99_s"] >= s["latency_p50_s"] >= 0
+
+
+def test_pools_have_one_owner_and_failures_release_their_budget(monkeypatch):
+    world = _World()
+    _install(monkeypatch, world)
+    reqs = _requests(24, seed=7)
+    world.fail_prepare = {3, 17}
+    world.fail_create = {5, 11, 20}
+    biggest = max(r.nx * r.nz * 100 for r in reqs)
+    budget = 4 * biggest
+    srv = S.Server(b=object(), budget_bytes=budget, chunk=16, capacity=8, workers=4)
+    me = threading.get_ident()
+    done = srv.serve(reqs, realtime=False)
+    failed = {f.rid: f.stage for f in srv.failed}
+    assert failed == {3: "prepare", 17: "prepare", 5: "admit", 11: "admit", 20: "admit"}
+    assert sorted(c.rid for c in done) == sorted(set(range(24)) - set(failed))
+    # one owner: every pool call on the serving thread; members built on workers
+    assert world.pool_threads == {me}
+    assert world.create_threads and me not in world.create_threads
+    # every completed member's device objects were freed, once, on the serving thread
+    assert sorted(r for r, _ in world.close_threads) == sorted(c.rid for c in done)
+    assert {t for _, t in world.close_threads} == {me}
+    # the failed reservations were released: the budget held and nothing stayed resident
+    assert world.max_resident_bytes <= budget and not world.resident
+
+
+def test_real_pool_refuses_calls_from_another_thread():
+    """serve.Pool's owner check runs before any library call."""
+    pool = S.Pool.__new__(S.Pool)
+    pool.owner = threading.get_ident()
+    pool.h = None
+    errors = []
+
+    def other():
+        for call in (lambda: pool.run(16), lambda: pool.synchronize(), lambda: pool.close()):
+            try:
+                call()
+            except RuntimeError as e:
+                errors.append(str(e))
+
+    t = threading.Thread(target=other)
+    t.start()
+    t.join()
+    assert len(errors) == 3 and all("does not own" in e for e in errors)
