@@ -5,8 +5,10 @@ stores), replayed HIP graphs.  Every field starts from a random state, so
 every HABC piece, corner, band item and free-surface row is active from the
 first step; after 20 steps the five fields and the gather equal the oracle's
 bit for bit (the oracle runs the same 20 steps on the host, about a second;
-for BF16 storage it emulates the storage's rounding at every field write and
-its dithered coefficients)."""
+for reduced storage it emulates the storage's rounding at every field write
+and its dithered coefficients).  Each storage of the precision form runs:
+FP32, BF16, FP16 (fields at the shot's power-of-two scales) and FP16
+coefficients with FP32 fields."""
 import numpy as np
 import pytest
 
@@ -27,7 +29,12 @@ def _bf16_rne(v):
     return u.astype(np.uint32).view(F32)
 
 
-@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def _f16_rne(v, scale):
+    """v stored as FP16 at `scale` (x * scale rounded to nearest even), as FP32 values."""
+    return (np.float16(v.astype(np.float64) * scale).astype(np.float64) / scale).astype(F32)
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16", "f16", "f16c"])
 def test_seam_random_state_bit_exact(storage):
     import oracle as O
     dx = F32(12.5)
@@ -51,19 +58,38 @@ def test_seam_random_state_bit_exact(storage):
     b.set_tiling(2, 0, 16)
     b.set_step_mode(1)
     W = E.Wavefield(M.nx, M.nz, b)
-    E.bind(b, W, M, H, a, p)
+    dm = E.bind(b, W, M, H, a, p)
+    scales = np.ones(5, F32)
+    if storage == "f16":
+        # a step from the zero state makes the wavefield adopt the shot's FP16
+        # field scales; the random state below is then stored at those scales
+        ds = E._shot_for(dm, src, rec, NT)
+        b.check(b.lib.fomo_time_step(dm.h, W._h, ds.h, 1), "scale adoption step")
+        b.synchronize()
+        scales = ds.field_scales()
+        assert all(float(x) == 2.0 ** np.round(np.log2(float(x))) for x in scales)
     r = O.OracleRun(nx=M.nx, nz=M.nz, pad=M.pad, nbc=nbc, fd_order=order, free_surface=True,
                     dt=p.dt, dtx=p.dtx, dtz=p.dtz, q=(H.qx, H.qz, H.qt_x, H.qt_z, H.qxt), a=a,
                     rho_vx=M.rho_vx, rho_vz=M.rho_vz, lam=M.lam, mu_txx=M.mu_txx,
                     mu_txz=M.mu_txz, w_vx=H.w_vx, w_vz=H.w_vz, w_tau=H.w_tau, src_i=src.i,
                     src_j=src.j, src_type=SRC_CODES[src.type], wavelet=src.wavelet, rec_i=rec.i,
                     rec_j=rec.j, rec_type=rec.type_codes(), nt=NT, dtype=F32, flags=1,
-                    storage=storage)
+                    storage=storage, fscale=[float(x) for x in scales])
     rng = np.random.default_rng(2026)
     for f in range(5):
         v = rng.standard_normal((M.nx, M.nz), dtype=F32)
+        if storage == "f16":
+            # about 100 in stored units: the 20 steps' stencil growth (the
+            # largest modulus times dt/dx over the smallest impedance, about
+            # 70x) stays inside FP16's range
+            v = (v * F32(100.0 / float(scales[f]))).astype(F32)
         W.upload(f, v)
-        r.field(f)[:] = _bf16_rne(v) if storage == "bf16" else v
+        if storage == "bf16":
+            r.field(f)[:] = _bf16_rne(v)
+        elif storage == "f16":
+            r.field(f)[:] = _f16_rne(v, float(scales[f]))
+        else:
+            r.field(f)[:] = v
         del v
     E.run_time_loop(b, W, M, H, a, src, rec, p)
     b.set_tiling(1, 0, 16)
