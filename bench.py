@@ -96,7 +96,7 @@ def check_roofline(frac: float, kernel_ms: float, ms_per_step: float) -> None:
                          f"a step ({ms_per_step:.4f} ms)")
 
 
-def build_seam(nt: int, nbc=50, fd_order=8, with_weights=False, storage="f32"):
+def build_seam(nt: int, nbc=50, fd_order=8, with_weights=False, storage="f32", rec_types=("vz",)):
     from fomo_julia_amd import elastic2d as E
     from fomo_julia_amd import init as I
     from fomo_julia_amd import media
@@ -112,8 +112,12 @@ def build_seam(nt: int, nbc=50, fd_order=8, with_weights=False, storage="f32"):
     src = E.Source(I.coord_to_index(F32(M.x_max / F32(2)), M.dx, M.pad),
                    I.coord_to_index(F32(0.0), M.dz, M.pad), wav, "pressure")
     nx_in = vp.shape[0]
-    rec = E.setup_receivers(np.arange(nx_in, dtype=F32) * dx, np.zeros(nx_in, F32), M, type="vz")
-    return dict(M=M, H=H, p=p, a=a, src=src, rec=rec, vp=vp, velocity=(vp, vs, rho))
+    # a receiver on every surface column for each type (types in blocks of nx_in)
+    xr = np.tile(np.arange(nx_in, dtype=F32) * dx, len(rec_types))
+    types = rec_types[0] if len(rec_types) == 1 else [t for t in rec_types for _ in range(nx_in)]
+    rec = E.setup_receivers(xr, np.zeros(len(xr), F32), M, type=types)
+    return dict(M=M, H=H, p=p, a=a, src=src, rec=rec, vp=vp, velocity=(vp, vs, rho),
+                dt=dt, dx=dx, nbc=nbc, fd_order=fd_order, rec_types=tuple(rec_types))
 
 
 def model_blocks(b, dm, M, H) -> int:
@@ -143,6 +147,80 @@ def cpu_baseline(case, steps: int, threads: int):
     el = time.perf_counter() - t0
     cells = M.nx * M.nz * steps
     return cells / el / 1e6, el
+
+
+def _oracle_for(case, steps: int, dtype, threads: int, storage: str = "f32", fscale=None):
+    """The oracle on `steps` steps of a build_seam case."""
+    import oracle as O
+    from fomo_julia_amd import init as I
+    M, H, p, src, rec = case["M"], case["H"], case["p"], case["src"], case["rec"]
+    w = [I.habc_weight_matrix(M.nx, M.nz, H.nbc, off) for off in (0.0, 0.5, 0.75)]
+    return O.OracleRun(nx=M.nx, nz=M.nz, pad=M.pad, nbc=H.nbc, fd_order=p.fd_order,
+                       free_surface=M.is_free_surface, dt=p.dt, dtx=p.dtx, dtz=p.dtz,
+                       q=(H.qx, H.qz, H.qt_x, H.qt_z, H.qxt), a=case["a"], rho_vx=M.rho_vx,
+                       rho_vz=M.rho_vz, lam=M.lam, mu_txx=M.mu_txx, mu_txz=M.mu_txz,
+                       w_vx=w[0], w_vz=w[1], w_tau=w[2], src_i=src.i, src_j=src.j, src_type=0,
+                       wavelet=src.wavelet[:steps], rec_i=rec.i, rec_j=rec.j,
+                       rec_type=rec.type_codes(), nt=steps, dtype=dtype, flags=O.FMA,
+                       threads=threads, storage=storage, fscale=fscale)
+
+
+def _component_errors(g, g64, types, n_col) -> dict:
+    """Per receiver type: relative L2 and peak errors of gather g against g64."""
+    out = {}
+    for k, t in enumerate(types):
+        a = np.asarray(g[:, k * n_col:(k + 1) * n_col], np.float64)
+        r = np.asarray(g64[:, k * n_col:(k + 1) * n_col], np.float64)
+        peak = float(np.abs(r).max())
+        out[t] = {"rel_l2": float(np.linalg.norm(a - r) / np.linalg.norm(r)),
+                  "max_abs_err_over_peak": float(np.abs(a - r).max() / peak),
+                  "peak_amplitude_err": float((np.abs(a).max() - peak) / peak)}
+    return out
+
+
+def bench_precision(args) -> dict:
+    """Each storage's error against the Float64 run of the same scheme on the
+    SEAM-shaped shot (4444 x 3819, fd_order 8, nbc 50, free surface, 10 Hz
+    Ricker pressure source at the surface centre): the first K steps (default
+    400), receivers on every surface column recording vx, vz and pressure.
+    GPU runs for FP32, FP16 coefficients (FP32 fields), FP16 and BF16; the
+    oracle's emulation of the block floating-point field format (no kernel)
+    beside them.  Float64 is the oracle on the host's CPUs."""
+    from fomo_julia_amd import elastic2d as E
+    steps = args.steps if args.steps != 11520 else 400
+    types = ("vx", "vz", "p")
+    case = build_seam(steps, rec_types=types)
+    n_col = case["vp"].shape[0]
+    threads = cpu_share()["usable"]
+    t0 = time.perf_counter()
+    r64 = _oracle_for(case, steps, np.float64, threads)
+    r64.run(1, steps)
+    g64 = r64.gather_nt_by_nrec().copy()
+    f64_s = time.perf_counter() - t0
+    del r64
+    b = E.backend("mi355x")
+    b.set_tiling(2, 0, 16)
+    vp, vs, rho = case["velocity"]
+    out = {"metric": "gather error vs Float64 per component, SEAM-shaped shot, 1 GPU",
+           "steps": steps, "components": list(types), "receivers_per_component": n_col,
+           "float64_oracle_seconds": round(f64_s, 1), "oracle_threads": threads,
+           "data": "synthetic (seeded SEAM-like medium)", "gpu": {}, "emulated": {}}
+    for storage in ("f32", "f16c", "f16", "bf16"):
+        Ms = E.init_medium(vp, vs, rho, case["dx"], case["dx"], case["nbc"], case["fd_order"], None,
+                           free_surface=True, storage=storage)
+        W = E.Wavefield(Ms.nx, Ms.nz, b)
+        res = E.run_shot(b, W, Ms, case["H"], case["a"], case["src"], case["rec"], case["p"])
+        out["gpu"][storage] = _component_errors(res.gather, g64, types, n_col)
+        print(json.dumps({storage: out["gpu"][storage]}), file=sys.stderr, flush=True)
+        W.close()
+        b._models.clear()
+    for storage in ("bfp16", "bfp16c"):
+        r = _oracle_for(case, steps, np.float32, threads, storage=storage)
+        r.run(1, steps)
+        out["emulated"][storage] = _component_errors(r.gather_nt_by_nrec(), g64, types, n_col)
+        print(json.dumps({storage: out["emulated"][storage]}), file=sys.stderr, flush=True)
+        del r
+    return out
 
 
 def bench_batch(args) -> dict:
@@ -199,6 +277,10 @@ def bench_batch(args) -> dict:
     wall = time.perf_counter() - t0
     lib.fomo_batch_destroy(bh)
     cells = M.nx * M.nz * nt * n_shots
+    serial = None
+    if args.serial_shots > 0:
+        del wfs, dss
+        serial = bench_batch_serial(b, (vp, vs, rho), dx, dt, xs, nt, args.serial_shots)
     return {"metric": "shots/hour and Mcell-updates/s, batched shots, 1 GPU",
             "value": round(n_shots / gpu_s * 3600.0, 1), "unit": "shots/hour", "n_gpus": 1,
             "steps": nt, "higher_is_better": True, "dtype": "f32", "storage": args.storage,
@@ -212,7 +294,58 @@ def bench_batch(args) -> dict:
             "wall_seconds_with_gathers": round(wall, 3),
             "tiling": {"cols_per_lane": tuning["tiling"][0], "rows_per_tile": tuning["tiling"][1],
                        "trials_ms_per_step": tuning["trials"]},
-            "gathers_finite": bool(all(np.isfinite(g).all() for g in gathers))}
+            "gathers_finite": bool(all(np.isfinite(g).all() for g in gathers)),
+            "serial_f32": serial,
+            "speedup_vs_serial_f32": (round(n_shots / gpu_s * 3600.0 / serial["shots_per_hour"], 3)
+                                      if serial else None)}
+
+
+def bench_batch_serial(b, velocity, dx, dt, xs, nt: int, n_timed: int) -> dict:
+    """The plain FP32 GPU path on the same shots: run_shots one shot at a time
+    (the reference's sequential loop, shot_manager.jl:157-176) on one
+    wavefield, FP32 storage, tile shape autotuned for the single shot.  One
+    untimed shot first (graph capture); then `n_timed` shots, synchronized
+    wall time, which gives seconds per shot and shots/hour."""
+    import torch  # noqa: F401  (device initialisation as in the other workloads)
+    from fomo_julia_amd import elastic2d as E
+    from fomo_julia_amd import init as I
+    from fomo_julia_amd import tune as T
+    F32 = np.float32
+    vp, vs, rho = velocity
+    M = E.init_medium(vp, vs, rho, dx, dx, 40, 12, None, free_surface=True, storage="f32")
+    H = E.init_habc(M.nx, M.nz, 40, dt, dx, dx, I.julia_mean_f32(vp), None, weights=False)
+    p = E.SimParams(dt, nt, dx, dx, 12)
+    a = I.get_fd_coefficients(12)
+    wav = I.ricker_wavelet(F32(25.0), dt, nt)
+    nx_in = vp.shape[0]
+    rec = E.setup_receivers(np.arange(nx_in, dtype=F32) * dx, np.zeros(nx_in, F32), M, type="vz")
+    W = E.Wavefield(M.nx, M.nz, b)
+    dm = E.bind(b, W, M, H, a, p)
+    src_t = E.Source(I.coord_to_index(F32(xs[0]), M.dx, M.pad), I.coord_to_index(F32(0.0), M.dz, M.pad),
+                     I.ricker_wavelet(F32(25.0), dt, TIMING_NT), "pressure")
+    tuning = T.autotune_batch(b, [dm], [W], [E._DeviceShot(dm, src_t, rec, TIMING_NT)])
+    n = min(n_timed + 1, len(xs))
+    cfg = E.MultiShotConfig(xs[:n], np.zeros(n, F32), wav, source_type="pressure")
+    times = []
+    E.run_shots(b, W, M, H, a, rec, E.MultiShotConfig(xs[:1], np.zeros(1, F32), wav), p, batch=1)
+    b.synchronize()
+    t_last = [time.perf_counter()]
+
+    def done(res):
+        b.synchronize()
+        t = time.perf_counter()
+        times.append(t - t_last[0])
+        t_last[0] = t
+
+    E.run_shots(b, W, M, H, a, rec, E.MultiShotConfig(xs[1:n], np.zeros(n - 1, F32), wav), p,
+                batch=1, on_shot_complete=done)
+    per_shot = float(np.mean(times))
+    return {"shots_timed": len(times), "seconds_per_shot": round(per_shot, 4),
+            "shots_per_hour": round(3600.0 / per_shot, 1),
+            "ms_per_step": round(per_shot * 1e3 / nt, 4),
+            "tiling": {"cols_per_lane": tuning["tiling"][0], "rows_per_tile": tuning["tiling"][1]},
+            "note": "one shot at a time through run_shots(batch=1): run_shot on one wavefield, "
+                    "gather downloaded after each shot; synchronized wall time per shot"}
 
 
 def bench_edges(args) -> dict:
@@ -401,11 +534,15 @@ def main():
     ap.add_argument("--two-pass", action="store_true", help="time the two-pass in-place step instead")
     ap.add_argument("--storage", default="f32", choices=sorted(BYTES_FUSED_STORAGE),
                     help="device storage of coefficients/fields (arithmetic stays FP32)")
-    ap.add_argument("--workload", default="seam", choices=("seam", "batch", "serve", "edges"),
+    ap.add_argument("--workload", default="seam",
+                    choices=("seam", "batch", "serve", "edges", "precision"),
                     help="seam: the single SEAM-shaped shot (default); batch: 64 batched "
                          "Marmousi-shaped shots; serve: the served request stream; edges: "
-                         "device-side setup and on-step frames on the SEAM-shaped shot")
+                         "device-side setup and on-step frames on the SEAM-shaped shot; "
+                         "precision: each storage's gather error vs Float64 on the SEAM shot")
     ap.add_argument("--shots", type=int, default=64, help="shots of the batch workload")
+    ap.add_argument("--serial-shots", type=int, default=4,
+                    help="batch workload: shots timed one at a time on the plain FP32 path")
     ap.add_argument("--requests", type=int, default=40,
                     help="requests of the served sample (default run) or stream (serve)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served sample")
@@ -435,6 +572,9 @@ def main():
         sys.exit(2)
     if args.workload == "edges":
         print(json.dumps(bench_edges(args)))
+        return
+    if args.workload == "precision":
+        print(json.dumps(bench_precision(args)))
         return
     if args.workload == "batch":
         print(json.dumps(bench_batch(args)))

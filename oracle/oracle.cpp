@@ -158,18 +158,24 @@ static void round_block(T* x, int n) {
 }
 template <typename T>
 static void round_fields(Model<T>& m, int c0, int c1) {
+  // every cell (segment) is rounded on its own, so rows go to threads
   if (m.storage == 4 || m.storage == 5) {
-    for (int c = c0; c < c1; ++c)
+    for (int c = c0; c < c1; ++c) {
+#pragma omp parallel for schedule(static) num_threads(m.d->threads)
       for (int j = 0; j < m.nz; ++j)
         for (int i0 = 0; i0 < m.nx; i0 += 64)
           round_block(m.f[c] + (size_t)j * m.nx + i0, std::min(64, m.nx - i0));
+    }
     return;
   }
   if (m.storage != 1 && m.storage != 2) return;
-  const size_t n = (size_t)m.nx * m.nz;
   for (int c = c0; c < c1; ++c) {
     const float sc = m.d->fscale[c];
-    for (size_t o = 0; o < n; ++o) m.f[c][o] = (T)round_storage(m.storage, (float)m.f[c][o], sc);
+#pragma omp parallel for schedule(static) num_threads(m.d->threads)
+    for (int j = 0; j < m.nz; ++j) {
+      T* row = m.f[c] + (size_t)j * m.nx;
+      for (int i = 0; i < m.nx; ++i) row[i] = (T)round_storage(m.storage, (float)row[i], sc);
+    }
   }
 }
 // Power of two putting max |x| in [2^14, 2^15) (FP16 coefficient scale).

@@ -184,3 +184,40 @@ def test_pool_runs_while_other_threads_create_and_free_objects(b):
     for p, dm, W, ds in members:
         for obj in (ds, W, dm):
             obj.close()
+
+
+@gpu
+def test_served_stream_at_its_real_grid_sizes(b):
+    """Requests drawn from the served stream's own ranges (nx 300-2000, nz
+    300-1200, fd_order 4/8/12, FP32 and BF16 storage), nt cut to about 200:
+    each served gather equals its solo run bit for bit, and one FP32 request
+    equals the oracle's run of the same medium, source and receivers."""
+    import oracle as O
+    from fomo_julia_amd import elastic2d as E
+    from fomo_julia_amd import init as I
+    reqs = S.c5_stream(8, seed=5, nt=(200, 240))
+    print("case C5 real sizes", [(r.nx, r.nz, r.nt, r.fd_order, r.storage) for r in reqs], flush=True)
+    assert {r.storage for r in reqs} == {"f32", "bf16"}
+    srv = S.Server(b, budget_bytes=32 << 30, chunk=64, capacity=16)
+    done = srv.serve(reqs, realtime=False)
+    assert not srv.failed and sorted(c.rid for c in done) == list(range(8))
+    for c in done:
+        assert np.array_equal(c.gather, _alone(b, reqs[c.rid])), f"request {c.rid} differs from its solo run"
+    # one FP32 request against the oracle (host setup of the same medium)
+    r = next(q for q in reqs if q.storage == "f32")
+    p = S.prepare(r)
+    vp, vs, rho = p.M._velocity
+    Mh = E.init_medium(vp, vs, rho, p.M.dx, p.M.dz, r.nbc, r.fd_order, None, free_surface=True)
+    w = [I.habc_weight_matrix(Mh.nx, Mh.nz, r.nbc, off) for off in (0.0, 0.5, 0.75)]
+    H = p.H
+    orc = O.OracleRun(nx=Mh.nx, nz=Mh.nz, pad=Mh.pad, nbc=r.nbc, fd_order=r.fd_order,
+                      free_surface=True, dt=p.p.dt, dtx=p.p.dtx, dtz=p.p.dtz,
+                      q=(H.qx, H.qz, H.qt_x, H.qt_z, H.qxt), a=p.a, rho_vx=Mh.rho_vx,
+                      rho_vz=Mh.rho_vz, lam=Mh.lam, mu_txx=Mh.mu_txx, mu_txz=Mh.mu_txz,
+                      w_vx=w[0], w_vz=w[1], w_tau=w[2], src_i=p.src.i, src_j=p.src.j, src_type=0,
+                      wavelet=p.src.wavelet, rec_i=p.rec.i, rec_j=p.rec.j, rec_type=0, nt=r.nt,
+                      dtype=np.float32, flags=O.FMA)
+    orc.run(1, r.nt)
+    served = next(c for c in done if c.rid == r.rid)
+    assert np.array_equal(served.gather, orc.gather_nt_by_nrec()), "served FP32 request vs oracle"
+    assert np.abs(served.gather).max() > 0
