@@ -63,6 +63,7 @@ SIGNATURES = {
     "fomo_shot_field_scales": (c_int32, [c_void_p, P_f32]),
     "fomo_shot_reset": (c_int32, [c_void_p]),
     "fomo_shot_gather": (c_int32, [c_void_p, P_f32]),
+    "fomo_shot_gather_noalloc": (c_int32, [c_void_p, P_f32]),
     "fomo_time_step": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32]),
     "fomo_run_time_loop": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32]),
     "fomo_run_time_loop_prepare": (c_int32, [c_void_p, c_void_p, c_void_p]),

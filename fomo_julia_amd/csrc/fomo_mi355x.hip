@@ -31,6 +31,7 @@ using namespace fomo;
 struct fomo_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  float* scratch = nullptr;  // this device's copy of the `scratch` array (fomo_shot_gather_noalloc)
   std::string err;
   int cpt = 1;         // columns per lane of the stencil tiles
   int tz = 0;          // rows per stencil tile; 0 = sized to fill the GPU in whole rounds
@@ -151,6 +152,18 @@ static uint64_t g_next_frames_id = 1;
 static hipStream_t lstream(const fomo_batch* b) { return b->ls ? b->ls : b->ctx->stream; }
 
 static thread_local std::string g_err;
+
+// Device array of fomo_shot_gather_noalloc: one copy per GPU the library runs
+// on, held until the process ends, so that entry point allocates nothing.
+// Host code must not pass this name to a kernel or copy: it stands for the
+// array only on the device.  fomo_ctx_create looks up the device address
+// (hipGetSymbolAddress, with the context's device current) and keeps it in
+// fomo_ctx::scratch; that pointer is what host code uses.  Calls on one GPU
+// share the array, so calls that overlap on one GPU race.
+constexpr size_t NOALLOC_SCRATCH_FLOATS = (size_t)4 << 20;  // 16 MiB
+constexpr int NOALLOC_MIN_STEPS = 64;  // time steps per chunk: at least one transpose tile
+constexpr int NOALLOC_MAX_RECEIVERS = (int)(NOALLOC_SCRATCH_FLOATS / NOALLOC_MIN_STEPS);
+static __device__ __align__(256) float scratch[NOALLOC_SCRATCH_FLOATS];
 
 // Held while a graph is captured and around every call that allocates, frees
 // or copies device memory outside the context's stream: such a call from
@@ -305,6 +318,9 @@ extern "C" int32_t fomo_ctx_create(int32_t device, fomo_ctx** out) {
   HIP_TRY(nullptr, hipSetDevice(device));
   HIP_TRY(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(nullptr, hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device));
+  void* sp = nullptr;  // this device's copy of `scratch` (the device is current)
+  HIP_TRY(nullptr, hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  c->scratch = (float*)sp;
   *out = c.release();
   return FOMO_OK;
 }
@@ -577,19 +593,19 @@ extern "C" int32_t fomo_model_create_from_velocity(fomo_ctx* c, const fomo_model
   HIP_TRY(c, hipMemset(m->dcoef, 0, nblocks * fs * es));
   const int blk_of[7] = {0, same ? 0 : 5, 1, same ? 1 : 6, 2, 3, 4};
   for (int k = 0; k < 7; ++k) m->coef[k] = block_origin(m->dcoef, fs, blk_of[k], P, es);
-  // inputs and the per-block maxima of the scan pass in one scratch buffer
+  // inputs and the per-block maxima of the scan pass in one staging buffer
   const int blocks = std::max(1, std::min(4 * std::max(c->cus, 1), (int)((size_t)nx * nz / 256 + 1)));
-  float* scratch = nullptr;
-  HIP_TRY(c, hipMalloc(&scratch, (3 * nin + (size_t)blocks * 9) * sizeof(float)));
-  std::unique_ptr<float, hipError_t (*)(void*)> hold(scratch, hipFree);
-  HIP_TRY(c, hipMemcpy(scratch, vp, nin * sizeof(float), hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(scratch + nin, vs, nin * sizeof(float), hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(scratch + 2 * nin, rho, nin * sizeof(float), hipMemcpyHostToDevice));
+  float* staging = nullptr;
+  HIP_TRY(c, hipMalloc(&staging, (3 * nin + (size_t)blocks * 9) * sizeof(float)));
+  std::unique_ptr<float, hipError_t (*)(void*)> hold(staging, hipFree);
+  HIP_TRY(c, hipMemcpy(staging, vp, nin * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(staging + nin, vs, nin * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(staging + 2 * nin, rho, nin * sizeof(float), hipMemcpyHostToDevice));
   SetupArgs A;
   std::memset(&A, 0, sizeof(A));
-  A.vp = scratch;
-  A.vs = scratch + nin;
-  A.rho = scratch + 2 * nin;
+  A.vp = staging;
+  A.vs = staging + nin;
+  A.rho = staging + 2 * nin;
   A.nxi = nx_in;
   A.nzi = nz_in;
   A.pad = d->pad;
@@ -603,7 +619,7 @@ extern "C" int32_t fomo_model_create_from_velocity(fomo_ctx* c, const fomo_model
     A.out[k] = const_cast<float*>(m->coef[k]);
     A.sc[k] = 1.f;
   }
-  A.partial = scratch + 3 * nin;
+  A.partial = staging + 3 * nin;
   A.out_rho = m->compact ? block_origin(m->dcoef, fs, 5, P, es) : nullptr;
   A.out_mu = m->compact ? block_origin(m->dcoef, fs, 6, P, es) : nullptr;
   // pass 0: coefficient maxima (FP16 scales) and the impedance range
@@ -925,25 +941,56 @@ extern "C" int32_t fomo_shot_gather(fomo_shot* s, float* out) {
   fomo_ctx* c = s->m->ctx;
   if (s->n_rec == 0 || s->nt == 0) return FOMO_OK;
   // device [k][r] -> Julia [nt x n_rec] column-major (out[k + r*nt]), transposed
-  // on the device into a scratch buffer, then one copy
+  // on the device into a staging buffer, then one copy
   const size_t n = (size_t)s->nt * s->n_rec;
-  float* scratch = nullptr;
+  float* tposed = nullptr;
   {
     API_LOCK;  // allocations must not interleave with a graph capture on another thread
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipMalloc(&scratch, n * sizeof(float)));
+    HIP_TRY(c, hipMalloc(&tposed, n * sizeof(float)));
   }
   const dim3 g((unsigned)((s->n_rec + 63) / 64), (unsigned)((s->nt + 63) / 64));
-  hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, c->stream, s->dgather, scratch, s->nt, s->n_rec);
+  hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, c->stream, s->dgather, tposed, s->nt, s->n_rec);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess)
-    e = hipMemcpyAsync(out, scratch, n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    e = hipMemcpyAsync(out, tposed, n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   {
     API_LOCK;
-    (void)hipFree(scratch);
+    (void)hipFree(tposed);
   }
   if (e != hipSuccess) return fail(c, FOMO_ERR_HIP, std::string("shot_gather: ") + hipGetErrorString(e));
+  return FOMO_OK;
+}
+
+// The same [nt x n_rec] download through the library's fixed device array:
+// chunks of kc time steps are transposed into `scratch` ([r][k - k0], n_rec
+// rows of kc floats) and copied with one strided copy each to out[k + r*nt].
+// Stream order keeps a chunk's transpose behind the previous chunk's copy.
+extern "C" int32_t fomo_shot_gather_noalloc(fomo_shot* s, float* out) {
+  if (!s || !out) return fail(s ? s->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
+  fomo_ctx* c = s->m->ctx;
+  if (s->n_rec == 0 || s->nt == 0) return FOMO_OK;
+  if (s->n_rec > NOALLOC_MAX_RECEIVERS)
+    return fail(c, FOMO_ERR_ARG, "fomo_shot_gather_noalloc: " + std::to_string(s->n_rec) +
+                                     " receivers exceed its maximum of " +
+                                     std::to_string(NOALLOC_MAX_RECEIVERS) +
+                                     " (64 steps of a 16 MiB device array); use fomo_shot_gather");
+  if (!c->scratch) return fail(c, FOMO_ERR_STATE, "context has no device scratch array");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const int nt = s->nt, nr = s->n_rec;
+  const int kc = (int)std::min<size_t>((size_t)nt, NOALLOC_SCRATCH_FLOATS / (size_t)nr);
+  for (int k0 = 0; k0 < nt; k0 += kc) {
+    const int n = std::min(kc, nt - k0);
+    const dim3 g((unsigned)((nr + 63) / 64), (unsigned)((n + 63) / 64));
+    hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, c->stream, s->dgather + (size_t)k0 * nr,
+                       c->scratch, n, nr);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpy2DAsync(out + k0, (size_t)nt * sizeof(float), c->scratch,
+                                (size_t)n * sizeof(float), (size_t)n * sizeof(float), (size_t)nr,
+                                hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
   return FOMO_OK;
 }
 

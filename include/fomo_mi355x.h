@@ -168,6 +168,15 @@ int32_t fomo_shot_field_scales(const fomo_shot* s, float* out5);
 int32_t fomo_shot_reset(fomo_shot* s);
 /* _get_gather (shot_manager.jl:71-72): [nt x n_rec] column-major into out. */
 int32_t fomo_shot_gather(fomo_shot* s, float* out);
+/* No counterpart in the reference.  fomo_shot_gather for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same [nt x n_rec] result, with the
+ * transpose done in chunks of time steps through a fixed 16 MiB device array
+ * of the library (one per GPU, held until the process ends), so the call
+ * allocates nothing.  n_rec above 65536 (64 steps per chunk) returns
+ * FOMO_ERR_ARG.  All calls on one GPU share that array: calls that overlap in
+ * time on one GPU (other threads, other contexts of the device) race. */
+int32_t fomo_shot_gather_noalloc(fomo_shot* s, float* out);
 
 /* ---- the time step ------------------------------------------------------ */
 /* time_step!(backend, W, M, H, a, src, rec, k, params) (time_stepper.jl:39-63), k 1-based. */
