@@ -85,6 +85,14 @@ def cpu_share() -> dict:
             "cgroup_quota": None if quota is None else round(quota, 2)}
 
 
+def split_interior_cells(nx: int, nz: int, nbc: int, M: int) -> int:
+    """Cells the split step's interior tiles write: columns [xi0, xi1), rows
+    [zi0, zi1), 2M + 2 cells inside the band frame of width nbc + M + 1."""
+    mg = nbc + M + 1 + 2 * M + 2
+    xi0, xi1, zi0, zi1 = (mg + 1) & ~1, (nx - mg) & ~1, mg, nz - mg
+    return max(0, xi1 - xi0) * max(0, zi1 - zi0)
+
+
 def check_roofline(frac: float, kernel_ms: float, ms_per_step: float) -> None:
     """A roofline line is only printed for a kernel that did its work: its
     fraction of peak lies in (0, 1] and one launch is no longer than a step."""
@@ -532,6 +540,9 @@ def main():
     ap.add_argument("--stage-reps", type=int, default=50)
     ap.add_argument("--tiling", default="", help="cols_per_lane,rows_per_tile,steps_per_graph")
     ap.add_argument("--two-pass", action="store_true", help="time the two-pass in-place step instead")
+    ap.add_argument("--split", default="", metavar="EDGE_ROWS,BLOCKS_PER_CU",
+                    help="split fused step: edge tiles first, then the interior tiles beside the "
+                         "strip kernels (rows per edge tile, interior blocks per CU, 0 = no cap)")
     ap.add_argument("--storage", default="f32", choices=sorted(BYTES_FUSED_STORAGE),
                     help="device storage of coefficients/fields (arithmetic stays FP32)")
     ap.add_argument("--workload", default="seam",
@@ -598,7 +609,11 @@ def main():
 
     b = E.backend("mi355x")
     fused = not args.two_pass
-    b.set_step_mode(1 if fused else 0)
+    split = fused and bool(args.split)
+    b.set_step_mode(2 if split else 1 if fused else 0)
+    if split:
+        er, bpc = (int(v) for v in args.split.split(","))
+        b.set_split(er, bpc)
     W = E.Wavefield(M.nx, M.nz, b)
     dm = E.bind(b, W, M, H, case["a"], p)
     ds = E._shot_for(dm, src, rec, p.nt)
@@ -669,6 +684,7 @@ def main():
         return e0.elapsed_time(e1) / reps
 
     reps = args.stage_reps
+    main_cells = cells
     if fused:
         main_ms = stage_ms(5, reps)
         kernels = {"fused": main_ms, "band_stress": stage_ms(6, reps),
@@ -679,18 +695,25 @@ def main():
         if args.storage == "f32" and model_blocks(b, dm, M, H) == 7:
             main_bytes, form = BYTES_FUSED_DERIVED, "coefficients formed from rho_vx, lam, mu_txx"
         main_name = f"k_fused, {form} ({main_bytes} B/cell-update x padded cells per launch)"
+        if split:
+            # the split step runs the fused pass as edge tiles, then interior tiles
+            kernels["fused_edge"] = stage_ms(9, reps)
+            kernels["fused_interior"] = main_ms = stage_ms(10, reps)
+            main_cells = split_interior_cells(M.nx, M.nz, H.nbc, p.fd_order // 2)
+            main_name = (f"k_fused over the split step's interior tiles, {form} ({main_bytes} "
+                         f"B/cell-update x {main_cells} interior cells per launch)")
     else:
         main_ms = stage_ms(1, reps)
         kernels = {"velocity": stage_ms(0, reps), "stress": main_ms,
                    "habc_velocity": stage_ms(2, reps), "habc_stress": stage_ms(3, reps),
                    "fs_record": stage_ms(4, reps)}
         main_bytes, main_name = BYTES_STRESS, "k_stress (44 B/cell-update x padded cells per launch)"
-    main_gbs = main_bytes * cells / (main_ms / 1e3) / 1e9
+    main_gbs = main_bytes * main_cells / (main_ms / 1e3) / 1e9
     step_bytes = main_bytes if fused else BYTES_VELOCITY + BYTES_STRESS
     step_gbs = step_bytes * cells / (ms_per_step / 1e3) / 1e9
     check_roofline(main_gbs / HBM_PEAK_GBS, main_ms, ms_per_step)
     traffic = None
-    if fused and args.storage == "f32" and os.path.exists(TRAFFIC_FILE):
+    if fused and not split and args.storage == "f32" and os.path.exists(TRAFFIC_FILE):
         for line in open(TRAFFIC_FILE):
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = int(line.split(":")[1])
@@ -762,12 +785,12 @@ def main():
                      "kernel": main_name, "kernel_ms": round(main_ms, 5),
                      "kernel_timing": f"HIP events around {reps} back-to-back launches on the "
                                       f"library's stream, step {k_stage} of a {TIMING_NT}-step shot",
-                     "algorithmic_bytes_per_launch": main_bytes * cells},
+                     "algorithmic_bytes_per_launch": main_bytes * main_cells},
         "kernels_ms": {k: round(v, 5) for k, v in kernels.items()},
         "tiling": ({"cols_per_lane": tuning["tiling"][0], "rows_per_tile": tuning["tiling"][1],
                     "autotuned": True, "trials_ms_per_step": tuning["trials"]} if tuning else
                    {"fixed": args.tiling or "library default"}),
-        "step_mode": "fused" if fused else "two-pass",
+        "step_mode": ("split fused " + args.split) if split else "fused" if fused else "two-pass",
         "step_gbs": round(step_gbs, 1),
         "step_frac_hbm": round(step_gbs / HBM_PEAK_GBS, 4),
         "cpu_baseline": cpu,

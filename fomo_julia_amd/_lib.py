@@ -43,6 +43,7 @@ SIGNATURES = {
     "fomo_ctx_stream": (c_void_p, [c_void_p]),
     "fomo_ctx_set_tiling": (c_int32, [c_void_p, c_int32, c_int32, c_int32]),
     "fomo_ctx_set_step_mode": (c_int32, [c_void_p, c_int32]),
+    "fomo_ctx_set_split": (c_int32, [c_void_p, c_int32, c_int32]),
     "fomo_model_create": (c_int32, [c_void_p, ctypes.POINTER(ModelDesc), P_f32, P_f32, P_f32,
                                     P_f32, P_f32, P_f32, P_f32, P_f32, P_f32,
                                     ctypes.POINTER(c_void_p)]),

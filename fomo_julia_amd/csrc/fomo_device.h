@@ -40,6 +40,8 @@ enum Part {
   PART_SHOT,          // one item per shot
   PART_FUSED,         // fused velocity+stress tiles (ping-pong step)
   PART_BAND,          // stress band tiles next to the HABC strips (ping-pong step)
+  PART_FUSED_EDGE,    // split fused step: tiles of the frame (see ShotDev::xi0)
+  PART_FUSED_INT,     // split fused step: tiles of the interior rectangle
   PART_COUNT
 };
 
@@ -64,6 +66,16 @@ struct ShotDev {
   int band;             // band width b = nbc + M + 1
   int nb_side, nb_mid;  // band items: per side band, per bottom/top band
   int band_rows;        // rows per band item
+  // Split fused step.  Interior: written columns [xi0, xi1), rows [zi0, zi1),
+  // in nix x niz tiles of ftw x tzi; the cells any of them reads are at least
+  // 2M + 2 cells inside the band frame that the strip kernels (HABC, band
+  // stress, free surface) write, so interior tiles of a step may run while
+  // those kernels run.  Edge: the rest, in ftw x tze tiles: top block rows
+  // [0, zi0) and bottom block rows [zi1, nz) across the grid (e_top, e_bot
+  // tiles), then the left block columns [0, xi0) and right block columns
+  // [xi1, nx) of rows [zi0, zi1) (ntx_l, ntx_r tile columns of e_mid rows).
+  int xi0, xi1, zi0, zi1, nix, niz, tzi, tze;
+  int e_top, e_bot, e_mid, ntx_l, ntx_r;
   int n_l, n_r, n_b, n_t;  // HABC side/bottom/top edge pieces per field
   float dtx, dtz;
   // Reduced-precision storage: a stored value is x * 2^e (exact), read back as

@@ -37,7 +37,10 @@ struct fomo_ctx {
   int tz = 0;          // rows per stencil tile; 0 = sized to fill the GPU in whole rounds
   int graph_steps = 16;
   int cus = 0;         // compute units of the device
-  int step_mode = 1;   // 1 = fused ping-pong step, 0 = two-pass in-place step
+  int step_mode = 1;   // 1 = fused ping-pong step, 0 = two-pass in-place step, 2 = split fused step
+  hipStream_t aux = nullptr;  // the split step's strip kernels (default priority, like `stream`)
+  int tze = 16;        // split step: rows per edge tile
+  int int_bpc = 0;     // split step: interior blocks per CU at most (0: as many as fit)
 };
 
 struct fomo_model {
@@ -101,6 +104,7 @@ struct fomo_shot {
   int* dk = nullptr;
   float src_scale = 0.f;
   float fsc[5] = {1, 1, 1, 1, 1};  // field store scales 2^e (FP16 storage)
+  std::vector<int> hrec;  // receivers (i, j, type), 0-based, as on the device
   uint64_t id = 0;  // unique per shot object, keys the cached batches
   int pool_refs = 0;  // 1 while in a pool (destroy is refused)
 };
@@ -134,6 +138,15 @@ struct fomo_batch {
   hipGraph_t graph[2] = {nullptr, nullptr};
   int grid_fixed[PART_COUNT] = {};  // > 0: fixed grid (pools: persistent over any membership)
   hipStream_t ls = nullptr;         // stream of all its launches (null: the context's)
+  // Split fused step: the edge tiles run first; then the interior tiles on
+  // the batch's stream beside the strip kernels on the context's aux stream,
+  // joined before the next step.  ev_edge / ev_chain order the two streams;
+  // rec_in_frame: every receiver lies outside the interior rectangles, so
+  // recording joins the strip kernels instead of waiting for the interior.
+  bool split = false, rec_in_frame = false;
+  int tzi = 0, tze = 16, int_bpc = 0;
+  hipEvent_t ev_edge = nullptr, ev_chain = nullptr;
+  mutable hipStream_t cur = nullptr;  // set while the strip kernels are queued on the aux stream
   // frame store written after every step (one-shot batches); 0: none.  The
   // captured chunks hold the store's arguments, so they are keyed by its id.
   uint64_t frames_id = 0;
@@ -149,7 +162,9 @@ struct fomo_frames {
 };
 static uint64_t g_next_frames_id = 1;
 
-static hipStream_t lstream(const fomo_batch* b) { return b->ls ? b->ls : b->ctx->stream; }
+static hipStream_t lstream(const fomo_batch* b) {
+  return b->cur ? b->cur : b->ls ? b->ls : b->ctx->stream;
+}
 
 static thread_local std::string g_err;
 
@@ -317,6 +332,7 @@ extern "C" int32_t fomo_ctx_create(int32_t device, fomo_ctx** out) {
   c->device = device;
   HIP_TRY(nullptr, hipSetDevice(device));
   HIP_TRY(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(nullptr, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
   HIP_TRY(nullptr, hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device));
   void* sp = nullptr;  // this device's copy of `scratch` (the device is current)
   HIP_TRY(nullptr, hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
@@ -329,7 +345,9 @@ extern "C" int32_t fomo_ctx_destroy(fomo_ctx* c) {
   if (!c) return FOMO_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->aux);
   (void)hipStreamDestroy(c->stream);
+  (void)hipStreamDestroy(c->aux);
   delete c;
   return FOMO_OK;
 }
@@ -349,7 +367,7 @@ extern "C" void* fomo_ctx_stream(fomo_ctx* c) { return c ? (void*)c->stream : nu
 
 extern "C" int32_t fomo_ctx_set_step_mode(fomo_ctx* c, int32_t mode) {
   if (!c) return fail(nullptr, FOMO_ERR_ARG, "ctx is NULL");
-  if (mode != 0 && mode != 1) return fail(c, FOMO_ERR_ARG, "step mode must be 0 or 1");
+  if (mode < 0 || mode > 2) return fail(c, FOMO_ERR_ARG, "step mode must be 0, 1 or 2");
   c->step_mode = mode;
   return FOMO_OK;
 }
@@ -363,6 +381,16 @@ extern "C" int32_t fomo_ctx_set_tiling(fomo_ctx* c, int32_t cpt, int32_t tz, int
   c->cpt = cpt;
   c->tz = tz;
   c->graph_steps = gsteps;
+  return FOMO_OK;
+}
+
+extern "C" int32_t fomo_ctx_set_split(fomo_ctx* c, int32_t edge_rows, int32_t interior_blocks_per_cu) {
+  if (!c) return fail(nullptr, FOMO_ERR_ARG, "ctx is NULL");
+  if (edge_rows < 1 || edge_rows > 256) return fail(c, FOMO_ERR_ARG, "edge_rows must be 1..256");
+  if (interior_blocks_per_cu < 0 || interior_blocks_per_cu > 64)
+    return fail(c, FOMO_ERR_ARG, "interior_blocks_per_cu must be 0..64 (0: as many as fit)");
+  c->tze = edge_rows;
+  c->int_bpc = interior_blocks_per_cu;
   return FOMO_OK;
 }
 
@@ -896,6 +924,7 @@ extern "C" int32_t fomo_shot_create(fomo_model* m, int32_t src_i, int32_t src_j,
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipMalloc(&s->dwav, std::max(nw, 1) * sizeof(float)));
   if (nw > 0) HIP_TRY(c, hipMemcpy(s->dwav, wavelet, nw * sizeof(float), hipMemcpyHostToDevice));
+  s->hrec = rec;
   HIP_TRY(c, hipMalloc(&s->drec, rec.size() * sizeof(int)));
   HIP_TRY(c, hipMemcpy(s->drec, rec.data(), rec.size() * sizeof(int), hipMemcpyHostToDevice));
   const size_t gbytes = (size_t)nt * std::max(n_rec, 1) * sizeof(float);
@@ -996,7 +1025,7 @@ extern "C" int32_t fomo_shot_gather_noalloc(fomo_shot* s, float* out) {
 
 // ------------------------------------------------------------------ batches
 static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const fomo_shot* s,
-                            int cpt, int tz, int table, int band_rows) {
+                            int cpt, int tz, int table, int band_rows, int tzi = 0, int tze = 16) {
   ShotDev sd;
   std::memset(&sd, 0, sizeof(sd));
   const fomo_model_desc& d = m->d;
@@ -1044,6 +1073,32 @@ static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const 
   sd.band_rows = band_rows;
   sd.nb_side = ((sd.band + 63) / 64) * ((d.nz + band_rows - 1) / band_rows);
   sd.nb_mid = ((d.nx - 2 * sd.band + 63) / 64) * ((sd.band + band_rows - 1) / band_rows);
+  // split step: the interior's written cells sit 2M + 2 cells inside the
+  // band frame, so the cells its tiles read are outside the frame that the
+  // strip kernels write (and the strip kernels read nothing it writes)
+  {
+    const int mg = sd.band + 2 * M + 2;
+    int xi0 = (mg + 1) & ~1, xi1 = (d.nx - mg) & ~1, zi0 = mg, zi1 = d.nz - mg;
+    if (xi1 <= xi0 || zi1 <= zi0) {  // no interior: the edge tiles cover the grid
+      xi0 = xi1 = 0;
+      zi0 = zi1 = d.nz;
+    }
+    tzi = tzi > 0 ? tzi : tz;
+    tze = std::max(1, tze);
+    sd.xi0 = xi0;
+    sd.xi1 = xi1;
+    sd.zi0 = zi0;
+    sd.zi1 = zi1;
+    sd.tzi = tzi;
+    sd.tze = tze;
+    sd.nix = xi1 > xi0 ? (xi1 - xi0 + sd.ftw - 1) / sd.ftw : 0;
+    sd.niz = zi1 > zi0 ? (zi1 - zi0 + tzi - 1) / tzi : 0;
+    sd.e_top = sd.ftiles_x * ((zi0 + tze - 1) / tze);
+    sd.e_bot = sd.ftiles_x * ((d.nz - zi1 + tze - 1) / tze);
+    sd.e_mid = (zi1 - zi0 + tze - 1) / tze;
+    sd.ntx_l = (xi0 + sd.ftw - 1) / sd.ftw;
+    sd.ntx_r = (d.nx - xi1 + sd.ftw - 1) / sd.ftw;
+  }
   sd.bvx_x = m->coef[0];
   sd.bvx_z = m->coef[1];
   sd.bvz_x = m->coef[2];
@@ -1096,6 +1151,8 @@ static int part_items(const ShotDev& sd, int part) {
     case PART_SHOT: return 1;
     case PART_FUSED: return sd.ftiles_x * sd.ftiles_z;
     case PART_BAND: return 2 * sd.nb_side + (sd.fs ? 1 : 2) * sd.nb_mid;
+    case PART_FUSED_EDGE: return sd.e_top + sd.e_bot + (sd.ntx_l + sd.ntx_r) * sd.e_mid;
+    case PART_FUSED_INT: return sd.nix * sd.niz;
   }
   return 0;
 }
@@ -1198,6 +1255,8 @@ static void destroy_batch(fomo_batch* b) {
   (void)hipFree(b->ds);
   (void)hipFree(b->dn);
   (void)hipFree(b->dprefix);
+  if (b->ev_edge) (void)hipEventDestroy(b->ev_edge);
+  if (b->ev_chain) (void)hipEventDestroy(b->ev_chain);
   delete b;
 }
 
@@ -1211,8 +1270,11 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
   b->tz = c->tz;
   b->tz_req = c->tz;
   b->graph_steps = c->graph_steps;
-  b->step_mode = c->step_mode;
+  b->split = c->step_mode == 2;
+  b->step_mode = c->step_mode == 2 ? 1 : c->step_mode;
   b->step_mode_req = c->step_mode;
+  b->tze = c->tze;
+  b->int_bpc = c->int_bpc;
   if (!ms[0]) return fail(c, FOMO_ERR_ARG, "NULL batch member");
   b->M = ms[0]->M;
   b->storage = ms[0]->storage;
@@ -1228,6 +1290,7 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
       if (b->storage != FOMO_STORAGE_F32)
         return fail(c, FOMO_ERR_UNSUPPORTED, "grid too large for the fused step with reduced storage");
       b->step_mode = 0;
+      b->split = false;
     }
   }
   for (int i = 0; i < n; ++i) {
@@ -1259,11 +1322,38 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
     b->tz = auto_tz(tx, nzs, stencil_slots(b->storage, b->M, b->cpt, c->cus, b->step_mode, b->compact), b->M,
                     b->step_mode == 1 ? fused_block(b->M) : 0);
   }
+  if (b->split) {
+    // interior tile height: whole rounds of the slots the interior may take
+    std::vector<int> tx, nzs;
+    for (int i = 0; i < n; ++i) {
+      const ShotDev g = make_shotdev(ms[i], ws[i], ss[i], b->cpt, b->tz, TBL_FUSED0, BAND_ROWS_SINGLE,
+                                     b->tz, b->tze);
+      tx.push_back(g.nix);
+      nzs.push_back(g.zi1 - g.zi0);
+    }
+    int slots = stencil_slots(b->storage, b->M, b->cpt, c->cus, 1, b->compact);
+    if (b->int_bpc > 0) slots = std::min(slots, b->int_bpc * WPB * std::max(c->cus, 1));
+    b->tzi = c->tz > 0 ? c->tz : auto_tz(tx, nzs, slots, b->M, fused_block(b->M));
+  }
   b->hs.resize((size_t)TBL_COUNT * n);
   for (int t = 0; t < TBL_COUNT; ++t)
     for (int i = 0; i < n; ++i)
       b->hs[(size_t)t * n + i] = make_shotdev(ms[i], ws[i], ss[i], b->cpt, b->tz, t,
-                                              n == 1 ? BAND_ROWS_SINGLE : BAND_ROWS_BATCH);
+                                              n == 1 ? BAND_ROWS_SINGLE : BAND_ROWS_BATCH,
+                                              b->split ? b->tzi : b->tz, b->tze);
+  if (b->split) {
+    b->rec_in_frame = true;
+    for (int i = 0; i < n; ++i) {
+      const ShotDev& g = b->hs[(size_t)TBL_FUSED0 * n + i];
+      const std::vector<int>& r = ss[i]->hrec;
+      for (int q = 0; q < ss[i]->n_rec; ++q) {
+        const int ri = r[3 * q], rj = r[3 * q + 1];
+        if (ri >= g.xi0 && ri < g.xi1 && rj >= g.zi0 && rj < g.zi1) b->rec_in_frame = false;
+      }
+    }
+    HIP_TRY(c, hipEventCreateWithFlags(&b->ev_edge, hipEventDisableTiming));
+    HIP_TRY(c, hipEventCreateWithFlags(&b->ev_chain, hipEventDisableTiming));
+  }
   b->hprefix.assign((size_t)PART_COUNT * (n + 1), 0);
   for (int p = 0; p < PART_COUNT; ++p) {
     int acc = 0;
@@ -1314,7 +1404,8 @@ static int single_batch(fomo_model* m, fomo_wavefield* w, fomo_shot* s, fomo_bat
   const int q = (s == w->placeholder) ? 0 : 1;
   fomo_batch* b = w->single[q];
   if (b && w->single_id[q] == s->id && b->cpt == c->cpt && b->tz_req == c->tz &&
-      b->graph_steps == c->graph_steps && b->step_mode_req == c->step_mode) {
+      b->graph_steps == c->graph_steps && b->step_mode_req == c->step_mode && b->tze == c->tze &&
+      b->int_bpc == c->int_bpc) {
     *out = b;
     return FOMO_OK;
   }
@@ -1349,18 +1440,25 @@ template <int K, int M, int CPT>
 static void launch_stencil_mc(const fomo_batch* b, int table, int kind, int flags, int k_off) {
   dim3 block(64 * WPB);
   hipStream_t st = lstream(b);
-  if (kind == 2) {  // grid rounded up to whole XCD rounds (see xcd_block)
-    const dim3 grid(round_up(blocks_for(b, PART_FUSED), NUM_XCD));
+  if (kind == 2 || kind == 4 || kind == 5) {  // grid rounded up to whole XCD rounds (see xcd_block)
+    // 4: the split step's edge tiles, 5: its interior tiles (at most int_bpc
+    // blocks per CU, so that the strip kernels beside them find free slots)
+    const int part = kind == 2 ? PART_FUSED : kind == 4 ? PART_FUSED_EDGE : PART_FUSED_INT;
+    if (kind == 4) flags |= FLAG_EDGE;
+    if (kind == 5) flags |= FLAG_INT;
+    int nb = blocks_for(b, part);
+    if (kind == 5 && b->int_bpc > 0) nb = std::min(nb, b->int_bpc * std::max(b->ctx->cus, 1));
+    const dim3 grid(round_up(nb, NUM_XCD));
     if (b->same && b->compact) {
       if constexpr (K == ST_F32)
         hipLaunchKernelGGL((k_fused<M, CPT, WPB, K, true, true>), grid, block, 0, st,
-                           args_for(b, table, PART_FUSED, k_off), flags);
+                           args_for(b, table, part, k_off), flags);
     } else if (b->same) {
       hipLaunchKernelGGL((k_fused<M, CPT, WPB, K, true, false>), grid, block, 0, st,
-                         args_for(b, table, PART_FUSED, k_off), flags);
+                         args_for(b, table, part, k_off), flags);
     } else {
       hipLaunchKernelGGL((k_fused<M, CPT, WPB, K, false, false>), grid, block, 0, st,
-                         args_for(b, table, PART_FUSED, k_off), flags);
+                         args_for(b, table, part, k_off), flags);
     }
   } else if (kind == 3) {
     if constexpr (CPT == 1)  // one instance per (K, M)
@@ -1461,6 +1559,28 @@ static void launch_frame(const fomo_batch* b, const FrameArgs& fa0, int table, i
 // velocity HABC, band stress, stress HABC, free surface + receivers.
 // Two-pass: in place, with backup and injection inside the stencil kernels.
 static void launch_step(const fomo_batch* b, int parity, int k_off) {
+  if (b->split) {
+    // edge tiles; then the interior tiles beside the strip kernels, which
+    // only read and write the band frame (the interior's cells are 2M + 2
+    // cells inside it); both joined before the next step
+    const int t = TBL_FUSED0 + parity;
+    hipStream_t st = lstream(b), ax = b->ctx->aux;
+    launch_stencil(b, t, 4, FLAG_INJECT, k_off);
+    (void)hipEventRecord(b->ev_edge, st);
+    (void)hipStreamWaitEvent(ax, b->ev_edge, 0);
+    b->cur = ax;
+    launch_habc(b, t, false, k_off);
+    launch_stencil(b, t, 3, FLAG_INJECT, k_off);
+    launch_habc(b, t, true, k_off);
+    launch_fsrec(b, t, b->rec_in_frame ? (MODE_FS | MODE_RECORD) : MODE_FS, k_off);
+    (void)hipEventRecord(b->ev_chain, ax);
+    b->cur = nullptr;
+    launch_stencil(b, t, 5, FLAG_INJECT, k_off);
+    (void)hipStreamWaitEvent(st, b->ev_chain, 0);
+    if (!b->rec_in_frame) launch_fsrec(b, t, MODE_RECORD, k_off);
+    if (b->frames_id) launch_frame(b, b->fargs, t, k_off, -1);
+    return;
+  }
   if (b->step_mode == 1) {
     const int t = TBL_FUSED0 + parity;
     launch_stencil(b, t, 2, FLAG_INJECT, k_off);
@@ -1799,7 +1919,7 @@ extern "C" int32_t fomo_record_receivers(fomo_model* m, fomo_wavefield* w, fomo_
 extern "C" int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t which,
                                      int32_t k, int32_t reps) {
   if (!m || !w || !s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
-  if (which < 0 || which > 8 || reps < 0) return fail(m->ctx, FOMO_ERR_ARG, "bad stage or reps");
+  if (which < 0 || which > 10 || reps < 0) return fail(m->ctx, FOMO_ERR_ARG, "bad stage or reps");
   if (which < 4 && m->storage != FOMO_STORAGE_F32)
     return fail(m->ctx, FOMO_ERR_UNSUPPORTED, "the in-place stages need FP32 storage");
   fomo_batch* b = nullptr;
@@ -1819,6 +1939,8 @@ extern "C" int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot
       case 6: launch_stencil(b, tf, 3, FLAG_INJECT, 0); break;
       case 7: launch_habc(b, tf, false, 0); break;
       case 8: launch_habc(b, tf, true, 0); break;
+      case 9: launch_stencil(b, tf, 4, FLAG_INJECT, 0); break;
+      case 10: launch_stencil(b, tf, 5, FLAG_INJECT, 0); break;
     }
   }
   return check_launch(m->ctx, "launch_stage");

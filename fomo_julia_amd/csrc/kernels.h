@@ -16,7 +16,9 @@
 
 namespace fomo {
 
-enum { FLAG_BACKUP = 1, FLAG_INJECT = 2 };
+// FLAG_EDGE / FLAG_INT: a fused launch over the edge or interior tiles of
+// the split step (ShotDev::xi0 ...); neither: the whole grid.
+enum { FLAG_BACKUP = 1, FLAG_INJECT = 2, FLAG_EDGE = 4, FLAG_INT = 8 };
 enum { MODE_FS = 1, MODE_RECORD = 2 };
 
 __device__ __forceinline__ float fmaf_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -786,6 +788,61 @@ __global__ __launch_bounds__(64 * WPB) void k_stress(BatchArgs ba, int flags) {
 #endif
 constexpr int fused_block(int M) { return 2 * M + 2 * FUSED_PD; }
 
+// Tile t of a fused launch: first written column o0, end of the written
+// columns xend, rows [j0, j1).  Whole grid: ftiles_x x ftiles_z tiles of
+// ftw x tz.  FLAG_INT: the interior rectangle in ftw x tzi tiles.  FLAG_EDGE:
+// the top and bottom blocks across the grid, then the left and right blocks
+// beside the interior, in ftw x tze tiles (see ShotDev).
+__device__ __forceinline__ void fused_tile(const ShotDev& sd, int t, int flags, int& o0, int& xend,
+                                           int& j0, int& j1) {
+  const int ftw = uniform(sd.ftw), nx = uniform(sd.nx), nz = uniform(sd.nz);
+  if (flags & FLAG_INT) {
+    const int nix = uniform(sd.nix), tzi = uniform(sd.tzi);
+    o0 = uniform(sd.xi0) + (t % nix) * ftw;
+    xend = uniform(sd.xi1);
+    j0 = uniform(sd.zi0) + (t / nix) * tzi;
+    j1 = min(j0 + tzi, uniform(sd.zi1));
+  } else if (flags & FLAG_EDGE) {
+    const int tze = uniform(sd.tze), ftx = uniform(sd.ftiles_x);
+    const int e_top = uniform(sd.e_top), e_bot = uniform(sd.e_bot);
+    const int zi0 = uniform(sd.zi0), zi1 = uniform(sd.zi1);
+    if (t < e_top) {
+      o0 = (t % ftx) * ftw;
+      xend = nx;
+      j0 = (t / ftx) * tze;
+      j1 = min(j0 + tze, zi0);
+    } else if (t < e_top + e_bot) {
+      const int u = t - e_top;
+      o0 = (u % ftx) * ftw;
+      xend = nx;
+      j0 = zi1 + (u / ftx) * tze;
+      j1 = min(j0 + tze, nz);
+    } else {
+      const int u = t - e_top - e_bot;
+      const int ntl = uniform(sd.ntx_l), ntr = uniform(sd.ntx_r), e_mid = uniform(sd.e_mid);
+      int row;
+      if (u < ntl * e_mid) {
+        o0 = (u % ntl) * ftw;
+        xend = uniform(sd.xi0);
+        row = u / ntl;
+      } else {
+        const int v = u - ntl * e_mid;
+        o0 = uniform(sd.xi1) + (v % ntr) * ftw;
+        xend = nx;
+        row = v / ntr;
+      }
+      j0 = zi0 + row * tze;
+      j1 = min(j0 + tze, zi1);
+    }
+  } else {
+    const int ftiles_x = uniform(sd.ftiles_x), tz = uniform(sd.tz);
+    o0 = (t % ftiles_x) * ftw;
+    xend = nx;
+    j0 = (t / ftiles_x) * tz;
+    j1 = min(j0 + tz, nz);
+  }
+}
+
 template <int M, int CPT, int WPB, int K, bool SAME, bool CMP>
 __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
   static_assert(!CMP || (K == ST_F32 && SAME), "derived coefficients: FP32, dtx == dtz");
@@ -815,10 +872,8 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     const int t = item - ba.prefix[s];
     const int nx = uniform(sd.nx), nz = uniform(sd.nz), P = uniform(sd.pitch);
     const int fs = uniform(sd.fs), band = uniform(sd.band);
-    const int ftiles_x = uniform(sd.ftiles_x), tz = uniform(sd.tz), ftw = uniform(sd.ftw);
-    const int o0 = (t % ftiles_x) * ftw;      // first written column
-    const int j0 = (t / ftiles_x) * tz;
-    const int j1 = min(j0 + tz, nz);
+    int o0, xend, j0, j1;  // first written column, end of the written columns, rows
+    fused_tile(sd, t, flags, o0, xend, j0, j1);
     const int ib0 = o0 - HL;                  // first lane column of the tile
     const int lo = lane * CPT;                // this lane's column offset in the tile
     const int ib = ib0 + lo;                  // this lane's first column
@@ -1039,13 +1094,13 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
         {
           const bool vrow = jv >= j0 && jv < j1;
           if (CPT == 2 && pairs) {
-            const uint32_t vo = (vrow && own_lane && ib < nx) ? lvo : OOB;
+            const uint32_t vo = (vrow && own_lane && ib < xend) ? lvo : OOB;
             bstore2<KF>(rB, vo, frow(F_VX, jv, ib0), nvx.v[0], nvx.v[CPT - 1], wvxs);
             bstore2<KF>(rB, vo, frow(F_VZ, jv, ib0), nvz.v[0], nvz.v[CPT - 1], wvzs);
           } else {
 #pragma unroll
             for (int c = 0; c < CPT; ++c) {
-              const uint32_t vo = (vrow && own_lane && ib + c < nx) ? lvo + (uint32_t)(c * FES) : OOB;
+              const uint32_t vo = (vrow && own_lane && ib + c < xend) ? lvo + (uint32_t)(c * FES) : OOB;
               bstore<KF>(rB, vo, frow(F_VX, jv, ib0), nvx.v[c], wvxs);
               bstore<KF>(rB, vo, frow(F_VZ, jv, ib0), nvz.v[c], wvzs);
             }
@@ -1113,7 +1168,8 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
           const bool band_row = j >= nz - band || (!fs && j < band);
           const bool wrow = srow && !band_row;
           if (CPT == 2 && pairs) {
-            const uint32_t vo = (wrow && own_lane && ib >= band_lo && ib < band_hi) ? lvo : OOB;
+            const uint32_t vo =
+                (wrow && own_lane && ib >= band_lo && ib < band_hi && ib < xend) ? lvo : OOB;
             bstore2<KF>(rB, vo, frow(F_TXX, j, ib0), oxx.v[0], oxx.v[CPT - 1], wxx);
             bstore2<KF>(rB, vo, frow(F_TZZ, j, ib0), ozz.v[0], ozz.v[CPT - 1], wzz);
             bstore2<KF>(rB, vo, frow(F_TXZ, j, ib0), oxz.v[0], oxz.v[CPT - 1], wxz);
@@ -1121,7 +1177,8 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
 #pragma unroll
             for (int c = 0; c < CPT; ++c) {
               const int i = ib + c;
-              const uint32_t vo = (wrow && own_lane && i >= band && i < nx - band) ? lvo + (uint32_t)(c * FES) : OOB;
+              const uint32_t vo = (wrow && own_lane && i >= band && i < nx - band && i < xend)
+                                      ? lvo + (uint32_t)(c * FES) : OOB;
               bstore<KF>(rB, vo, frow(F_TXX, j, ib0), oxx.v[c], wxx);
               bstore<KF>(rB, vo, frow(F_TZZ, j, ib0), ozz.v[c], wzz);
               bstore<KF>(rB, vo, frow(F_TXZ, j, ib0), oxz.v[c], wxz);

@@ -96,8 +96,14 @@ class MI355XBackend:
     tiling = (1, 0, 16)  # the library's default shape
 
     def set_step_mode(self, mode: int = 1) -> None:
-        """1 = fused single-pass step on ping-pong buffers (default), 0 = two-pass in place."""
+        """1 = fused single-pass step on ping-pong buffers (default), 0 = two-pass in
+        place, 2 = fused with the interior tiles beside the strip kernels."""
         self.check(self.lib.fomo_ctx_set_step_mode(self.ctx, int(mode)), "set_step_mode")
+
+    def set_split(self, edge_rows: int = 16, interior_blocks_per_cu: int = 0) -> None:
+        """Split-step shape (step mode 2): rows per edge tile, interior blocks per CU cap."""
+        self.check(self.lib.fomo_ctx_set_split(self.ctx, int(edge_rows), int(interior_blocks_per_cu)),
+                   "set_split")
 
     def stream_handle(self) -> int:
         return int(self.lib.fomo_ctx_stream(self.ctx) or 0)

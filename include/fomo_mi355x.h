@@ -292,9 +292,19 @@ int32_t fomo_pool_next_step(const fomo_pool* p, const fomo_shot* s, int32_t* k_n
  *   HABC, the stress cells beside the HABC strips, the stress HABC, free
  *   surface and receivers;
  * 0 = two-pass in place: velocity pass (with strip backup and source), HABC,
- *   stress pass, HABC, free surface and receivers.
- * Both give identical results. */
+ *   stress pass, HABC, free surface and receivers;
+ * 2 = split fused: the fused pass over the tiles near the edges first, then
+ *   the interior tiles (whose cells lie 2M + 2 cells inside the band frame
+ *   the strip kernels write) on the context's stream while the HABC, band
+ *   stress and free surface run on a second stream of the context; the two
+ *   are joined before the next step.  Batches only (pools run mode 1).
+ * All give identical results. */
 int32_t fomo_ctx_set_step_mode(fomo_ctx* ctx, int32_t mode);
+/* Split step shape (mode 2): rows per edge tile (1..256, default 16) and the
+ * most interior blocks per compute unit (0 = as many as fit), which leaves
+ * room on the GPU for the strip kernels that run beside them.  Applies to
+ * runs created after the call.  No counterpart in the reference. */
+int32_t fomo_ctx_set_split(fomo_ctx* ctx, int32_t edge_rows, int32_t interior_blocks_per_cu);
 /* Stencil tile: 64*cols_per_lane columns (cols_per_lane in {1,2,4}) by
  * rows_per_tile rows per wavefront (0 = chosen so that the tiles fill the
  * GPU's resident wavefronts in whole rounds); steps_per_graph = whole steps
@@ -306,7 +316,9 @@ int32_t fomo_ctx_set_tiling(fomo_ctx* ctx, int32_t cols_per_lane, int32_t rows_p
  * for timing: 0 = velocity pass, 1 = stress pass (two-pass step, with strip
  * backup and injection), 2 = HABC velocity, 3 = HABC stress (in place),
  * 4 = free surface + record, 5 = fused velocity+stress pass, 6 = band
- * stress, 7 = HABC velocity, 8 = HABC stress (fused-step buffer roles). */
+ * stress, 7 = HABC velocity, 8 = HABC stress (fused-step buffer roles),
+ * 9 = fused pass over the split step's edge tiles, 10 = over its interior
+ * tiles (mode 2's geometry). */
 int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t stage,
                           int32_t k, int32_t reps);
 

@@ -34,8 +34,9 @@ def _f16_rne(v, scale):
     return (np.float16(v.astype(np.float64) * scale).astype(np.float64) / scale).astype(F32)
 
 
-@pytest.mark.parametrize("storage", ["f32", "bf16", "f16", "f16c"])
-def test_seam_random_state_bit_exact(storage):
+@pytest.mark.parametrize("storage,mode", [("f32", 1), ("bf16", 1), ("f16", 1), ("f16c", 1),
+                                          ("f32", 2), ("bf16", 2)])
+def test_seam_random_state_bit_exact(storage, mode):
     import oracle as O
     dx = F32(12.5)
     vp, vs, rho = media.seam_like()
@@ -51,12 +52,12 @@ def test_seam_random_state_bit_exact(storage):
     xr = np.arange(0, vp.shape[0], 7, dtype=F32) * dx
     types = [("vz", "vx", "p")[k % 3] for k in range(len(xr))]
     rec = E.setup_receivers(xr, np.full(len(xr), 2 * dx, F32), M, type=types)
-    print(f"case SEAM random state {storage} {M.nx}x{M.nz}, {NT} steps, {len(xr)} receivers",
-          flush=True)
+    print(f"case SEAM random state {storage} mode {mode} {M.nx}x{M.nz}, {NT} steps, "
+          f"{len(xr)} receivers", flush=True)
 
     b = E.backend("mi355x")
     b.set_tiling(2, 0, 16)
-    b.set_step_mode(1)
+    b.set_step_mode(mode)
     W = E.Wavefield(M.nx, M.nz, b)
     dm = E.bind(b, W, M, H, a, p)
     scales = np.ones(5, F32)
@@ -91,8 +92,11 @@ def test_seam_random_state_bit_exact(storage):
         else:
             r.field(f)[:] = v
         del v
-    E.run_time_loop(b, W, M, H, a, src, rec, p)
-    b.set_tiling(1, 0, 16)
+    try:
+        E.run_time_loop(b, W, M, H, a, src, rec, p)
+    finally:
+        b.set_tiling(1, 0, 16)
+        b.set_step_mode(1)
     r.run(1, NT)
     assert np.array_equal(rec.data, r.gather_nt_by_nrec())
     for f in range(5):

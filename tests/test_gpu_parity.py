@@ -44,7 +44,7 @@ def assert_same_state(W, r, what=""):
                                  f"got {got[i, j]!r} want {want[i, j]!r}")
 
 
-@pytest.mark.parametrize("mode", [1, 0])
+@pytest.mark.parametrize("mode", [1, 0, 2])
 @pytest.mark.parametrize("fs", [True, False])
 @pytest.mark.parametrize("src_type", ["pressure", "force_z", "force_x"])
 def test_full_shot_bit_exact(b, fs, src_type, mode):
@@ -58,7 +58,7 @@ def test_full_shot_bit_exact(b, fs, src_type, mode):
     assert_same_state(W, r)
 
 
-@pytest.mark.parametrize("mode", [1, 0])
+@pytest.mark.parametrize("mode", [1, 0, 2])
 @pytest.mark.parametrize("fd_order", [2, 4, 6, 10, 12, 16])
 def test_orders_bit_exact(b, fd_order, mode):
     c = Case(fd_order=fd_order, nt=80, nbc=10)
@@ -71,7 +71,7 @@ def test_orders_bit_exact(b, fd_order, mode):
     assert_same_state(W, r)
 
 
-@pytest.mark.parametrize("mode", [1, 0])
+@pytest.mark.parametrize("mode", [1, 0, 2])
 @pytest.mark.parametrize("cpt,tz,gsteps", [(1, 8, 2), (2, 17, 6), (4, 64, 32), (4, 5, 16), (1, 0, 4)])
 def test_tilings_bit_exact(b, cpt, tz, gsteps, mode):
     c = Case(nx_in=150, nz_in=90, nbc=20, nt=71)
@@ -83,6 +83,32 @@ def test_tilings_bit_exact(b, cpt, tz, gsteps, mode):
     assert np.array_equal(res.gather, r.gather_nt_by_nrec())
     assert_same_state(W, r)
     b.set_tiling(1, 0, 16)
+
+
+@pytest.mark.parametrize("edge_rows,bpc", [(1, 0), (7, 1), (16, 2), (64, 0)])
+@pytest.mark.parametrize("rec_where", ["frame", "interior"])
+def test_split_step_shapes_bit_exact(b, edge_rows, bpc, rec_where):
+    """Step mode 2 (edge tiles, then interior tiles beside the strip kernels):
+    edge tile heights, caps on the interior's blocks per CU, receivers only
+    in the frame (recorded beside the strip kernels) or also inside the
+    interior (recorded after the join), two columns per lane, odd nx."""
+    c = Case(nx_in=151, nz_in=97, nbc=14, nt=75, rec_types=("vz", "vx", "p"))
+    if rec_where == "interior":
+        xr = np.linspace(0.0, float(c.M.x_max), 41).astype(F32)
+        zr = np.linspace(0.0, 90.0 * float(c.M.dz), 41).astype(F32)
+        c.rec = E.setup_receivers(xr, zr, c.M, type=[("vz", "vx", "p")[k % 3] for k in range(41)])
+    print(f"case split edge_rows={edge_rows} bpc={bpc} receivers={rec_where}", flush=True)
+    b.set_split(edge_rows, bpc)
+    try:
+        res, W = gpu_shot(b, c, 2, 0, 8, 2)
+    finally:
+        b.set_split(16, 0)
+        b.set_step_mode(1)
+        b.set_tiling(1, 0, 16)
+    r = c.oracle()
+    r.run(1, c.nt)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    assert_same_state(W, r)
 
 
 STAGES = [("backup", E.backup_boundary), ("velocity", E.update_velocity),
