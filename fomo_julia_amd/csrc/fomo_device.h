@@ -70,7 +70,7 @@ struct ShotDev {
   // in nix x niz tiles of ftw x tzi; the cells any of them reads are at least
   // 2M + 2 cells inside the band frame that the strip kernels (HABC, band
   // stress, free surface) write, so interior tiles of a step may run while
-  // those kernels run.  Edge: the rest, in ftw x tze tiles: top block rows
+  // the edge tiles and then those kernels run.  Edge: the rest, in ftw x tze tiles: top block rows
   // [0, zi0) and bottom block rows [zi1, nz) across the grid (e_top, e_bot
   // tiles), then the left block columns [0, xi0) and right block columns
   // [xi1, nx) of rows [zi0, zi1) (ntx_l, ntx_r tile columns of e_mid rows).

@@ -1560,15 +1560,18 @@ static void launch_frame(const fomo_batch* b, const FrameArgs& fa0, int table, i
 // Two-pass: in place, with backup and injection inside the stencil kernels.
 static void launch_step(const fomo_batch* b, int parity, int k_off) {
   if (b->split) {
-    // edge tiles; then the interior tiles beside the strip kernels, which
-    // only read and write the band frame (the interior's cells are 2M + 2
-    // cells inside it); both joined before the next step
+    // The interior tiles on the batch's stream, beside the edge tiles and
+    // then the strip kernels on the aux stream: the interior and the edge
+    // tiles both read only the previous state and write disjoint cells, and
+    // the strip kernels read and write only the band frame, whose cells the
+    // interior tiles neither read nor write (they lie 2M + 2 cells inside
+    // it).  Both streams are joined before the next step.
     const int t = TBL_FUSED0 + parity;
     hipStream_t st = lstream(b), ax = b->ctx->aux;
-    launch_stencil(b, t, 4, FLAG_INJECT, k_off);
     (void)hipEventRecord(b->ev_edge, st);
     (void)hipStreamWaitEvent(ax, b->ev_edge, 0);
     b->cur = ax;
+    launch_stencil(b, t, 4, FLAG_INJECT, k_off);
     launch_habc(b, t, false, k_off);
     launch_stencil(b, t, 3, FLAG_INJECT, k_off);
     launch_habc(b, t, true, k_off);

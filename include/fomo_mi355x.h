@@ -293,11 +293,12 @@ int32_t fomo_pool_next_step(const fomo_pool* p, const fomo_shot* s, int32_t* k_n
  *   surface and receivers;
  * 0 = two-pass in place: velocity pass (with strip backup and source), HABC,
  *   stress pass, HABC, free surface and receivers;
- * 2 = split fused: the fused pass over the tiles near the edges first, then
- *   the interior tiles (whose cells lie 2M + 2 cells inside the band frame
- *   the strip kernels write) on the context's stream while the HABC, band
- *   stress and free surface run on a second stream of the context; the two
- *   are joined before the next step.  Batches only (pools run mode 1).
+ * 2 = split fused: the fused pass over the interior tiles (whose cells lie
+ *   2M + 2 cells inside the band frame the strip kernels write) runs on the
+ *   context's stream while, on a second stream of the context, the fused pass
+ *   over the tiles near the edges and then the HABC, band stress and free
+ *   surface run; the two are joined before the next step.  Batches only
+ *   (pools run mode 1).
  * All give identical results. */
 int32_t fomo_ctx_set_step_mode(fomo_ctx* ctx, int32_t mode);
 /* Split step shape (mode 2): rows per edge tile (1..256, default 16) and the
