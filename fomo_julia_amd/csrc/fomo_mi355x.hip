@@ -376,7 +376,7 @@ extern "C" int32_t fomo_ctx_set_tiling(fomo_ctx* c, int32_t cpt, int32_t tz, int
   if (!c) return fail(nullptr, FOMO_ERR_ARG, "ctx is NULL");
   if (cpt != 1 && cpt != 2 && cpt != 4) return fail(c, FOMO_ERR_ARG, "cols_per_lane must be 1, 2 or 4");
   if (tz < 0 || tz > 4096) return fail(c, FOMO_ERR_ARG, "rows_per_tile out of range (0 = auto)");
-  if (gsteps < 2 || gsteps > 256 || gsteps % 2)
+  if (gsteps != 0 && (gsteps < 2 || gsteps > 256 || gsteps % 2))
     return fail(c, FOMO_ERR_ARG, "steps_per_graph must be even, 2..256");
   c->cpt = cpt;
   c->tz = tz;
@@ -1677,8 +1677,8 @@ static int run_batch(fomo_batch* b, int k_first, int n_steps) {
   }
   HIP_TRY(c, hipSetDevice(c->device));
   launch_set_step(b, k_first);
-  const int G = b->graph_steps;  // even: a chunk ends on the parity it started from
-  const int chunks = n_steps / G, rem = n_steps % G;
+  const int G = b->graph_steps;  // even: a chunk ends on the parity it started from; 0: no graphs
+  const int chunks = G > 0 ? n_steps / G : 0, rem = G > 0 ? n_steps % G : n_steps;
   const int flip = b->step_mode == 1 ? 1 : 0;
   if (chunks > 0 && !b->gexec[parity]) {
     int rc = capture_chunk(b, parity);
@@ -1731,7 +1731,7 @@ extern "C" int32_t fomo_run_time_loop_prepare(fomo_model* m, fomo_wavefield* w, 
   if (rc) return rc;
   set_frames(b, nullptr);
   HIP_TRY(m->ctx, hipSetDevice(m->ctx->device));
-  const int parities = b->step_mode == 1 ? 2 : 1;
+  const int parities = b->graph_steps == 0 ? 0 : b->step_mode == 1 ? 2 : 1;
   for (int q = 0; q < parities; ++q) {
     const int par = (w->cur + q) & 1;
     if (!b->gexec[par]) {
@@ -2249,8 +2249,8 @@ extern "C" int32_t fomo_pool_run(fomo_pool* p, int32_t n_steps) {
     if (rc) return rc;
   }
   if (p->ss.empty() || n_steps == 0) return FOMO_OK;
-  const int G = b->graph_steps;  // even: every chunk starts on table 0
-  const int chunks = n_steps / G, rem = n_steps % G;
+  const int G = b->graph_steps;  // even: every chunk starts on table 0; 0: no graphs
+  const int chunks = G > 0 ? n_steps / G : 0, rem = G > 0 ? n_steps % G : n_steps;
   if (chunks > 0 && !b->gexec[0]) {
     int rc = capture_chunk(b, 0);
     if (rc) return rc;

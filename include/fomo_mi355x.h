@@ -309,7 +309,8 @@ int32_t fomo_ctx_set_split(fomo_ctx* ctx, int32_t edge_rows, int32_t interior_bl
 /* Stencil tile: 64*cols_per_lane columns (cols_per_lane in {1,2,4}) by
  * rows_per_tile rows per wavefront (0 = chosen so that the tiles fill the
  * GPU's resident wavefronts in whole rounds); steps_per_graph = whole steps
- * per captured graph (even).  Applies to runs created after the call on
+ * per captured graph (even, 2..256), or 0 to launch every step's kernels
+ * directly, without graphs.  Applies to runs created after the call on
  * this context. */
 int32_t fomo_ctx_set_tiling(fomo_ctx* ctx, int32_t cols_per_lane, int32_t rows_per_tile,
                             int32_t steps_per_graph);
