@@ -47,6 +47,8 @@ BYTES_FUSED_STORAGE = {"f32": 60, "bf16": 30, "f16": 30, "f16c": 50}
 BYTES_VELOCITY = 36  # two-pass velocity: read txx tzz txz, 2 buoyancies; read+write vx vz
 BYTES_STRESS = 44    # two-pass stress: read vx vz lam lam+2mu mu_txz; read+write txx tzz txz
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fused_seam_traffic.txt")
+# the same for the split step's interior launches of k_fused
+TRAFFIC_FILE_SPLIT = os.path.join(ROOT, "profiles", "k_fused_interior_seam_traffic.txt")
 METRIC = "Mcell-updates/s & %HBM roofline, 1 GPU; shots/hr + p99 latency (served)"
 # The autotuner's trials and the per-kernel timings run on a timing shot of
 # TIMING_NT steps (same medium, source cell and receivers), so that every
@@ -540,9 +542,10 @@ def main():
     ap.add_argument("--stage-reps", type=int, default=50)
     ap.add_argument("--tiling", default="", help="cols_per_lane,rows_per_tile,steps_per_graph")
     ap.add_argument("--two-pass", action="store_true", help="time the two-pass in-place step instead")
-    ap.add_argument("--split", default="", metavar="EDGE_ROWS,BLOCKS_PER_CU",
-                    help="split fused step: edge tiles first, then the interior tiles beside the "
-                         "strip kernels (rows per edge tile, interior blocks per CU, 0 = no cap)")
+    ap.add_argument("--split", default="16,1", metavar="EDGE_ROWS,BLOCKS_PER_CU",
+                    help="split fused step (the default, 16,1): the interior tiles beside the edge "
+                         "tiles and the strip kernels (rows per edge tile, interior blocks per CU, "
+                         "0 = no cap); 'off' runs the whole-grid fused step")
     ap.add_argument("--storage", default="f32", choices=sorted(BYTES_FUSED_STORAGE),
                     help="device storage of coefficients/fields (arithmetic stays FP32)")
     ap.add_argument("--workload", default="seam",
@@ -609,7 +612,7 @@ def main():
 
     b = E.backend("mi355x")
     fused = not args.two_pass
-    split = fused and bool(args.split)
+    split = fused and args.split not in ("", "off")
     b.set_step_mode(2 if split else 1 if fused else 0)
     if split:
         er, bpc = (int(v) for v in args.split.split(","))
@@ -713,8 +716,10 @@ def main():
     step_gbs = step_bytes * cells / (ms_per_step / 1e3) / 1e9
     check_roofline(main_gbs / HBM_PEAK_GBS, main_ms, ms_per_step)
     traffic = None
-    if fused and not split and args.storage == "f32" and os.path.exists(TRAFFIC_FILE):
-        for line in open(TRAFFIC_FILE):
+    tfile = TRAFFIC_FILE_SPLIT if split else TRAFFIC_FILE
+    if fused and args.storage == "f32" and os.path.exists(tfile) and \
+            (not split or args.split == "16,1"):
+        for line in open(tfile):
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = int(line.split(":")[1])
 
@@ -779,7 +784,7 @@ def main():
                                              + upload_s + gather_s, 3)}},
         "roofline": {"bound": "hbm", "achieved": round(main_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(main_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_source": ("profiles/k_fused_seam_traffic.txt: rocprofv3 --pmc "
+                     "traffic_source": (f"profiles/{os.path.basename(tfile)}: rocprofv3 --pmc "
                                         "FETCH_SIZE and WRITE_SIZE per launch, scaled by a plain "
                                         "copy's counts (not measured in this run)") if traffic else None,
                      "kernel": main_name, "kernel_ms": round(main_ms, 5),
