@@ -455,19 +455,32 @@ def bench_serve(n_req: int, seed: int = 5, load: float = 0.8) -> dict:
     from fomo_julia_amd import serve as S
     b = E.backend("mi355x")
     reqs = S.c5_stream(n_req, seed=seed)
+    every = max(1, n_req // 20)
+
+    def progress(phase):
+        count = [0]
+        t_start = time.perf_counter()
+
+        def cb(c):
+            count[0] += 1
+            if count[0] % every == 0 or count[0] == n_req:
+                print(f"served sample, {phase}: {count[0]}/{n_req} shots done, "
+                      f"{time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
+        return cb
+
     t = time.perf_counter()
     prep_times = []
-    ser = S.serial(reqs, b, storage="f32", prep_times=prep_times)
+    ser = S.serial(reqs, b, storage="f32", prep_times=prep_times, on_complete=progress("serial FP32"))
     w_ser = time.perf_counter() - t
     w_ser_gpu = w_ser - sum(prep_times)  # the serial loop without its host preparation
     srv = S.Server(b, chunk=64, capacity=64)
     t = time.perf_counter()
-    sat = srv.serve(reqs, realtime=False)
+    sat = srv.serve(reqs, realtime=False, on_complete=progress("capacity"))
     w_sat = time.perf_counter() - t
     rate = load / (w_ser / n_req)
     srv2 = S.Server(b, chunk=64, capacity=64)
     t = time.perf_counter()
-    poi = srv2.serve(S.with_arrivals(reqs, rate), realtime=True)
+    poi = srv2.serve(S.with_arrivals(reqs, rate), realtime=True, on_complete=progress("Poisson"))
     w_poi = time.perf_counter() - t
     s_ser, s_sat, s_poi = S.summarize(ser, w_ser), S.summarize(sat, w_sat), S.summarize(poi, w_poi)
     return {"requests": n_req, "serial_f32": {**s_ser, "prepare_s": round(sum(prep_times), 3)},

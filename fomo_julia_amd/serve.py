@@ -436,7 +436,7 @@ class Server:
 
 
 def serial(requests: list, b: Optional[E.MI355XBackend] = None, storage: Optional[str] = "f32",
-           prep_times: Optional[list] = None) -> list:
+           prep_times: Optional[list] = None, on_complete=None) -> list:
     """The same requests one at a time with the plain single-shot path
     (``run_shot``), storage forced to `storage` (None keeps each request's).
     Each request's host preparation runs in the loop, as a client's would;
@@ -455,6 +455,8 @@ def serial(requests: list, b: Optional[E.MI355XBackend] = None, storage: Optiona
             res = E.run_shot(b, W, prep.M, prep.H, prep.a, prep.src, prep.rec, prep.p)
         out.append(Completed(r.rid, ta, ta, time.perf_counter() - t0, res.gather, rr.storage,
                              rr.fd_order, prep.M.nx * prep.M.nz * prep.p.nt))
+        if on_complete:
+            on_complete(out[-1])
         b._models.clear()
         del W
     return out
