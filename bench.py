@@ -255,6 +255,11 @@ def bench_batch(args) -> dict:
     a = I.get_fd_coefficients(12)
     wav = I.ricker_wavelet(F32(25.0), dt, nt)
     b = E.backend("mi355x")
+    split = args.split not in (None, "", "off")
+    if split:
+        er, bpc = (int(v) for v in args.split.split(","))
+        b.set_step_mode(2)
+        b.set_split(er, bpc)
     W = E.Wavefield(M.nx, M.nz, b)
     dm = E.bind(b, W, M, H, a, p)
     nx_in = vp.shape[0]
@@ -298,7 +303,8 @@ def bench_batch(args) -> dict:
             "config": {"workload": f"{n_shots} batched shots, Marmousi-shaped 1700x350, dx=dz=10 m, "
                                    f"fd_order=12, nbc=40, nt={nt}, 25 Hz Ricker, 1700 receivers, "
                                    "one persistent launch per kernel, captured steps",
-                       "padded_cells_per_shot": M.nx * M.nz},
+                       "padded_cells_per_shot": M.nx * M.nz,
+                       "step_mode": ("split fused " + args.split) if split else "fused"},
             "mcell_updates_per_s": round(cells / gpu_s / 1e6, 1),
             "ms_per_step": round(gpu_s * 1e3 / nt, 4), "gpu_seconds": round(gpu_s, 3),
             "wall_seconds_with_gathers": round(wall, 3),
@@ -355,7 +361,8 @@ def bench_batch_serial(b, velocity, dx, dt, xs, nt: int, n_timed: int) -> dict:
             "ms_per_step": round(per_shot * 1e3 / nt, 4),
             "tiling": {"cols_per_lane": tuning["tiling"][0], "rows_per_tile": tuning["tiling"][1]},
             "note": "one shot at a time through run_shots(batch=1): run_shot on one wavefield, "
-                    "gather downloaded after each shot; synchronized wall time per shot"}
+                    "gather downloaded after each shot; synchronized wall time per shot; "
+                    "the batch's step mode"}
 
 
 def bench_edges(args) -> dict:
@@ -511,19 +518,34 @@ def pattern_probe(reps: int = 20) -> dict:
     lib.fomo_probe_pattern.argtypes = [ctypes.c_int] * 9 + [ctypes.POINTER(ctypes.c_double)]
     lib.fomo_probe_stream.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_double)]
+    lib.fomo_probe_runtime.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double)]
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     us = ctypes.c_double()
-    out = {"cus": cus, "stream": [], "pattern": []}
-    nbytes = 1 << 30
-    for mode, name, moved in ((0, "copy", 2), (1, "read", 1), (2, "write", 1)):
-        for grid in (cus * 8, cus * 32):
-            rc = lib.fomo_probe_stream(mode, nbytes, grid, reps, ctypes.byref(us))
-            if rc:
-                raise RuntimeError(f"fomo_probe_stream({name}, grid {grid}) returned {rc}")
-            gbs = moved * nbytes / (us.value * 1e3)
-            out["stream"].append({"kind": name, "grid": grid, "us": round(us.value, 1),
-                                  "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 3)})
-            print(json.dumps(out["stream"][-1]), file=sys.stderr, flush=True)
+    out = {"cus": cus, "stream": [], "runtime": [], "pattern": []}
+    for nbytes in (1 << 30, 4 << 30):
+        for mode, name, moved in ((0, "copy", 2), (1, "read", 1), (2, "write", 1)):
+            for grid in (cus * 8, cus * 32):
+                rc = lib.fomo_probe_stream(mode, nbytes, grid, reps, ctypes.byref(us))
+                if rc:
+                    raise RuntimeError(f"fomo_probe_stream({name}, grid {grid}) returned {rc}")
+                gbs = moved * nbytes / (us.value * 1e3)
+                out["stream"].append({"kind": name, "bytes": nbytes, "grid": grid,
+                                      "us": round(us.value, 1), "GB/s": round(gbs, 1),
+                                      "frac": round(gbs / HBM_PEAK_GBS, 3)})
+                print(json.dumps(out["stream"][-1]), file=sys.stderr, flush=True)
+    # the runtime's own copy and fill over 4 GiB: points that do not depend
+    # on the probe kernels above
+    nbytes = 4 << 30
+    for mode, name, moved in ((0, "hipMemcpyAsync D2D", 2), (1, "hipMemsetAsync", 1),
+                              (2, "hipMemsetD32Async", 1)):
+        rc = lib.fomo_probe_runtime(mode, nbytes, reps, ctypes.byref(us))
+        if rc:
+            raise RuntimeError(f"fomo_probe_runtime({name}) returned {rc}")
+        gbs = moved * nbytes / (us.value * 1e3)
+        out["runtime"].append({"kind": name, "bytes": nbytes, "us": round(us.value, 1),
+                               "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 3)})
+        print(json.dumps(out["runtime"][-1]), file=sys.stderr, flush=True)
     nx, nz, tz = 4608, 3920, 40
     for nread in (8, 12):
         for cpt in (1, 2, 4):
@@ -537,7 +559,7 @@ def pattern_probe(reps: int = 20) -> dict:
                                        "lds": lds, "us": round(us.value, 1), "GB/s": round(gbs, 1),
                                        "frac": round(gbs / HBM_PEAK_GBS, 3)})
                 print(json.dumps(out["pattern"][-1]), file=sys.stderr, flush=True)
-    for key in ("stream", "pattern"):
+    for key in ("stream", "runtime", "pattern"):
         best = max(out[key], key=lambda r: r["GB/s"])
         out[f"best_{key}"] = best
     capped = [r for r in out["pattern"] if r["nread"] == 8 and r["lds"]]
@@ -555,10 +577,12 @@ def main():
     ap.add_argument("--stage-reps", type=int, default=50)
     ap.add_argument("--tiling", default="", help="cols_per_lane,rows_per_tile,steps_per_graph")
     ap.add_argument("--two-pass", action="store_true", help="time the two-pass in-place step instead")
-    ap.add_argument("--split", default="16,1", metavar="EDGE_ROWS,BLOCKS_PER_CU",
-                    help="split fused step (the default, 16,1): the interior tiles beside the edge "
-                         "tiles and the strip kernels (rows per edge tile, interior blocks per CU, "
-                         "0 = no cap); 'off' runs the whole-grid fused step")
+    ap.add_argument("--split", default=None, metavar="EDGE_ROWS,BLOCKS_PER_CU",
+                    help="split fused step: the interior tiles beside the edge tiles and the strip "
+                         "kernels (rows per edge tile, interior blocks per CU, 0 = no cap); 'off' "
+                         "runs the whole-grid fused step.  Default: 16,1 for the single shot, off "
+                         "for the batch (whose edge tiles and strip kernels cover a third of its "
+                         "cells: split, its step is slower)")
     ap.add_argument("--storage", default="f32", choices=sorted(BYTES_FUSED_STORAGE),
                     help="device storage of coefficients/fields (arithmetic stays FP32)")
     ap.add_argument("--workload", default="seam",
@@ -625,6 +649,8 @@ def main():
 
     b = E.backend("mi355x")
     fused = not args.two_pass
+    if args.split is None:
+        args.split = "16,1"
     split = fused and args.split not in ("", "off")
     b.set_step_mode(2 if split else 1 if fused else 0)
     if split:

@@ -194,4 +194,30 @@ int fomo_probe_stream(int mode, size_t bytes, int grid, int reps, double* us) {
   return rc;
 }
 
+// The runtime's own streaming over `bytes` (a multiple of 16), as independent
+// points beside the kernels above: mode 0 hipMemcpyAsync device to device,
+// 1 hipMemsetAsync (bytes), 2 hipMemsetD32Async (32-bit words), all on the
+// null stream.  Same return codes as above.
+int fomo_probe_runtime(int mode, size_t bytes, int reps, double* us) {
+  if (!us || mode < 0 || mode > 2 || bytes == 0 || bytes % 16 || reps <= 0) return -1;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess) return -2;
+  if (mode == 0 && hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return -2;
+  }
+  int rc = -2;
+  if (hipMemset(a, 0, bytes) == hipSuccess && (!b || hipMemset(b, 0, bytes) == hipSuccess)) {
+    if (mode == 0)
+      rc = time_launches(nullptr, reps, us, [&] { (void)hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, nullptr); });
+    else if (mode == 1)
+      rc = time_launches(nullptr, reps, us, [&] { (void)hipMemsetAsync(a, 1, bytes, nullptr); });
+    else
+      rc = time_launches(nullptr, reps, us, [&] { (void)hipMemsetD32Async((hipDeviceptr_t)a, 0x3f800000u, bytes / 4, nullptr); });
+  }
+  (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
 }  // extern "C"

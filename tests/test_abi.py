@@ -61,3 +61,9 @@ def test_probe_library_rejects_shapes_before_any_device_call():
     assert lib.fomo_probe_pattern(*ok.values(), None) == -1
     for mode, nbytes, grid in ((3, 1 << 20, 8), (0, 0, 8), (0, 100, 8), (1, 1 << 20, 0)):
         assert lib.fomo_probe_stream(mode, nbytes, grid, 20, ctypes.byref(us)) == -1
+    lib.fomo_probe_runtime.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double)]
+    for mode, nbytes, reps in ((3, 1 << 20, 20), (-1, 1 << 20, 20), (0, 0, 20), (1, 100, 20),
+                               (2, 1 << 20, 0)):
+        assert lib.fomo_probe_runtime(mode, nbytes, reps, ctypes.byref(us)) == -1
+    assert lib.fomo_probe_runtime(0, 1 << 20, 20, None) == -1
