@@ -9,7 +9,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfomo_mi355x.so")
+# FOMO_MI355X_LIB selects another build of the library (the Julia binding
+# reads the same variable)
+LIB_PATH = os.environ.get("FOMO_MI355X_LIB") or os.path.join(HERE, "libfomo_mi355x.so")
 
 c_int32 = ctypes.c_int32
 c_int64 = ctypes.c_int64

@@ -108,12 +108,15 @@ def test_seam_random_state_bit_exact(storage, mode):
                                  f"got {got[i, j]!r} want {want[i, j]!r}")
 
 
-def test_marmousi_batch_of_64_random_states_bit_exact():
+@pytest.mark.parametrize("mode", [1, 2])
+def test_marmousi_batch_of_64_random_states_bit_exact(mode):
     """The batched workload at its full size: 64 Marmousi-shaped shots (1700 x
     350, nbc 40, order 12, padded 1792 x 442) in one batch, two columns per
     lane (the batch's autotuned shape), each from its own random state and
     source position; after 20 steps every shot's fields and gather equal its
-    own oracle run."""
+    own oracle run.  Mode 1 is the whole-grid fused step, mode 2 the split
+    step (interior tiles of all 64 shots beside their edge tiles and strip
+    kernels, 16 edge rows, one interior block per CU)."""
     import ctypes
     import oracle as O
     dx = F32(10.0)
@@ -127,11 +130,13 @@ def test_marmousi_batch_of_64_random_states_bit_exact():
     wav = I.ricker_wavelet(F32(25.0), dt, NT)
     nx_in = vp.shape[0]
     rec = E.setup_receivers(np.arange(nx_in, dtype=F32) * dx, np.zeros(nx_in, F32), M, type="vz")
-    print(f"case Marmousi batch {n_shots} x {M.nx}x{M.nz}, order {order}, {NT} steps", flush=True)
+    print(f"case Marmousi batch {n_shots} x {M.nx}x{M.nz}, order {order}, {NT} steps, "
+          f"step mode {mode}", flush=True)
 
     b = E.backend("mi355x")
     b.set_tiling(2, 0, 16)
-    b.set_step_mode(1)
+    b.set_step_mode(mode)
+    b.set_split(16, 1)
     W0 = E.Wavefield(M.nx, M.nz, b)
     dm = E.bind(b, W0, M, H, a, p)
     xs = ((np.arange(n_shots) + 0.5) * (float(M.x_max) / n_shots)).astype(F32)
@@ -159,6 +164,8 @@ def test_marmousi_batch_of_64_random_states_bit_exact():
     finally:
         b.lib.fomo_batch_destroy(bh)
         b.set_tiling(1, 0, 16)
+        b.set_step_mode(1)
+        b.set_split(16, 0)
     for s in range(n_shots):
         r = O.OracleRun(nx=M.nx, nz=M.nz, pad=M.pad, nbc=nbc, fd_order=order, free_surface=True,
                         dt=p.dt, dtx=p.dtx, dtz=p.dtz, q=(H.qx, H.qz, H.qt_x, H.qt_z, H.qxt), a=a,

@@ -244,14 +244,19 @@ def test_time_step_api_matches_run_loop(b, mode):
     b.set_step_mode(1)
 
 
-def test_batched_uneven_grids_bit_exact(b):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batched_uneven_grids_bit_exact(b, mode):
     cases = [Case(nx_in=72, nz_in=56, nt=90, seed=1),
              Case(nx_in=130, nz_in=64, nt=90, seed=2, fs=False),
-             Case(nx_in=60, nz_in=90, nt=90, seed=3, src_type="force_z")]
+             Case(nx_in=60, nz_in=90, nt=90, seed=3, src_type="force_z"),
+             Case(nx_in=180, nz_in=120, nt=90, seed=4, src_type="force_x")]
     import ctypes
     from fomo_julia_amd import _lib
     lib = _lib.load()
+    print(f"case uneven batch of {len(cases)}, step mode {mode}", flush=True)
     b.set_tiling(2, 16, 8)
+    b.set_step_mode(mode)
+    b.set_split(7, 1)
     hs = []
     for c in cases:
         dm = E._model(b, c.M, c.H, c.a, c.p)
@@ -274,6 +279,8 @@ def test_batched_uneven_grids_bit_exact(b):
         assert np.array_equal(ds.gather(), r.gather_nt_by_nrec())
         assert_same_state(W, r, "batch")
     b.set_tiling(1, 0, 16)
+    b.set_step_mode(1)
+    b.set_split(16, 0)
 
 
 def test_run_shots_batched_equals_sequential(b):
