@@ -43,7 +43,10 @@ BYTES_FUSED = 60     # read 5 old fields + dt/rho_vx, dt/rho_vz, lam, lam+2mu, m
 # reads rho_vx, lam, mu_txx and forms the other coefficients (5 + 3 reads, 5 writes)
 BYTES_FUSED_DERIVED = 52
 # the same 15 array accesses per cell with 2-byte coefficients and/or fields
-BYTES_FUSED_STORAGE = {"f32": 60, "bf16": 30, "f16": 30, "f16c": 50}
+BYTES_FUSED_STORAGE = {"f32": 60, "bf16": 30, "f16": 30, "f16c": 50, "f24": 50}
+# the derived-coefficient form per storage: FP32 coefficients (3 reads) and
+# 4-byte (FP32) or 3-byte (F24: 16-bit plane + byte plane) fields
+BYTES_FUSED_DERIVED_STORAGE = {"f32": BYTES_FUSED_DERIVED, "f24": 42}
 BYTES_VELOCITY = 36  # two-pass velocity: read txx tzz txz, 2 buoyancies; read+write vx vz
 BYTES_STRESS = 44    # two-pass stress: read vx vz lam lam+2mu mu_txz; read+write txx tzz txz
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fused_seam_traffic.txt")
@@ -734,8 +737,9 @@ def main():
                    "fs_record": stage_ms(4, reps)}
         main_bytes = BYTES_FUSED_STORAGE[args.storage]
         form = "stored coefficients"
-        if args.storage == "f32" and model_blocks(b, dm, M, H) == 7:
-            main_bytes, form = BYTES_FUSED_DERIVED, "coefficients formed from rho_vx, lam, mu_txx"
+        if args.storage in BYTES_FUSED_DERIVED_STORAGE and model_blocks(b, dm, M, H) == 7:
+            main_bytes = BYTES_FUSED_DERIVED_STORAGE[args.storage]
+            form = "coefficients formed from rho_vx, lam, mu_txx"
         main_name = f"k_fused, {form} ({main_bytes} B/cell-update x padded cells per launch)"
         if split:
             # the split step runs the fused pass as edge tiles, then interior tiles

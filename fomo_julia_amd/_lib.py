@@ -23,7 +23,7 @@ P_i32 = ctypes.POINTER(ctypes.c_int32)
 FOMO_OK, FOMO_ERR_ARG, FOMO_ERR_HIP, FOMO_ERR_UNSUPPORTED, FOMO_ERR_STATE = 0, -1, -2, -3, -4
 SRC_TYPES = {"pressure": 0, "force_z": 1, "force_x": 2}
 REC_TYPES = {"vz": 0, "vx": 1, "p": 2}
-STORAGE = {"f32": 0, "bf16": 1, "f16": 2, "f16c": 3}
+STORAGE = {"f32": 0, "bf16": 1, "f16": 2, "f16c": 3, "f24": 4}
 FIELDS = {"vx": 0, "vz": 1, "txx": 2, "tzz": 3, "txz": 4,
           "vx_old": 5, "vz_old": 6, "txx_old": 7, "tzz_old": 8, "txz_old": 9}
 

@@ -22,9 +22,11 @@ constexpr int MAX_NBC = 63;     // one wavefront spans an HABC strip's width
 
 enum SrcType { SRC_PRESSURE = 0, SRC_FORCE_Z = 1, SRC_FORCE_X = 2 };
 // Storage modes of wavefield and medium arrays (arithmetic is always FP32):
-// element kinds ST_F32 / ST_BF16 / ST_F16 for both, or ST_F16C = FP16
-// coefficients with FP32 fields.
-enum StorageKind { ST_F32 = 0, ST_BF16 = 1, ST_F16 = 2, ST_F16C = 3 };
+// element kinds ST_F32 / ST_BF16 / ST_F16 for both, ST_F16C = FP16
+// coefficients with FP32 fields, or ST_F24 = fields rounded to a 16-bit
+// significand (a 16-bit plane and a byte plane per field) with FP32
+// coefficients.
+enum StorageKind { ST_F32 = 0, ST_BF16 = 1, ST_F16 = 2, ST_F16C = 3, ST_F24 = 4 };
 enum Coef { C_BVX_X = 0, C_BVX_Z, C_BVZ_X, C_BVZ_Z, C_LAM, C_L2M, C_MU };
 enum RecType { REC_VZ = 0, REC_VX = 1, REC_P = 2 };
 enum Field { F_VX = 0, F_VZ = 1, F_TXX = 2, F_TZZ = 3, F_TXZ = 4 };
@@ -114,6 +116,9 @@ struct ShotDev {
   void* bufB;
   const void* bufC;
   uint32_t fstr_b, forg_b, cstr_b, corg_b, recF, recC;
+  // F24 storage: the byte-plane element of the 16-bit element at address p
+  // is at (p >> 1) + loc (one constant per wavefield allocation; 0 otherwise)
+  uint64_t loc;
 };
 
 struct BatchArgs {

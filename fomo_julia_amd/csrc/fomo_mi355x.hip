@@ -92,6 +92,9 @@ struct fomo_wavefield {
   fomo_batch* single[2] = {nullptr, nullptr};
   uint64_t single_id[2] = {0, 0};
   int pool_refs = 0;  // 1 while in a pool (destroy is refused)
+  // F24 storage: the byte planes are a half-size image of the 16-bit planes
+  // after them in dbuf; the byte of the element at p is at (p >> 1) + loc
+  uint64_t loc = 0;
 };
 
 struct fomo_shot {
@@ -314,10 +317,22 @@ static void encode_coef(int kind, const float* x, int nx, int nz, float scale, i
     }
 }
 static int coef_kind(int storage) {
-  return storage == FOMO_STORAGE_BF16 ? ST_BF16 : storage == FOMO_STORAGE_F32 ? ST_F32 : ST_F16;
+  return storage == FOMO_STORAGE_BF16                                    ? ST_BF16
+         : storage == FOMO_STORAGE_F32 || storage == FOMO_STORAGE_F24 ? ST_F32
+                                                                       : ST_F16;
 }
 static int field_kind(int storage) {
-  return storage == FOMO_STORAGE_BF16 ? ST_BF16 : storage == FOMO_STORAGE_F16 ? ST_F16 : ST_F32;
+  return storage == FOMO_STORAGE_BF16  ? ST_BF16
+         : storage == FOMO_STORAGE_F16 ? ST_F16
+         : storage == FOMO_STORAGE_F24 ? ST_F24
+                                       : ST_F32;
+}
+// F24: x rounded to a 16-bit significand (nearest, ties to even) as FP32
+// bits, the same integer rounding as the kernels' f24_bits
+static uint32_t f24_round_bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  return (u + 0x7Fu + ((u >> 8) & 1u)) & 0xFFFFFF00u;
 }
 
 // ------------------------------------------------------------------ context
@@ -452,7 +467,7 @@ extern "C" int32_t fomo_model_create(fomo_ctx* c, const fomo_model_desc* d, cons
     return fail(c, FOMO_ERR_ARG, "grid too small for nbc and fd_order");
   if (d->nx > (1 << 20) || d->nz > (1 << 20) || (int64_t)d->nx * d->nz > (int64_t)1 << 31)
     return fail(c, FOMO_ERR_ARG, "grid too large");
-  if (d->storage < FOMO_STORAGE_F32 || d->storage > FOMO_STORAGE_F16C)
+  if (d->storage < FOMO_STORAGE_F32 || d->storage > FOMO_STORAGE_F24)
     return fail(c, FOMO_ERR_ARG, "unknown storage type");
   if (d->storage != FOMO_STORAGE_F32 && !reduced_m_ok(M))
     return fail(c, FOMO_ERR_UNSUPPORTED, "reduced-precision storage supports fd_order 4, 8 and 12");
@@ -585,7 +600,7 @@ extern "C" int32_t fomo_model_create_from_velocity(fomo_ctx* c, const fomo_model
   if (d->nx < nmin || d->nz < nmin) return fail(c, FOMO_ERR_ARG, "grid too small for nbc and fd_order");
   if (d->nx > (1 << 20) || d->nz > (1 << 20) || (int64_t)d->nx * d->nz > (int64_t)1 << 31)
     return fail(c, FOMO_ERR_ARG, "grid too large");
-  if (d->storage < FOMO_STORAGE_F32 || d->storage > FOMO_STORAGE_F16C)
+  if (d->storage < FOMO_STORAGE_F32 || d->storage > FOMO_STORAGE_F24)
     return fail(c, FOMO_ERR_ARG, "unknown storage type");
   if (d->storage != FOMO_STORAGE_F32 && !reduced_m_ok(M))
     return fail(c, FOMO_ERR_UNSUPPORTED, "reduced-precision storage supports fd_order 4, 8 and 12");
@@ -718,9 +733,11 @@ extern "C" int32_t fomo_wavefield_create(fomo_model* m, fomo_wavefield** out) {
   fomo_ctx* c = m->ctx;
   auto w = std::make_unique<fomo_wavefield>();
   w->m = m;
-  // two state buffers + (FP32 only: the two-pass path's strip backups)
-  const bool f32 = m->storage == FOMO_STORAGE_F32;
-  const size_t bytes = (f32 ? 15 : 10) * m->fstride * m->fes;
+  // two state buffers + (FP32 only: the two-pass path's strip backups);
+  // F24: + the byte-plane image of the two buffers' 16-bit planes
+  const bool f32 = m->storage == FOMO_STORAGE_F32, f24 = m->fkind == ST_F24;
+  const size_t hi_bytes = (f32 ? 15 : 10) * m->fstride * m->fes;
+  const size_t bytes = hi_bytes + (f24 ? hi_bytes / 2 : 0);
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipMalloc(&w->dbuf, bytes));
   // zeroed on the context's stream: single-shot runs and batches use that
@@ -731,6 +748,7 @@ extern "C" int32_t fomo_wavefield_create(fomo_model* m, fomo_wavefield** out) {
     w->buf[1][k] = block_origin(w->dbuf, m->fstride, 5 + k, m->pitch, m->fes);
     w->old[k] = f32 ? block_origin(w->dbuf, m->fstride, 10 + k, m->pitch, m->fes) : nullptr;
   }
+  if (f24) w->loc = ((uint64_t)w->dbuf + hi_bytes) - ((uint64_t)w->dbuf >> 1);
   w->bytes = (int64_t)bytes;
   default_field_scales(m, w->fsc);
   *out = w.release();
@@ -766,6 +784,10 @@ extern "C" int64_t fomo_wavefield_device_bytes(const fomo_wavefield* w) { return
 static float* field_ptr(fomo_wavefield* w, int field) {
   return field < 5 ? w->buf[w->cur][field] : w->old[field - 5];
 }
+// F24: the byte-plane element of the 16-bit element at p
+static void* f24_plane(const fomo_wavefield* w, const void* p) {
+  return (void*)(((uint64_t)p >> 1) + w->loc);
+}
 
 extern "C" int32_t fomo_wavefield_download(fomo_wavefield* w, int32_t field, float* host) {
   if (!w || !host) return fail(w ? w->m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
@@ -790,6 +812,17 @@ extern "C" int32_t fomo_wavefield_download(fomo_wavefield* w, int32_t field, flo
   std::vector<uint16_t> tmp((size_t)nx * nz);
   HIP_TRY(c, hipMemcpy2DAsync(tmp.data(), (size_t)nx * es, field_ptr(w, field), (size_t)P * es,
                               (size_t)nx * es, nz, hipMemcpyDeviceToHost, c->stream));
+  if (fk == ST_F24) {  // + the byte plane: value bits = high << 16 | byte << 8
+    std::vector<uint8_t> lo((size_t)nx * nz);
+    HIP_TRY(c, hipMemcpy2DAsync(lo.data(), (size_t)nx, f24_plane(w, field_ptr(w, field)), (size_t)P,
+                                (size_t)nx, nz, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < tmp.size(); ++i) {
+      const uint32_t b = ((uint32_t)tmp[i] << 16) | ((uint32_t)lo[i] << 8);
+      std::memcpy(&host[i], &b, 4);
+    }
+    return FOMO_OK;
+  }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   decode(fk, tmp.data(), tmp.size(), 1.0f / w->fsc[field], host);
   return FOMO_OK;
@@ -817,6 +850,20 @@ extern "C" int32_t fomo_wavefield_upload(fomo_wavefield* w, int32_t field, const
     return FOMO_OK;
   }
   std::vector<uint16_t> tmp((size_t)nx * nz);
+  if (fk == ST_F24) {  // each value rounded to F24, split into its two planes
+    std::vector<uint8_t> lo(tmp.size());
+    for (size_t i = 0; i < tmp.size(); ++i) {
+      const uint32_t b = f24_round_bits(host[i]);
+      tmp[i] = (uint16_t)(b >> 16);
+      lo[i] = (uint8_t)(b >> 8);
+    }
+    HIP_TRY(c, hipMemcpy2DAsync(field_ptr(w, field), (size_t)P * es, tmp.data(), (size_t)nx * es,
+                                (size_t)nx * es, nz, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpy2DAsync(f24_plane(w, field_ptr(w, field)), (size_t)P, lo.data(), (size_t)nx,
+                                (size_t)nx, nz, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return FOMO_OK;
+  }
   encode(fk, host, tmp.size(), w->fsc[field], tmp.data());
   HIP_TRY(c, hipMemcpy2DAsync(field_ptr(w, field), (size_t)P * es, tmp.data(), (size_t)nx * es,
                               (size_t)nx * es, nz, hipMemcpyHostToDevice, c->stream));
@@ -1137,6 +1184,7 @@ static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const 
   sd.bufA = sd.fo[0] ? (const char*)sd.fo[0] - sd.forg_b : nullptr;
   sd.bufB = sd.f[0] ? (char*)sd.f[0] - sd.forg_b : nullptr;
   sd.bufC = m->dcoef;
+  sd.loc = w->loc;
   return sd;
 }
 
@@ -1165,7 +1213,7 @@ static int stencil_slots_mc(int cus, int mode, bool compact) {
   int bv = 0, bs = 0;
   if (mode == 1 || K != ST_F32) {
     hipError_t e;
-    if constexpr (K == ST_F32)
+    if constexpr (K == ST_F32 || K == ST_F24)
       e = compact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused<M, CPT, WPB, K, true, true>,
                                                                  64 * WPB, 0)
                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused<M, CPT, WPB, K, true, false>,
@@ -1190,17 +1238,18 @@ static int stencil_slots_m(int cpt, int cus, int mode, bool compact = false) {
                     : stencil_slots_mc<K, M, 4>(cus, mode, compact);
 }
 template <int K>
-static int stencil_slots_reduced(int M, int cpt, int cus) {
+static int stencil_slots_reduced(int M, int cpt, int cus, bool compact = false) {
   switch (M) {
-    case 2: return stencil_slots_m<K, 2>(cpt, cus, 1);
-    case 4: return stencil_slots_m<K, 4>(cpt, cus, 1);
-    default: return stencil_slots_m<K, 6>(cpt, cus, 1);
+    case 2: return stencil_slots_m<K, 2>(cpt, cus, 1, compact);
+    case 4: return stencil_slots_m<K, 4>(cpt, cus, 1, compact);
+    default: return stencil_slots_m<K, 6>(cpt, cus, 1, compact);
   }
 }
 static int stencil_slots(int storage, int M, int cpt, int cus, int mode, bool compact = false) {
   if (storage == FOMO_STORAGE_BF16) return stencil_slots_reduced<ST_BF16>(M, cpt, cus);
   if (storage == FOMO_STORAGE_F16) return stencil_slots_reduced<ST_F16>(M, cpt, cus);
   if (storage == FOMO_STORAGE_F16C) return stencil_slots_reduced<ST_F16C>(M, cpt, cus);
+  if (storage == FOMO_STORAGE_F24) return stencil_slots_reduced<ST_F24>(M, cpt, cus, compact);
   switch (M) {
     case 1: return stencil_slots_m<ST_F32, 1>(cpt, cus, mode, compact);
     case 2: return stencil_slots_m<ST_F32, 2>(cpt, cus, mode, compact);
@@ -1306,7 +1355,7 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
     b->ss.push_back(ss[i]);
     if (ms[i]->d.dtx != ms[i]->d.dtz) b->same = false;
   }
-  b->compact = b->same && b->storage == FOMO_STORAGE_F32;
+  b->compact = b->same && (b->storage == FOMO_STORAGE_F32 || b->storage == FOMO_STORAGE_F24);
   for (int i = 0; i < n; ++i) {
     b->compact = b->compact && ms[i]->compact;
     b->habc_rows = std::max(b->habc_rows, ms[i]->d.nbc + 1);
@@ -1450,7 +1499,7 @@ static void launch_stencil_mc(const fomo_batch* b, int table, int kind, int flag
     if (kind == 5 && b->int_bpc > 0) nb = std::min(nb, b->int_bpc * std::max(b->ctx->cus, 1));
     const dim3 grid(round_up(nb, NUM_XCD));
     if (b->same && b->compact) {
-      if constexpr (K == ST_F32)
+      if constexpr (K == ST_F32 || K == ST_F24)
         hipLaunchKernelGGL((k_fused<M, CPT, WPB, K, true, true>), grid, block, 0, st,
                            args_for(b, table, part, k_off), flags);
     } else if (b->same) {
@@ -1499,6 +1548,7 @@ static void launch_stencil_reduced(const fomo_batch* b, int table, int kind, int
 static void launch_stencil(const fomo_batch* b, int table, int kind, int flags, int k_off) {
   if (b->storage == FOMO_STORAGE_BF16) return launch_stencil_reduced<ST_BF16>(b, table, kind, flags, k_off);
   if (b->storage == FOMO_STORAGE_F16) return launch_stencil_reduced<ST_F16>(b, table, kind, flags, k_off);
+  if (b->storage == FOMO_STORAGE_F24) return launch_stencil_reduced<ST_F24>(b, table, kind, flags, k_off);
   if (b->storage == FOMO_STORAGE_F16C)
     return launch_stencil_reduced<ST_F16C>(b, table, kind, flags, k_off);
   switch (b->M) {
@@ -1525,6 +1575,8 @@ static void launch_habc(const fomo_batch* b, int table, bool stress, int k_off) 
     hipLaunchKernelGGL(k_habc<ST_BF16>, g, dim3(64), lds, lstream(b), a, stress ? 1 : 0, rows);
   else if (b->storage == FOMO_STORAGE_F16)
     hipLaunchKernelGGL(k_habc<ST_F16>, g, dim3(64), lds, lstream(b), a, stress ? 1 : 0, rows);
+  else if (b->storage == FOMO_STORAGE_F24)
+    hipLaunchKernelGGL(k_habc<ST_F24>, g, dim3(64), lds, lstream(b), a, stress ? 1 : 0, rows);
   else
     hipLaunchKernelGGL(k_habc<ST_F32>, g, dim3(64), lds, lstream(b), a, stress ? 1 : 0, rows);
 }
@@ -1536,6 +1588,8 @@ static void launch_fsrec(const fomo_batch* b, int table, int mode, int k_off) {
     hipLaunchKernelGGL((k_fsrec<WPB, ST_BF16>), g, blk, 0, lstream(b), a, mode);
   else if (b->storage == FOMO_STORAGE_F16)
     hipLaunchKernelGGL((k_fsrec<WPB, ST_F16>), g, blk, 0, lstream(b), a, mode);
+  else if (b->storage == FOMO_STORAGE_F24)
+    hipLaunchKernelGGL((k_fsrec<WPB, ST_F24>), g, blk, 0, lstream(b), a, mode);
   else
     hipLaunchKernelGGL((k_fsrec<WPB, ST_F32>), g, blk, 0, lstream(b), a, mode);
 }
@@ -1550,6 +1604,8 @@ static void launch_frame(const fomo_batch* b, const FrameArgs& fa0, int table, i
     hipLaunchKernelGGL(k_frame<ST_BF16>, g, dim3(256), 0, lstream(b), a, fa);
   else if (b->storage == FOMO_STORAGE_F16)
     hipLaunchKernelGGL(k_frame<ST_F16>, g, dim3(256), 0, lstream(b), a, fa);
+  else if (b->storage == FOMO_STORAGE_F24)
+    hipLaunchKernelGGL(k_frame<ST_F24>, g, dim3(256), 0, lstream(b), a, fa);
   else
     hipLaunchKernelGGL(k_frame<ST_F32>, g, dim3(256), 0, lstream(b), a, fa);
 }
@@ -2023,7 +2079,7 @@ extern "C" int32_t fomo_pool_create(fomo_ctx* c, int32_t fd_order, int32_t stora
   if (!c || !out) return fail(c, FOMO_ERR_ARG, "NULL argument");
   if (fd_order < 2 || fd_order > 2 * MAX_M || fd_order % 2)
     return fail(c, FOMO_ERR_UNSUPPORTED, "fd_order must be even, 2..16");
-  if (storage < FOMO_STORAGE_F32 || storage > FOMO_STORAGE_F16C)
+  if (storage < FOMO_STORAGE_F32 || storage > FOMO_STORAGE_F24)
     return fail(c, FOMO_ERR_ARG, "unknown storage type");
   const int M = fd_order / 2;
   if (storage != FOMO_STORAGE_F32 && !reduced_m_ok(M))
@@ -2040,7 +2096,8 @@ extern "C" int32_t fomo_pool_create(fomo_ctx* c, int32_t fd_order, int32_t stora
   b->M = M;
   b->storage = storage;
   b->same = true;
-  b->compact = storage == FOMO_STORAGE_F32;  // until a member without rho_vx / mu_txx blocks joins
+  // until a member without rho_vx / mu_txx blocks joins
+  b->compact = storage == FOMO_STORAGE_F32 || storage == FOMO_STORAGE_F24;
   b->cpt = c->cpt;
   b->graph_steps = c->graph_steps;
   b->step_mode = 1;

@@ -149,6 +149,12 @@ __device__ __forceinline__ void stv<4>(float* p, const V<4>& a) {
 
 // ---- storage traits: FP32 is the identity (no scale multiply), BF16 rounds
 // to nearest even, FP16 uses the hardware conversion (round to nearest even).
+// F24 keeps FP32 values rounded to a 16-bit significand (nearest, ties to
+// even): bits 31..16 in the field's 16-bit plane, bits 15..8 in its byte
+// plane, bits 7..0 zero.  The byte plane of a wavefield is a half-size image
+// of its 16-bit planes: the byte of the element at address p is at
+// (p >> 1) + loc (ShotDev::loc), so one constant per wavefield serves every
+// field, buffer and row.
 template <int K>
 struct Store;
 template <>
@@ -163,6 +169,10 @@ template <>
 struct Store<ST_F16> {
   typedef uint16_t T;
 };
+template <>
+struct Store<ST_F24> {
+  typedef uint16_t T;  // the 16-bit plane
+};
 
 // Element kinds of the fields (F) and of the coefficient arrays (C) per mode.
 template <int K>
@@ -173,6 +183,39 @@ template <>
 struct Mode<ST_F16C> {
   static constexpr int F = ST_F32, C = ST_F16;
 };
+template <>
+struct Mode<ST_F24> {
+  static constexpr int F = ST_F24, C = ST_F32;
+};
+
+// The per-field argument of the load and store helpers ("codec"): the scale
+// of a 16-bit kind (unused for FP32, a float either way), and for F24 the
+// wavefield's byte-plane constant loc (pointer helpers) or a buffer
+// descriptor of the byte-plane image (buffer helpers).
+typedef __attribute__((address_space(1))) uint8_t gu8;
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+// codec of a pointer helper: scale sc, or loc for F24
+template <int K>
+__device__ __forceinline__ auto pcodec(float sc, uint64_t loc) {
+  if constexpr (K == ST_F24) return uniform_u64(loc);
+  else return uniform_f(sc);
+}
+// F24: x rounded to a 16-bit significand (nearest, ties to even), as FP32
+// bits; infinities stay, finite values past the largest one round to them
+__device__ __forceinline__ uint32_t f24_bits(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u + 0x7Fu + ((u >> 8) & 1u)) & 0xFFFFFF00u;
+}
+__device__ __forceinline__ const gu8* f24_lo(const void* p, uint64_t loc) {
+  return (const gu8*)((((uint64_t)p) >> 1) + loc);
+}
+__device__ __forceinline__ gu8* f24_lo_w(void* p, uint64_t loc) {
+  return (gu8*)((((uint64_t)p) >> 1) + loc);
+}
 
 template <int K>
 __device__ __forceinline__ float from_bits(uint32_t b) {
@@ -217,10 +260,18 @@ __device__ __forceinline__ float scl(float x, float s) {
 }
 
 // N consecutive stored values at p, as floats (x stored * s).
-template <int K, int N>
-__device__ __forceinline__ V<N> ldf(const typename Store<K>::T* p, float s) {
+template <int K, int N, class S>
+__device__ __forceinline__ V<N> ldf(const typename Store<K>::T* p, S s) {
   if constexpr (K == ST_F32) {
     return ldv<N>(p);
+  } else if constexpr (K == ST_F24) {
+    static_assert(sizeof(S) == 8, "F24 pointer helpers take the byte-plane constant");
+    V<N> r;
+    const gu8* q = f24_lo(p, s);
+#pragma unroll
+    for (int c = 0; c < N; ++c)
+      r.v[c] = __uint_as_float(((uint32_t)((const gu16*)p)[c] << 16) | ((uint32_t)q[c] << 8));
+    return r;
   } else {
     V<N> r;
     if constexpr (N == 1) {
@@ -239,10 +290,19 @@ __device__ __forceinline__ V<N> ldf(const typename Store<K>::T* p, float s) {
     return r;
   }
 }
-template <int K, int N>
-__device__ __forceinline__ void stf(typename Store<K>::T* p, const V<N>& a, float s) {
+template <int K, int N, class S>
+__device__ __forceinline__ void stf(typename Store<K>::T* p, const V<N>& a, S s) {
   if constexpr (K == ST_F32) {
     stv<N>(p, a);
+  } else if constexpr (K == ST_F24) {
+    static_assert(sizeof(S) == 8, "F24 pointer helpers take the byte-plane constant");
+    gu8* q = f24_lo_w(p, s);
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      const uint32_t b = f24_bits(a.v[c]);
+      ((gu16*)p)[c] = (uint16_t)(b >> 16);
+      q[c] = (uint8_t)(b >> 8);
+    }
   } else if constexpr (N == 1) {
     *(gu16*)p = (uint16_t)to_bits<K>(scl<K>(a.v[0], s));
   } else if constexpr (N == 2) {
@@ -254,9 +314,11 @@ __device__ __forceinline__ void stf(typename Store<K>::T* p, const V<N>& a, floa
     *(gu2v*)p = w;
   }
 }
-template <int K>
-__device__ __forceinline__ float ld1(const typename Store<K>::T* p, float s) {
+template <int K, class S>
+__device__ __forceinline__ float ld1(const typename Store<K>::T* p, S s) {
   if constexpr (K == ST_F32) return ldg(p);
+  else if constexpr (K == ST_F24)
+    return __uint_as_float(((uint32_t)*(const gu16*)p << 16) | ((uint32_t)*f24_lo(p, s) << 8));
   else return scl<K>(from_bits<K>(*(const gu16*)p), s);
 }
 // The stored bits of one element (16-bit kinds zero-extended) and their value:
@@ -264,20 +326,28 @@ __device__ __forceinline__ float ld1(const typename Store<K>::T* p, float s) {
 // loads of a chunk are in flight before the first wait (a conditional load
 // whose value is decoded in the same branch makes the compiler wait on it
 // right there).
-template <int K>
-__device__ __forceinline__ uint32_t ld_bits(const typename Store<K>::T* p) {
+template <int K, class S>
+__device__ __forceinline__ uint32_t ld_bits(const typename Store<K>::T* p, S s) {
   if constexpr (K == ST_F32) return __float_as_uint(ldg(p));
+  else if constexpr (K == ST_F24) return ((uint32_t)*(const gu16*)p << 16) | ((uint32_t)*f24_lo(p, s) << 8);
   else return (uint32_t)*(const gu16*)p;
 }
-template <int K>
-__device__ __forceinline__ float from_stored(uint32_t b, float s) {
-  if constexpr (K == ST_F32) return __uint_as_float(b);
+template <int K, class S>
+__device__ __forceinline__ float from_stored(uint32_t b, S s) {
+  if constexpr (K == ST_F32 || K == ST_F24) return __uint_as_float(b);
   else return scl<K>(from_bits<K>(b), s);
 }
-template <int K>
-__device__ __forceinline__ void st1(typename Store<K>::T* p, float x, float s) {
-  if constexpr (K == ST_F32) stg(p, x);
-  else *(gu16*)p = (uint16_t)to_bits<K>(scl<K>(x, s));
+template <int K, class S>
+__device__ __forceinline__ void st1(typename Store<K>::T* p, float x, S s) {
+  if constexpr (K == ST_F32) {
+    stg(p, x);
+  } else if constexpr (K == ST_F24) {
+    const uint32_t b = f24_bits(x);
+    *(gu16*)p = (uint16_t)(b >> 16);
+    *f24_lo_w(p, s) = (uint8_t)(b >> 8);
+  } else {
+    *(gu16*)p = (uint16_t)to_bits<K>(scl<K>(x, s));
+  }
 }
 
 // ---- buffer addressing: a descriptor (base + length in SGPRs), a per-lane
@@ -290,10 +360,39 @@ constexpr uint32_t OOB = 0x80000000u;  // >= every buffer length used (< 2^31)
 __device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
 }
-template <int K, int N>
-__device__ __forceinline__ V<N> bload(Rsrc r, uint32_t vo, uint32_t so, float s) {
+// codec of a buffer helper: scale sc, or the byte-plane descriptor for F24
+template <int K>
+__device__ __forceinline__ auto bcodec(float sc, Rsrc lo) {
+  if constexpr (K == ST_F24) return lo;
+  else return uniform_f(sc);
+}
+// F24 through buffers: the codec is a descriptor of the byte-plane image of
+// r's range (base (r's base >> 1) + loc), addressed with offsets halved.
+template <int K, int N, class S>
+__device__ __forceinline__ V<N> bload(Rsrc r, uint32_t vo, uint32_t so, S s) {
   V<N> x;
-  if constexpr (K == ST_F32) {
+  if constexpr (K == ST_F24) {
+    static_assert(!__is_same(S, float), "F24 buffer helpers take the byte-plane descriptor");
+    const uint32_t lv = vo >> 1, ls = so >> 1;
+    if constexpr (N == 1) {
+      const uint32_t h = __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
+      const uint32_t l = __builtin_amdgcn_raw_buffer_load_b8(s, lv, ls, 0);
+      x.v[0] = __uint_as_float((h << 16) | (l << 8));
+    } else if constexpr (N == 2) {
+      const uint32_t h = __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0);
+      const uint32_t l = __builtin_amdgcn_raw_buffer_load_b16(s, lv, ls, 0);
+      x.v[0] = __uint_as_float((h << 16) | ((l & 0xFFu) << 8));
+      x.v[1] = __uint_as_float((h & 0xFFFF0000u) | (l & 0xFF00u));
+    } else {
+      const auto h = __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0);
+      const uint32_t l = __builtin_amdgcn_raw_buffer_load_b32(s, lv, ls, 0);
+      const uint32_t h0 = (uint32_t)h[0], h1 = (uint32_t)h[1];
+      x.v[0] = __uint_as_float((h0 << 16) | ((l & 0xFFu) << 8));
+      x.v[1] = __uint_as_float((h0 & 0xFFFF0000u) | (l & 0xFF00u));
+      x.v[2] = __uint_as_float((h1 << 16) | ((l >> 8) & 0xFF00u));
+      x.v[3] = __uint_as_float((h1 & 0xFFFF0000u) | ((l >> 16) & 0xFF00u));
+    }
+  } else if constexpr (K == ST_F32) {
     if constexpr (N == 1) {
       x.v[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
     } else if constexpr (N == 2) {
@@ -324,32 +423,43 @@ __device__ __forceinline__ V<N> bload(Rsrc r, uint32_t vo, uint32_t so, float s)
   }
   return x;
 }
-template <int K>
-__device__ __forceinline__ void bstore(Rsrc r, uint32_t vo, uint32_t so, float x, float s) {
-  if constexpr (K == ST_F32)
+template <int K, class S>
+__device__ __forceinline__ void bstore(Rsrc r, uint32_t vo, uint32_t so, float x, S s) {
+  if constexpr (K == ST_F32) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, vo, so, 0);
-  else
+  } else if constexpr (K == ST_F24) {
+    const uint32_t b = f24_bits(x);
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(b >> 16), r, vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(b >> 8), s, vo >> 1, so >> 1, 0);
+  } else {
     __builtin_amdgcn_raw_buffer_store_b16((unsigned short)to_bits<K>(scl<K>(x, s)), r, vo, so, 0);
+  }
 }
 
 // A lane's two adjacent elements at one offset: one 64-bit (FP32) or 32-bit
 // (16-bit kinds) store, the same bits as two bstore calls.
-template <int K>
-__device__ __forceinline__ void bstore2(Rsrc r, uint32_t vo, uint32_t so, float x0, float x1, float s) {
+template <int K, class S>
+__device__ __forceinline__ void bstore2(Rsrc r, uint32_t vo, uint32_t so, float x0, float x1, S s) {
   if constexpr (K == ST_F32) {
     u2v w;
     w.x = __float_as_uint(x0);
     w.y = __float_as_uint(x1);
     __builtin_amdgcn_raw_buffer_store_b64(w, r, vo, so, 0);
+  } else if constexpr (K == ST_F24) {
+    const uint32_t b0 = f24_bits(x0), b1 = f24_bits(x1);
+    __builtin_amdgcn_raw_buffer_store_b32((b0 >> 16) | (b1 & 0xFFFF0000u), r, vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(((b0 >> 8) & 0xFFu) | (b1 & 0xFF00u)), s, vo >> 1,
+                                          so >> 1, 0);
   } else {
     __builtin_amdgcn_raw_buffer_store_b32(to_bits2<K>(scl<K>(x0, s), scl<K>(x1, s)), r, vo, so, 0);
   }
 }
 
 // x as the storage holds it (stored with scale s, read back with is = 1/s).
-template <int K>
-__device__ __forceinline__ float as_stored(float x, float s, float is) {
+template <int K, class S>
+__device__ __forceinline__ float as_stored(float x, S s, S is) {
   if constexpr (K == ST_F32) return x;
+  else if constexpr (K == ST_F24) return __uint_as_float(f24_bits(x));
   else return scl<K>(from_bits<K>(to_bits<K>(scl<K>(x, s))), is);
 }
 
@@ -845,7 +955,7 @@ __device__ __forceinline__ void fused_tile(const ShotDev& sd, int t, int flags, 
 
 template <int M, int CPT, int WPB, int K, bool SAME, bool CMP>
 __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
-  static_assert(!CMP || (K == ST_F32 && SAME), "derived coefficients: FP32, dtx == dtz");
+  static_assert(!CMP || (Mode<K>::C == ST_F32 && SAME), "derived coefficients: FP32, dtx == dtz");
   constexpr int KF = Mode<K>::F, KC = Mode<K>::C;
   typedef typename Store<KF>::T T;
   typedef typename Store<KC>::T TC;
@@ -885,6 +995,12 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     constexpr int FES = (int)sizeof(T), CES = (int)sizeof(TC);
     const Rsrc rA = make_rsrc(uniform_p(sd.bufA), (uint32_t)uniform((int)sd.recF));
     const Rsrc rB = make_rsrc(uniform_p(sd.bufB), (uint32_t)uniform((int)sd.recF));
+    // F24: the byte-plane images of the read and written ranges
+    const uint64_t loc = uniform_u64(sd.loc);
+    const Rsrc rAl = make_rsrc((const void*)(((uint64_t)uniform_p(sd.bufA) >> 1) + loc),
+                               (uint32_t)uniform((int)sd.recF) >> 1);
+    const Rsrc rBl = make_rsrc((const void*)(((uint64_t)uniform_p(sd.bufB) >> 1) + loc),
+                               (uint32_t)uniform((int)sd.recF) >> 1);
     const Rsrc rC = make_rsrc(uniform_p(sd.bufC), (uint32_t)uniform((int)sd.recC));
     const uint32_t fstr = (uint32_t)uniform((int)sd.fstr_b), forg = (uint32_t)uniform((int)sd.forg_b);
     const uint32_t cstr = (uint32_t)uniform((int)sd.cstr_b), corg = (uint32_t)uniform((int)sd.corg_b);
@@ -901,12 +1017,12 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     const float src_scale = uniform_f(sd.src_scale);
     const float dtx = uniform_f(sd.dtx), dtz = uniform_f(sd.dtz);
     // load (r*) and store (w*) scales of reduced-precision storage (unused for FP32)
-    const float rvx = uniform_f(sd.fisc[F_VX]), rvz = uniform_f(sd.fisc[F_VZ]);
-    const float rxx = uniform_f(sd.fisc[F_TXX]), rzz = uniform_f(sd.fisc[F_TZZ]);
-    const float rxz = uniform_f(sd.fisc[F_TXZ]);
-    const float wvxs = uniform_f(sd.fsc[F_VX]), wvzs = uniform_f(sd.fsc[F_VZ]);
-    const float wxx = uniform_f(sd.fsc[F_TXX]), wzz = uniform_f(sd.fsc[F_TZZ]);
-    const float wxz = uniform_f(sd.fsc[F_TXZ]);
+    const auto rvx = bcodec<KF>(sd.fisc[F_VX], rAl), rvz = bcodec<KF>(sd.fisc[F_VZ], rAl);
+    const auto rxx = bcodec<KF>(sd.fisc[F_TXX], rAl), rzz = bcodec<KF>(sd.fisc[F_TZZ], rAl);
+    const auto rxz = bcodec<KF>(sd.fisc[F_TXZ], rAl);
+    const auto wvxs = bcodec<KF>(sd.fsc[F_VX], rBl), wvzs = bcodec<KF>(sd.fsc[F_VZ], rBl);
+    const auto wxx = bcodec<KF>(sd.fsc[F_TXX], rBl), wzz = bcodec<KF>(sd.fsc[F_TZZ], rBl);
+    const auto wxz = bcodec<KF>(sd.fsc[F_TXZ], rBl);
     const float r1 = uniform_f(sd.csc[C_BVX_X]), r2 = uniform_f(sd.csc[C_BVX_Z]);
     const float r3 = uniform_f(sd.csc[C_BVZ_X]), r4 = uniform_f(sd.csc[C_BVZ_Z]);
     const float rl = uniform_f(sd.csc[C_LAM]), rl2 = uniform_f(sd.csc[C_L2M]);
@@ -922,12 +1038,12 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     auto frow = [&](int c, int r, int col) {
       return (uint32_t)c * fstr + forg + (uint32_t)((min(r, zlast) * P + col) * FES);
     };
-    auto fld = [&](int c, int r, float sc) { return bload<KF, CPT>(rA, lvo, frow(c, r, ib0), sc); };
+    auto fld = [&](int c, int r, auto sc) { return bload<KF, CPT>(rA, lvo, frow(c, r, ib0), sc); };
     auto cof = [&](int q, int r, float sc) {
       return bload<KC, CPT>(rC, lco, (uint32_t)q * cstr + corg + (uint32_t)((min(r, zlast) * P + ib0) * CES),
                             sc);
     };
-    auto hal = [&](int c, int r, float sc) { return bload<KF, 1>(rA, hvo, frow(c, r, ib0 - M), sc).v[0]; };
+    auto hal = [&](int c, int r, auto sc) { return bload<KF, 1>(rA, hvo, frow(c, r, ib0 - M), sc).v[0]; };
     // pressure source as seen by every reader of old txx / tzz at row r; it is
     // added where a row is first used (one row after its load is issued), so
     // that no prefetched value is touched before it is needed
@@ -1264,11 +1380,11 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
     const float wav = src_on ? uniform_f(uniform_p(sd.wavelet)[k - 1]) : 0.f;
     const int si = uniform(sd.si), sj = uniform(sd.sj);
     const float dtx = uniform_f(sd.dtx), dtz = uniform_f(sd.dtz);
-    const float rvx = uniform_f(sd.fisc[F_VX]), rvz = uniform_f(sd.fisc[F_VZ]);
-    const float rxx = uniform_f(sd.fisc[F_TXX]), rzz = uniform_f(sd.fisc[F_TZZ]);
-    const float rxz = uniform_f(sd.fisc[F_TXZ]);
-    const float wxx = uniform_f(sd.fsc[F_TXX]), wzz = uniform_f(sd.fsc[F_TZZ]);
-    const float wxz = uniform_f(sd.fsc[F_TXZ]);
+    const auto rvx = pcodec<KF>(sd.fisc[F_VX], sd.loc), rvz = pcodec<KF>(sd.fisc[F_VZ], sd.loc);
+    const auto rxx = pcodec<KF>(sd.fisc[F_TXX], sd.loc), rzz = pcodec<KF>(sd.fisc[F_TZZ], sd.loc);
+    const auto rxz = pcodec<KF>(sd.fisc[F_TXZ], sd.loc);
+    const auto wxx = pcodec<KF>(sd.fsc[F_TXX], sd.loc), wzz = pcodec<KF>(sd.fsc[F_TZZ], sd.loc);
+    const auto wxz = pcodec<KF>(sd.fsc[F_TXZ], sd.loc);
     const float rl = uniform_f(sd.csc[C_LAM]), rl2 = uniform_f(sd.csc[C_L2M]);
     const float rmu = uniform_f(sd.csc[C_MU]);
     float a[M];
@@ -1438,16 +1554,17 @@ constexpr int HABC_WT_LEN = 64 + 2 * MAX_NBC + 12;  // weight table (s_w / s_rw 
 // (s[r*HS + c], so[r*HS + c]), nc <= 64.  Loads of both arrays go out a chunk
 // of rows at a time before any LDS store, so a chunk costs one memory latency:
 // 32 rows for FP32 fields, a whole piece (64 rows) for 16-bit ones (measured
-// fastest on the SEAM shot and the 64-shot batch in each case).
+// fastest on the SEAM shot and the 64-shot batch in each case), 32 for F24's
+// two loads per element (64 rows of them need more than 256 registers).
 template <int K>
-constexpr int stage_chunk() { return sizeof(typename Store<K>::T) == 2 ? 64 : 32; }
+constexpr int stage_chunk() { return K != ST_F24 && sizeof(typename Store<K>::T) == 2 ? 64 : 32; }
 // Recurrence loops read their operands (pre-HABC values, which no earlier
 // step of the loop writes) this many steps ahead, so LDS latency stays off the
 // chain of dependent steps.
 constexpr int HPF = 4;
-template <int K>
+template <int K, class S>
 __device__ __forceinline__ void lds_in2(float* s, float* so, const typename Store<K>::T* g,
-                                        const typename Store<K>::T* go, float rs, int P, int r0,
+                                        const typename Store<K>::T* go, S rs, int P, int r0,
                                         int nr, int c0, int nc, int lane) {
   constexpr int STAGE_CHUNK = stage_chunk<K>();
   const bool on = lane < nc;
@@ -1457,8 +1574,8 @@ __device__ __forceinline__ void lds_in2(float* s, float* so, const typename Stor
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r) {
       const ptrdiff_t o = (ptrdiff_t)(r0 + min(rb + r, nr - 1)) * P + c0 + cl;
-      v[r] = ld_bits<K>(g + o);
-      w[r] = ld_bits<K>(go + o);
+      v[r] = ld_bits<K>(g + o, rs);
+      w[r] = ld_bits<K>(go + o, rs);
     }
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r)
@@ -1469,9 +1586,9 @@ __device__ __forceinline__ void lds_in2(float* s, float* so, const typename Stor
   }
 }
 // s[(sr+r)*HS + sc + c] -> rows [r0, r0+nr) x cols [c0, c0+nc) of g   (nc <= 64)
-template <int K>
+template <int K, class S>
 __device__ __forceinline__ void lds_out(const float* s, int sr, int sc, typename Store<K>::T* g,
-                                        float ws, int P, int r0, int nr, int c0, int nc, int lane) {
+                                        S ws, int P, int r0, int nr, int c0, int nc, int lane) {
   constexpr int STAGE_CHUNK = stage_chunk<K>();
   const bool on = lane < nc;
   for (int rb = 0; rb < nr; rb += STAGE_CHUNK) {
@@ -1488,9 +1605,9 @@ __device__ __forceinline__ void lds_out(const float* s, int sr, int sc, typename
 // Transposed staging for the side edges: rows [r0, r0+nr) x cols [c0, c0+nc)
 // of g and its backup -> (s[c*HS + r], so[c*HS + r]), nr <= 64, nc <= 64, so a
 // side piece takes nc (= nbc+1) LDS rows like every other piece.
-template <int K>
+template <int K, class S>
 __device__ __forceinline__ void lds_in2_t(float* s, float* so, const typename Store<K>::T* g,
-                                          const typename Store<K>::T* go, float rs, int P, int r0,
+                                          const typename Store<K>::T* go, S rs, int P, int r0,
                                           int nr, int c0, int nc, int lane) {
   constexpr int STAGE_CHUNK = stage_chunk<K>();
   const bool on = lane < nc;
@@ -1500,8 +1617,8 @@ __device__ __forceinline__ void lds_in2_t(float* s, float* so, const typename St
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r) {
       const ptrdiff_t o = (ptrdiff_t)(r0 + min(rb + r, nr - 1)) * P + c0 + cl;
-      v[r] = ld_bits<K>(g + o);
-      w[r] = ld_bits<K>(go + o);
+      v[r] = ld_bits<K>(g + o, rs);
+      w[r] = ld_bits<K>(go + o, rs);
     }
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r)
@@ -1512,8 +1629,8 @@ __device__ __forceinline__ void lds_in2_t(float* s, float* so, const typename St
   }
 }
 // s[(sc+c)*HS + r] -> rows [r0, r0+nr) x cols [c0, c0+nc) of g   (nr, nc <= 64)
-template <int K>
-__device__ __forceinline__ void lds_out_t(const float* s, int sc, typename Store<K>::T* g, float ws,
+template <int K, class S>
+__device__ __forceinline__ void lds_out_t(const float* s, int sc, typename Store<K>::T* g, S ws,
                                           int P, int r0, int nr, int c0, int nc, int lane) {
   constexpr int STAGE_CHUNK = stage_chunk<K>();
   const bool on = lane < nc;
@@ -1533,9 +1650,9 @@ __device__ __forceinline__ void lds_out_t(const float* s, int sc, typename Store
 // float after it (backup).  Row -1 and column -1 are padding: only lanes whose
 // result is discarded read them, and the first float is never read.  Every
 // operand of a corner cell is then a fixed offset from one per-lane index.
-template <int K>
+template <int K, class S>
 __device__ __forceinline__ void lds_in2_fo(float* s, const typename Store<K>::T* g,
-                                           const typename Store<K>::T* go, float rs, int P, int r0,
+                                           const typename Store<K>::T* go, S rs, int P, int r0,
                                            int nr, int c0, int nc, int lane) {
   constexpr int STAGE_CHUNK = stage_chunk<K>();
   const bool on = lane < nc;
@@ -1545,8 +1662,8 @@ __device__ __forceinline__ void lds_in2_fo(float* s, const typename Store<K>::T*
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r) {
       const ptrdiff_t o = (ptrdiff_t)(r0 + min(rb + r, nr - 1)) * P + c0 + cl;
-      v[r] = ld_bits<K>(g + o);
-      w[r] = ld_bits<K>(go + o);
+      v[r] = ld_bits<K>(g + o, rs);
+      w[r] = ld_bits<K>(go + o, rs);
     }
 #pragma unroll
     for (int r = 0; r < STAGE_CHUNK; ++r)
@@ -1558,8 +1675,8 @@ __device__ __forceinline__ void lds_in2_fo(float* s, const typename Store<K>::T*
   }
 }
 // fields of the interleaved block -> rows [r0, r0+nr) x cols [c0, c0+nc) of g
-template <int K>
-__device__ __forceinline__ void lds_out_fo(const float* s, typename Store<K>::T* g, float ws, int P,
+template <int K, class S>
+__device__ __forceinline__ void lds_out_fo(const float* s, typename Store<K>::T* g, S ws, int P,
                                            int r0, int nr, int c0, int nc, int lane) {
   constexpr int STAGE_CHUNK = stage_chunk<K>();
   const bool on = lane < nc;
@@ -1613,7 +1730,7 @@ __global__ __launch_bounds__(64) void k_habc(BatchArgs ba, int stress, int rows)
     HabcField h = habc_field(sd, stress, fi);
     T* hf = (T*)uniform_p(h.f);
     const T* hfo = (const T*)uniform_p(h.fo);
-    const float hrs = uniform_f(h.rs), hws = uniform_f(h.ws);
+    const auto hrs = pcodec<KF>(h.rs, sd.loc), hws = pcodec<KF>(h.ws, sd.loc);
     const Hq q{uniform_f(sd.qx), uniform_f(sd.qz), uniform_f(sd.qt_x), uniform_f(sd.qt_z),
                uniform_f(sd.qxt)};
     const int n = uniform(sd.nbc), nx = uniform(sd.nx), nz = uniform(sd.nz), P = uniform(sd.pitch);
@@ -1992,8 +2109,9 @@ __global__ __launch_bounds__(64 * WPB) void k_fsrec(BatchArgs ba, int mode) {
       if (!fs_on) continue;
       const int j = sd.pad - 5 + idx / nx, i = idx % nx;
       if (j >= 0 && j < sd.nz) {
-        st1<KF>(rowt((T*)sd.f[F_TZZ], j, P) + i, 0.f, 1.f);
-        st1<KF>(rowt((T*)sd.f[F_TXZ], j, P) + i, 0.f, 1.f);
+        const auto zc = pcodec<KF>(1.f, sd.loc);  // zero is stored exactly at any scale
+        st1<KF>(rowt((T*)sd.f[F_TZZ], j, P) + i, 0.f, zc);
+        st1<KF>(rowt((T*)sd.f[F_TXZ], j, P) + i, 0.f, zc);
       }
       continue;
     }
@@ -2001,7 +2119,9 @@ __global__ __launch_bounds__(64 * WPB) void k_fsrec(BatchArgs ba, int mode) {
     if (!(mode & MODE_RECORD) || r >= sd.n_rec) continue;
     const int ri = sd.rec[3 * r], rj = sd.rec[3 * r + 1], rt = sd.rec[3 * r + 2];
     float v;
-    auto fld = [&](int c) { return ld1<KF>(rowt((const T*)sd.f[c], rj, P) + ri, sd.fisc[c]); };
+    auto fld = [&](int c) {
+      return ld1<KF>(rowt((const T*)sd.f[c], rj, P) + ri, pcodec<KF>(sd.fisc[c], sd.loc));
+    };
     if (rt == REC_VZ) v = fld(F_VZ);
     else if (rt == REC_VX) v = fld(F_VX);
     else {
@@ -2051,7 +2171,7 @@ __global__ __launch_bounds__(256) void k_frame(BatchArgs ba, FrameArgs fa) {
     const int id = (int)(r % fa.nx_d), jd = (int)(r / fa.nx_d);
     const int i = min(id * fa.ds, sd.nx - 1), j = min(jd * fa.ds, sd.nz - 1);
     auto fld = [&](int c) {
-      return ld1<KF>(rowt((const T*)sd.f[c], j, P) + i, fa.own_sc ? fa.isc[c] : sd.fisc[c]);
+      return ld1<KF>(rowt((const T*)sd.f[c], j, P) + i, pcodec<KF>(fa.own_sc ? fa.isc[c] : sd.fisc[c], sd.loc));
     };
     const int which = fa.field[q];
     float v;

@@ -157,8 +157,10 @@ class Medium:
     lam: np.ndarray
     mu_txx: np.ndarray
     mu_txz: np.ndarray
-    # device storage of coefficients and wavefields: "f32", "bf16" or "f16"
-    # (FP32 arithmetic in every case); not a field of the reference's Medium
+    # device storage of coefficients and wavefields: "f32", "bf16", "f16",
+    # "f16c" (FP16 coefficients, FP32 fields) or "f24" (FP32 coefficients,
+    # fields rounded to a 16-bit significand); FP32 arithmetic in every case.
+    # Not a field of the reference's Medium
     storage: str = "f32"
 
 
@@ -268,7 +270,8 @@ class ShotResult:
 def init_medium(vp, vs, rho, dx, dz, nbc: int, fd_order: int, b=None,
                 free_surface: bool = True, storage: str = "f32") -> Medium:
     """``init_medium`` (utils/init.jl:68-99).  ``storage`` selects how the
-    device keeps the coefficients and wavefields ("f32", "bf16", "f16")."""
+    device keeps the coefficients and wavefields ("f32", "bf16", "f16", "f16c",
+    "f24")."""
     if fd_order < 2 or fd_order % 2:
         raise ValueError(f"Unsupported FD order: {fd_order}")
     if storage not in _lib.STORAGE:

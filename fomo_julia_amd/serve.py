@@ -144,11 +144,12 @@ def device_bytes(nx_pad: int, nz_pad: int, storage: str, nt: int, n_rec: int) ->
     """Device memory of one shot: coefficient blocks, wavefield state and
     gather, with the library's layout (pitched rows, zero guard rows)."""
     stride = (nz_pad + 36) * _pitch(nx_pad) + 512
-    ces = 4 if storage == "f32" else 2
-    fes = 2 if storage in ("bf16", "f16") else 4
+    ces = 4 if storage in ("f32", "f24") else 2
+    # f24: 2 bytes per field cell plus the byte-plane image (1 byte)
+    fes = {"bf16": 2, "f16": 2, "f24": 3}.get(storage, 4)
     fields = 15 if storage == "f32" else 10
-    # FP32 models also hold rho_vx and mu_txx for the derived-coefficient fused kernel
-    coef = 7 if storage == "f32" else 5
+    # FP32 coefficients also hold rho_vx and mu_txx for the derived-coefficient fused kernel
+    coef = 7 if storage in ("f32", "f24") else 5
     return coef * stride * ces + fields * stride * fes + nt * max(n_rec, 1) * 4 + 3 * 4096
 
 

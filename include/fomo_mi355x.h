@@ -65,6 +65,9 @@ extern "C" {
  *   BF16  coefficients and fields BF16
  *   F16   coefficients and fields FP16
  *   F16C  coefficients FP16, fields FP32
+ *   F24   coefficients FP32, fields rounded to a 16-bit significand (FP32's
+ *         exponent range, no scale): 3 bytes per field cell, a plane of each
+ *         value's high 16 bits and a plane of its next 8 bits
  * Fields are rounded to nearest (ties to even) whenever a stage writes them.
  * FP16 arrays carry a power-of-two scale (exact): per coefficient array from
  * its maximum, per field from a bound of the injected wavelet, so values stay
@@ -80,6 +83,7 @@ extern "C" {
 #define FOMO_STORAGE_BF16 1
 #define FOMO_STORAGE_F16 2
 #define FOMO_STORAGE_F16C 3
+#define FOMO_STORAGE_F24 4
 
 typedef struct fomo_ctx fomo_ctx;
 typedef struct fomo_model fomo_model;

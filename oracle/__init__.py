@@ -38,7 +38,7 @@ class OracleDesc(ctypes.Structure):
 # "bfp16" (block floating point fields, FP32 coefficients) and "bfp16c" (the
 # same with FP16 coefficients) are a design study of a 16-bit field format:
 # the oracle emulates them, no kernel implements them.
-STORAGE = {"f32": 0, "bf16": 1, "f16": 2, "f16c": 3, "bfp16": 4, "bfp16c": 5}
+STORAGE = {"f32": 0, "bf16": 1, "f16": 2, "f16c": 3, "bfp16": 4, "bfp16c": 5, "f24": 6}
 
 
 def build() -> str:
@@ -99,7 +99,8 @@ class OracleRun:
     "bf16" / "f16" / "f16c" (FP32 only) emulates reduced-precision storage of
     the coefficients and fields; ``fscale`` are the FP16 field store scales.
     "bfp16" emulates the block floating-point field format of the design study
-    (FP32 coefficients).
+    (FP32 coefficients); "f24" the library's F24 storage (FP32 coefficients,
+    fields rounded to a 16-bit significand).
     """
 
     def __init__(self, *, nx, nz, pad, nbc, fd_order, free_surface, dt, dtx, dtz,

@@ -55,6 +55,10 @@
 // [2^(e-1), 2^e), and per cell a signed 16-bit mantissa on the spacing
 // 2^(e-15), rounded to nearest (ties to even) and clamped to +-32767.
 // Storage 5 is the same field format with storage 3's FP16 coefficients.
+// Storage 6 (F24): FP32 coefficients; after each stage every cell of the
+// fields it writes is rounded to nearest, ties to even, on the grid of a
+// 16-bit significand with FP32's exponent range (spacing 2^(e-15) for
+// |x| in [2^(e-1), 2^e), 2^-141 below 2^-126).
 //
 // Pinning: see tests/test_oracle.py (polynomial-exactness of the stencils,
 // closed-form HABC cases, free surface, symmetry, 2^k linearity, energy decay,
@@ -80,7 +84,7 @@ typedef struct {
   double qx, qz, qt_x, qt_z, qxt;
   int32_t flags;
   int32_t threads;
-  int32_t storage;   // 0 = FP32 (no rounding), 1 = BF16, 2 = FP16, 3, 4 (see above)
+  int32_t storage;   // 0 = FP32 (no rounding), 1 = BF16, 2 = FP16, 3-6 (see above)
   float fscale[5];   // FP16 store scales of vx vz txx tzz txz
 } oracle_desc;
 }
@@ -117,6 +121,7 @@ static double round_format(double x, int p, int emin) {
 static float round_storage(int storage, float x, float scale) {
   if (storage == 1) return (float)round_format((double)x, 8, -126);          // BF16
   if (storage == 2) return (float)(round_format((double)x * scale, 11, -14) / scale);  // FP16
+  if (storage == 6) return (float)round_format((double)x, 16, -126);        // F24
   return x;
 }
 static uint32_t fmix32(uint32_t h) {
@@ -168,7 +173,7 @@ static void round_fields(Model<T>& m, int c0, int c1) {
     }
     return;
   }
-  if (m.storage != 1 && m.storage != 2) return;
+  if (m.storage != 1 && m.storage != 2 && m.storage != 6) return;
   for (int c = c0; c < c1; ++c) {
     const float sc = m.d->fscale[c];
 #pragma omp parallel for schedule(static) num_threads(m.d->threads)

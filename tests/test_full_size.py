@@ -34,8 +34,14 @@ def _f16_rne(v, scale):
     return (np.float16(v.astype(np.float64) * scale).astype(np.float64) / scale).astype(F32)
 
 
+def _f24_rne(v):
+    """v stored as F24 (a 16-bit significand, nearest, ties to even), as FP32 values."""
+    u = v.view(np.uint32).astype(np.uint64)
+    return ((u + 0x7F + ((u >> 8) & 1)) & 0xFFFFFF00).astype(np.uint32).view(F32)
+
+
 @pytest.mark.parametrize("storage,mode", [("f32", 1), ("bf16", 1), ("f16", 1), ("f16c", 1),
-                                          ("f32", 2), ("bf16", 2)])
+                                          ("f24", 1), ("f32", 2), ("bf16", 2), ("f24", 2)])
 def test_seam_random_state_bit_exact(storage, mode):
     import oracle as O
     dx = F32(12.5)
@@ -87,6 +93,8 @@ def test_seam_random_state_bit_exact(storage, mode):
         W.upload(f, v)
         if storage == "bf16":
             r.field(f)[:] = _bf16_rne(v)
+        elif storage == "f24":
+            r.field(f)[:] = _f24_rne(v)
         elif storage == "f16":
             r.field(f)[:] = _f16_rne(v, float(scales[f]))
         else:

@@ -1,4 +1,5 @@
-"""Reduced-precision storage (BF16, FP16, FP16 coefficients with FP32 fields).
+"""Reduced-precision storage (BF16, FP16, FP16 coefficients with FP32 fields,
+F24 fields with FP32 coefficients).
 
 The reference is FP32 only, so parity here has two parts:
   * the kernels against the oracle's sequential restatement of the storage
@@ -9,7 +10,8 @@ The reference is FP32 only, so parity here has two parts:
     (vx, vz, pressure), bounds written below.
 
 The rounding functions the oracle uses are pinned against numpy's float16 and
-torch's bfloat16 conversions (both round to nearest, ties to even)."""
+torch's bfloat16 conversions (both round to nearest, ties to even), and F24's
+against the integer rounding of the FP32 bit pattern and float64 arithmetic."""
 import numpy as np
 import pytest
 
@@ -24,7 +26,8 @@ F32 = np.float32
 C1_BOUNDS = {"f32": 1e-5,    # ~1.0e-6
              "f16c": 1e-3,   # ~8.2e-4 (vx), 5.3e-4 (vz), 5.6e-4 (p)
              "f16": 4e-3,    # ~2.7e-3 (vx), 1.9e-3 (vz), 3.0e-3 (p)
-             "bf16": 3e-2}   # ~2.2e-2 (vx), 1.7e-2 (vz), 2.4e-2 (p)
+             "bf16": 3e-2,   # ~2.2e-2 (vx), 1.7e-2 (vz), 2.4e-2 (p)
+             "f24": 3e-4}    # ~8.1e-5 (vx), 5.7e-5 (vz), 9.2e-5 (p) (oracle emulation)
 
 
 def _values(rng, n):
@@ -63,6 +66,49 @@ def test_bf16_rounding_matches_torch_and_bits():
     assert np.array_equal(want, bits)
     got = np.array([O.round_storage("bf16", float(v)) for v in x], F32)
     assert np.array_equal(got, want)
+
+
+def _f24_bits(x):
+    """x rounded to a 16-bit significand by the integer rounding of its FP32
+    bit pattern (add 0x7F plus the kept lowest bit, clear the low byte)."""
+    u = np.asarray(x, F32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7F + ((u >> 8) & 1)) & 0xFFFFFF00).astype(np.uint32).view(F32)
+
+
+def test_f24_rounding_matches_bits_and_float64():
+    """F24 (16-bit significand, FP32's exponent range): the oracle's rounding
+    equals the bit-pattern rounding the kernels use, and both equal rounding
+    in float64 to the grid 2^(e-15) (|x| in [2^(e-1), 2^e)), nearest, ties to
+    even; exact ties, subnormals, the largest finite values and FP32's
+    extremes included."""
+    O.build()
+    rng = np.random.default_rng(3)
+    mags = np.exp2(rng.uniform(-149, 127.9, 6000))
+    x = (mags * rng.choice([-1.0, 1.0], mags.size)).astype(F32)
+    base = (rng.integers(1 << 15, 1 << 16, 800).astype(np.float64) *
+            np.exp2(rng.integers(-140, 100, 800) - 15.0))
+    ties = np.concatenate([base + np.exp2(np.floor(np.log2(base)) - 16),     # halfway up
+                           base - np.exp2(np.floor(np.log2(base)) - 16)]).astype(F32)
+    special = np.array([0.0, -0.0, 1.0, 1.0 + 2.0 ** -16, 1.0 + 3 * 2.0 ** -16, 2.0 ** -149,
+                        2.0 ** -126, 3.4e38, -3.4e38, np.finfo(F32).max], F32)
+    x = np.concatenate([x, ties, special])
+    got = np.array([O.round_storage("f24", float(v)) for v in x], F32)
+    assert np.array_equal(got.view(np.uint32), _f24_bits(x).view(np.uint32))
+    want = []
+    for v in x.astype(np.float64):
+        if v == 0:
+            want.append(v)
+            continue
+        e = int(np.floor(np.log2(abs(v)))) + 1  # |v| in [2^(e-1), 2^e)
+        sp = 2.0 ** (max(e - 1, -126) - 15)
+        want.append(np.round(v / sp) * sp)  # numpy rounds half to even
+    with np.errstate(over="ignore"):
+        want = np.array(want).astype(F32)
+    assert np.array_equal(got, want)
+    # ties went to the even neighbour, and every result has its low byte clear
+    assert float(O.round_storage("f24", 1.0 + 2.0 ** -16)) == 1.0
+    assert float(O.round_storage("f24", 1.0 + 3 * 2.0 ** -16)) == 1.0 + 2.0 ** -14
+    assert not np.any(got.view(np.uint32) & 0xFF)
 
 
 @pytest.fixture(scope="module")
@@ -169,7 +215,7 @@ def _assert_same(res, W, r, what):
 
 
 @gpu
-@pytest.mark.parametrize("storage", ["bf16", "f16", "f16c"])
+@pytest.mark.parametrize("storage", ["bf16", "f16", "f16c", "f24"])
 @pytest.mark.parametrize("fd_order,fs,src_type", [(8, True, "pressure"), (4, False, "force_z"),
                                                   (12, True, "force_x")])
 def test_reduced_storage_bit_exact_vs_emulation(b, storage, fd_order, fs, src_type):
@@ -182,7 +228,7 @@ def test_reduced_storage_bit_exact_vs_emulation(b, storage, fd_order, fs, src_ty
 
 
 @gpu
-@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("storage", ["bf16", "f16", "f24"])
 @pytest.mark.parametrize("cpt,tz", [(2, 17), (4, 0)])
 def test_reduced_storage_tilings(b, storage, cpt, tz):
     c = Case(nx_in=150, nz_in=90, nbc=20, nt=71, storage=storage)
@@ -196,7 +242,7 @@ def test_reduced_storage_tilings(b, storage, cpt, tz):
 
 
 @gpu
-@pytest.mark.parametrize("storage", ["bf16", "f16", "f16c"])
+@pytest.mark.parametrize("storage", ["bf16", "f16", "f16c", "f24"])
 @pytest.mark.parametrize("cpt", [2, 4])
 def test_reduced_storage_order12_wide_tiles(b, storage, cpt):
     """fd_order 12 with 2 and 4 columns per lane in each reduced storage: the
@@ -212,7 +258,7 @@ def test_reduced_storage_order12_wide_tiles(b, storage, cpt):
 
 
 @gpu
-@pytest.mark.parametrize("storage", ["f32", "f16c", "f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f32", "f16c", "f16", "bf16", "f24"])
 def test_c1_accuracy_vs_float64(b, storage):
     """C1 with each storage: bit-exact against the emulation and within the
     stated relative L2 of the Float64 oracle, per component."""
@@ -233,9 +279,16 @@ def test_c1_accuracy_vs_float64(b, storage):
 
 
 @gpu
-def test_reduced_batch_uneven_matches_single(b):
-    cases = [Case(nx_in=90, nz_in=60, nt=50, storage="bf16", seed=1),
-             Case(nx_in=130, nz_in=45, nt=50, storage="bf16", seed=2, src_type="force_z")]
+@pytest.mark.parametrize("storage,mode", [("bf16", 1), ("f24", 1), ("f24", 2)])
+def test_reduced_batch_uneven_matches_single(b, storage, mode):
+    """A batch of uneven grids equals each shot run alone (and, for F24, the
+    oracle's emulation), in the whole-grid and the split step."""
+    cases = [Case(nx_in=90, nz_in=60, nt=50, storage=storage, seed=1),
+             Case(nx_in=130, nz_in=45, nt=50, storage=storage, seed=2, src_type="force_z"),
+             Case(nx_in=200, nz_in=110, nt=50, storage=storage, seed=3, src_type="force_x")]
+    print(f"case uneven batch {storage} mode {mode}", flush=True)
+    b.set_step_mode(mode)
+    b.set_split(7, 1)
     singles = []
     for c in cases:
         W = E.Wavefield(c.M.nx, c.M.nz, b)
@@ -257,11 +310,22 @@ def test_reduced_batch_uneven_matches_single(b):
     hb = ctypes.c_void_p()
     b.check(lib.fomo_batch_create(b.ctx, len(cases), arr(models), arr(ws), arr(shots),
                                   ctypes.byref(hb)), "batch_create")
-    b.check(lib.fomo_batch_run(hb, 1, cases[0].nt), "batch_run")
-    b.synchronize()
+    try:
+        b.check(lib.fomo_batch_run(hb, 1, cases[0].nt), "batch_run")
+        b.synchronize()
+    finally:
+        lib.fomo_batch_destroy(hb)
+        b.set_step_mode(1)
+        b.set_split(16, 0)
     for q, want in enumerate(singles):
         assert np.array_equal(keep[2 * q + 1].gather(), want)
-    lib.fomo_batch_destroy(hb)
+    if storage == "f24":
+        for q, c in enumerate(cases):
+            r = c.oracle(storage="f24")
+            r.run(1, c.nt)
+            assert np.array_equal(keep[2 * q + 1].gather(), r.gather_nt_by_nrec()), q
+            same = [np.array_equal(keep[2 * q].download(f), r.field(f)) for f in range(5)]
+            assert all(same), (q, same)
 
 
 @gpu
