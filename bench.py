@@ -196,7 +196,8 @@ def bench_precision(args) -> dict:
     SEAM-shaped shot (4444 x 3819, fd_order 8, nbc 50, free surface, 10 Hz
     Ricker pressure source at the surface centre): the first K steps (default
     400), receivers on every surface column recording vx, vz and pressure.
-    GPU runs for FP32, FP16 coefficients (FP32 fields), FP16 and BF16; the
+    GPU runs for FP32, F24 (fields rounded to a 16-bit significand, FP32
+    coefficients), FP16 coefficients (FP32 fields), FP16 and BF16; the
     oracle's emulation of the block floating-point field format (no kernel)
     beside them.  Float64 is the oracle on the host's CPUs."""
     from fomo_julia_amd import elastic2d as E
@@ -218,7 +219,7 @@ def bench_precision(args) -> dict:
            "steps": steps, "components": list(types), "receivers_per_component": n_col,
            "float64_oracle_seconds": round(f64_s, 1), "oracle_threads": threads,
            "data": "synthetic (seeded SEAM-like medium)", "gpu": {}, "emulated": {}}
-    for storage in ("f32", "f16c", "f16", "bf16"):
+    for storage in ("f32", "f24", "f16c", "f16", "bf16"):
         Ms = E.init_medium(vp, vs, rho, case["dx"], case["dx"], case["nbc"], case["fd_order"], None,
                            free_surface=True, storage=storage)
         W = E.Wavefield(Ms.nx, Ms.nz, b)
@@ -455,16 +456,16 @@ def bench_edges(args) -> dict:
             "data": "synthetic (seeded SEAM-like medium)"}
 
 
-def bench_serve(n_req: int, seed: int = 5, load: float = 0.8) -> dict:
+def bench_serve(n_req: int, seed: int = 5, load: float = 0.8, storages=("f32", "bf16")) -> dict:
     """Served stream (requests with grids nx 300-2000, nz 300-1200, nt
-    1000-6000, fd_order 4/8/12, FP32 and BF16 storage alternating): serial
-    plain FP32 execution, then the continuous-batching server with every
-    request queued at once (capacity), then with Poisson arrivals at `load`
-    times the serial service rate (latency)."""
+    1000-6000, fd_order 4/8/12, storage alternating over `storages`: FP32 and
+    BF16 by default): serial plain FP32 execution, then the continuous-batching
+    server with every request queued at once (capacity), then with Poisson
+    arrivals at `load` times the serial service rate (latency)."""
     from fomo_julia_amd import elastic2d as E
     from fomo_julia_amd import serve as S
     b = E.backend("mi355x")
-    reqs = S.c5_stream(n_req, seed=seed)
+    reqs = S.c5_stream(n_req, seed=seed, storages=tuple(storages))
     every = max(1, n_req // 20)
 
     def progress(phase):
@@ -493,7 +494,8 @@ def bench_serve(n_req: int, seed: int = 5, load: float = 0.8) -> dict:
     poi = srv2.serve(S.with_arrivals(reqs, rate), realtime=True, on_complete=progress("Poisson"))
     w_poi = time.perf_counter() - t
     s_ser, s_sat, s_poi = S.summarize(ser, w_ser), S.summarize(sat, w_sat), S.summarize(poi, w_poi)
-    return {"requests": n_req, "serial_f32": {**s_ser, "prepare_s": round(sum(prep_times), 3)},
+    return {"requests": n_req, "storages": list(storages),
+            "serial_f32": {**s_ser, "prepare_s": round(sum(prep_times), 3)},
             "served_capacity": s_sat,
             "served_poisson": {**s_poi, "arrival_rate_per_s": round(rate, 3), "load": load},
             "speedup_shots_per_hour": round(s_sat["shots_per_hour"] / s_ser["shots_per_hour"], 3),
@@ -583,9 +585,9 @@ def main():
     ap.add_argument("--split", default=None, metavar="EDGE_ROWS,BLOCKS_PER_CU",
                     help="split fused step: the interior tiles beside the edge tiles and the strip "
                          "kernels (rows per edge tile, interior blocks per CU, 0 = no cap); 'off' "
-                         "runs the whole-grid fused step.  Default: 16,1 for the single shot, off "
-                         "for the batch (whose edge tiles and strip kernels cover a third of its "
-                         "cells: split, its step is slower)")
+                         "runs the whole-grid fused step.  Default: 16,1 for the single shot "
+                         "(16,2 with F24 storage), off for the batch (whose edge tiles and strip "
+                         "kernels cover a third of its cells: split, its step is slower)")
     ap.add_argument("--storage", default="f32", choices=sorted(BYTES_FUSED_STORAGE),
                     help="device storage of coefficients/fields (arithmetic stays FP32)")
     ap.add_argument("--workload", default="seam",
@@ -600,6 +602,9 @@ def main():
     ap.add_argument("--requests", type=int, default=40,
                     help="requests of the served sample (default run) or stream (serve)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served sample")
+    ap.add_argument("--serve-storages", default="f32,bf16",
+                    help="storages the served requests alternate over (f32,bf16: the stream as "
+                         "specified; f32,f24: its reduced half in the F24 form within 1e-3)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -635,7 +640,7 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps({"metric": "shots/hour + p50/p99 latency (served), 1 GPU",
-                          **bench_serve(args.requests)}))
+                          **bench_serve(args.requests, storages=args.serve_storages.split(","))}))
         return
 
     import torch
@@ -653,7 +658,9 @@ def main():
     b = E.backend("mi355x")
     fused = not args.two_pass
     if args.split is None:
-        args.split = "16,1"
+        # F24's interior kernel gains more from a second block per CU than the
+        # strip kernels beside it lose (0.2665 against 0.277 ms on SEAM)
+        args.split = "16,2" if args.storage == "f24" else "16,1"
     split = fused and args.split not in ("", "off")
     b.set_step_mode(2 if split else 1 if fused else 0)
     if split:

@@ -50,7 +50,7 @@ def _alone(b, r):
 
 @gpu
 def test_served_equals_alone_mixed_orders_and_storage(b):
-    reqs = S.c5_stream(10, seed=21, storages=("f32", "bf16", "f16", "f16c"), **SMALL)
+    reqs = S.c5_stream(10, seed=21, storages=("f32", "bf16", "f16", "f16c", "f24"), **SMALL)
     print("case served mix of", [(r.nx, r.nz, r.nt, r.fd_order, r.storage) for r in reqs], flush=True)
     srv = S.Server(b, budget_bytes=8 << 30, chunk=16, capacity=8)
     done = srv.serve(reqs, realtime=False)
@@ -79,7 +79,7 @@ def test_staggered_joins_and_small_budget(b):
 @gpu
 def test_device_bytes_estimate_matches_library(b):
     from fomo_julia_amd import elastic2d as E
-    for storage in ("f32", "bf16", "f16c"):
+    for storage in ("f32", "bf16", "f16c", "f24"):
         r = S.c5_stream(1, seed=3, storages=(storage,), **SMALL)[0]
         p = S.prepare(r)
         dm = E._DeviceModel(b, p.M, p.H, p.a, p.p)
