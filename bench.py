@@ -52,6 +52,8 @@ BYTES_STRESS = 44    # two-pass stress: read vx vz lam lam+2mu mu_txz; read+writ
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fused_seam_traffic.txt")
 # the same for the split step's interior launches of k_fused
 TRAFFIC_FILE_SPLIT = os.path.join(ROOT, "profiles", "k_fused_interior_seam_traffic.txt")
+# and for the double step's two-step interior launches of k_fused2
+TRAFFIC_FILE_DOUBLE = os.path.join(ROOT, "profiles", "k_fused2_interior_seam_traffic.txt")
 METRIC = "Mcell-updates/s & %HBM roofline, 1 GPU; shots/hr + p99 latency (served)"
 # The autotuner's trials and the per-kernel timings run on a timing shot of
 # TIMING_NT steps (same medium, source cell and receivers), so that every
@@ -98,15 +100,16 @@ def split_interior_cells(nx: int, nz: int, nbc: int, M: int) -> int:
     return max(0, xi1 - xi0) * max(0, zi1 - zi0)
 
 
-def check_roofline(frac: float, kernel_ms: float, ms_per_step: float) -> None:
+def check_roofline(frac: float, kernel_ms: float, ms_per_step: float, steps_per_launch: int = 1) -> None:
     """A roofline line is only printed for a kernel that did its work: its
-    fraction of peak lies in (0, 1] and one launch is no longer than a step."""
+    fraction of peak lies in (0, 1] and one launch is no longer than the steps
+    it advances (one, or two for the double step's interior kernel)."""
     if not (0.0 < frac <= 1.0):
         raise SystemExit(f"bench.py: roofline fraction {frac:.4f} is outside (0, 1]: "
                          "the timed kernel did not do a step's work")
-    if not (0.0 < kernel_ms <= ms_per_step):
+    if not (0.0 < kernel_ms <= steps_per_launch * ms_per_step):
         raise SystemExit(f"bench.py: one kernel launch ({kernel_ms:.4f} ms) is not within "
-                         f"a step ({ms_per_step:.4f} ms)")
+                         f"{steps_per_launch} step(s) ({steps_per_launch * ms_per_step:.4f} ms)")
 
 
 def build_seam(nt: int, nbc=50, fd_order=8, with_weights=False, storage="f32", rec_types=("vz",)):
@@ -588,6 +591,11 @@ def main():
                          "runs the whole-grid fused step.  Default: 16,1 for the single shot "
                          "(16,2 with F24 storage), off for the batch (whose edge tiles and strip "
                          "kernels cover a third of its cells: split, its step is slower)")
+    ap.add_argument("--double", default=True, action=argparse.BooleanOptionalAction,
+                    help="double step (step mode 3, the default with FP32 storage): the interior "
+                         "advanced two time steps per pass, the split step's edge chain for each "
+                         "step beside it; --no-double runs the split step (or --split off, the "
+                         "whole-grid step)")
     ap.add_argument("--storage", default="f32", choices=sorted(BYTES_FUSED_STORAGE),
                     help="device storage of coefficients/fields (arithmetic stays FP32)")
     ap.add_argument("--workload", default="seam",
@@ -662,7 +670,8 @@ def main():
         # strip kernels beside it lose (0.2665 against 0.277 ms on SEAM)
         args.split = "16,2" if args.storage == "f24" else "16,1"
     split = fused and args.split not in ("", "off")
-    b.set_step_mode(2 if split else 1 if fused else 0)
+    double = split and args.double and args.storage == "f32"
+    b.set_step_mode(3 if double else 2 if split else 1 if fused else 0)
     if split:
         er, bpc = (int(v) for v in args.split.split(","))
         b.set_split(er, bpc)
@@ -755,6 +764,15 @@ def main():
             main_cells = split_interior_cells(M.nx, M.nz, H.nbc, p.fd_order // 2)
             main_name = (f"k_fused over the split step's interior tiles, {form} ({main_bytes} "
                          f"B/cell-update x {main_cells} interior cells per launch)")
+        if double:
+            # two steps per launch over the interior: one read of the 8 arrays
+            # and one write of the 5 fields per cell and launch
+            kernels["dbl_interior_2steps"] = main_ms = stage_ms(11, reps)
+            kernels["dbl_edge_step_k"] = stage_ms(12, reps)
+            kernels["dbl_edge_step_k1"] = stage_ms(13, reps)
+            main_name = (f"k_fused2 over the double step's interior tiles (two time steps per "
+                         f"launch), {form} ({main_bytes} B per cell and launch x {main_cells} "
+                         "interior cells)")
     else:
         main_ms = stage_ms(1, reps)
         kernels = {"velocity": stage_ms(0, reps), "stress": main_ms,
@@ -764,9 +782,9 @@ def main():
     main_gbs = main_bytes * main_cells / (main_ms / 1e3) / 1e9
     step_bytes = main_bytes if fused else BYTES_VELOCITY + BYTES_STRESS
     step_gbs = step_bytes * cells / (ms_per_step / 1e3) / 1e9
-    check_roofline(main_gbs / HBM_PEAK_GBS, main_ms, ms_per_step)
+    check_roofline(main_gbs / HBM_PEAK_GBS, main_ms, ms_per_step, 2 if double else 1)
     traffic = None
-    tfile = TRAFFIC_FILE_SPLIT if split else TRAFFIC_FILE
+    tfile = TRAFFIC_FILE_DOUBLE if double else TRAFFIC_FILE_SPLIT if split else TRAFFIC_FILE
     if fused and args.storage == "f32" and os.path.exists(tfile) and \
             (not split or args.split == "16,1"):
         for line in open(tfile):
@@ -845,9 +863,12 @@ def main():
         "tiling": ({"cols_per_lane": tuning["tiling"][0], "rows_per_tile": tuning["tiling"][1],
                     "autotuned": True, "trials_ms_per_step": tuning["trials"]} if tuning else
                    {"fixed": args.tiling or "library default"}),
-        "step_mode": ("split fused " + args.split) if split else "fused" if fused else "two-pass",
+        "step_mode": ("double, split edges " + args.split) if double else
+                     ("split fused " + args.split) if split else "fused" if fused else "two-pass",
         "step_gbs": round(step_gbs, 1),
         "step_frac_hbm": round(step_gbs / HBM_PEAK_GBS, 4),
+        "step_bytes_model": f"{step_bytes} B per cell-update (one step's compulsory traffic) x "
+                            "padded cells / time per step",
         "cpu_baseline": cpu,
         "served": served,
         "reference_published": {"cpu_wall_min": 35, "rtx3060_wall_min_upper": 3,

@@ -44,6 +44,9 @@ enum Part {
   PART_BAND,          // stress band tiles next to the HABC strips (ping-pong step)
   PART_FUSED_EDGE,    // split fused step: tiles of the frame (see ShotDev::xi0)
   PART_FUSED_INT,     // split fused step: tiles of the interior rectangle
+  PART_DBL_EDGE_A,    // double step, step k: edge tiles around the interior shrunk by 2M
+  PART_DBL_EDGE_B,    // double step, step k + 1: edge tiles around the interior
+  PART_DBL_INT,       // double step: two-step tiles of the interior rectangle
   PART_COUNT
 };
 

@@ -37,7 +37,9 @@ struct fomo_ctx {
   int tz = 0;          // rows per stencil tile; 0 = sized to fill the GPU in whole rounds
   int graph_steps = 16;
   int cus = 0;         // compute units of the device
-  int step_mode = 1;   // 1 = fused ping-pong step, 0 = two-pass in-place step, 2 = split fused step
+  // 1 = fused ping-pong step, 0 = two-pass in-place step, 2 = split fused step,
+  // 3 = double step (two steps per pass over the interior, split step otherwise)
+  int step_mode = 1;
   hipStream_t aux = nullptr;  // the split step's strip kernels (default priority, like `stream`)
   int tze = 16;        // split step: rows per edge tile
   int int_bpc = 0;     // split step: interior blocks per CU at most (0: as many as fit)
@@ -86,6 +88,13 @@ struct fomo_wavefield {
   // after a fused step the backup is the previous buffer's strip cells; old[]
   // is refreshed from it only when the in-place path or a download needs it
   bool backup_is_prev = false;
+  // the double step's third state buffer (allocated when a double-step batch
+  // is built): step k of a double step goes there outside the interior, so
+  // after one the previous state's strips are in it (prev_buf2)
+  void* dbuf2 = nullptr;
+  float* buf2[5] = {};
+  int64_t bytes2 = 0;
+  bool prev_buf2 = false;
   int64_t bytes = 0;
   fomo_shot* placeholder = nullptr;  // idle shot for stage calls that take none
   // one-shot batches cached per wavefield: [0] placeholder, [1] last real shot
@@ -117,7 +126,16 @@ static uint64_t g_next_shot_id = 1;
 // Shot tables of a batch: in-place kernels on buffer 0 / 1 (f = buffer,
 // fo = strip backups) and the fused step from buffer 0 / 1 (fo = read
 // buffer, f = written buffer).
-enum Table { TBL_IN0 = 0, TBL_IN1, TBL_FUSED0, TBL_FUSED1, TBL_COUNT };
+// Double step from buffer p (A = buffer p, B = buffer 1 - p, C = the third
+// buffer): TBL_DA* step k outside the interior shrunk by 2M, A -> C;
+// TBL_DB* step k + 1 outside the interior, C -> B; TBL_DI* the two-step
+// interior tiles, A -> B.  TBL_IN2: the third buffer with the strip backups
+// (refreshing old[] after a double step).
+enum Table {
+  TBL_IN0 = 0, TBL_IN1, TBL_FUSED0, TBL_FUSED1,
+  TBL_IN2, TBL_DA0, TBL_DA1, TBL_DB0, TBL_DB1, TBL_DI0, TBL_DI1,
+  TBL_COUNT
+};
 
 struct fomo_batch {
   fomo_ctx* ctx = nullptr;
@@ -139,6 +157,16 @@ struct fomo_batch {
   std::vector<int> hprefix;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // by the parity the chunk starts from
   hipGraph_t graph[2] = {nullptr, nullptr};
+  // Double step (step mode 3): two steps per launch over the interior
+  // (k_fused2), two split-step chains beside it; its own captured chunks.
+  bool dbl = false;
+  int tzd = 0;  // rows per two-step interior tile
+  // columns per lane of the double step's edge tiles: one, whose 88 VGPRs fit
+  // beside k_fused2's one wave per SIMD (a two-column edge wave does not, and
+  // then waits for the whole interior launch)
+  int ecpt = 1;
+  hipGraphExec_t gexec2[2] = {nullptr, nullptr};
+  hipGraph_t graph2[2] = {nullptr, nullptr};
   int grid_fixed[PART_COUNT] = {};  // > 0: fixed grid (pools: persistent over any membership)
   hipStream_t ls = nullptr;         // stream of all its launches (null: the context's)
   // Split fused step: the edge tiles run first; then the interior tiles on
@@ -382,7 +410,7 @@ extern "C" void* fomo_ctx_stream(fomo_ctx* c) { return c ? (void*)c->stream : nu
 
 extern "C" int32_t fomo_ctx_set_step_mode(fomo_ctx* c, int32_t mode) {
   if (!c) return fail(nullptr, FOMO_ERR_ARG, "ctx is NULL");
-  if (mode < 0 || mode > 2) return fail(c, FOMO_ERR_ARG, "step mode must be 0, 1 or 2");
+  if (mode < 0 || mode > 3) return fail(c, FOMO_ERR_ARG, "step mode must be 0, 1, 2 or 3");
   c->step_mode = mode;
   return FOMO_OK;
 }
@@ -764,6 +792,7 @@ extern "C" int32_t fomo_wavefield_destroy(fomo_wavefield* w) {
   (void)hipStreamSynchronize(w->m->ctx->stream);
   drop_single(w);
   (void)hipFree(w->dbuf);
+  if (w->dbuf2) (void)hipFree(w->dbuf2);
   delete w;
   return FOMO_OK;
 }
@@ -775,11 +804,14 @@ extern "C" int32_t fomo_wavefield_reset(fomo_wavefield* w) {
   HIP_TRY(c, hipMemsetAsync(w->dbuf, 0, (size_t)w->bytes, c->stream));
   w->cur = 0;
   w->backup_is_prev = false;
+  w->prev_buf2 = false;
   w->zero = true;
   return FOMO_OK;
 }
 
-extern "C" int64_t fomo_wavefield_device_bytes(const fomo_wavefield* w) { return w ? w->bytes : 0; }
+extern "C" int64_t fomo_wavefield_device_bytes(const fomo_wavefield* w) {
+  return w ? w->bytes + w->bytes2 : 0;
+}
 
 static float* field_ptr(fomo_wavefield* w, int field) {
   return field < 5 ? w->buf[w->cur][field] : w->old[field - 5];
@@ -1072,7 +1104,8 @@ extern "C" int32_t fomo_shot_gather_noalloc(fomo_shot* s, float* out) {
 
 // ------------------------------------------------------------------ batches
 static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const fomo_shot* s,
-                            int cpt, int tz, int table, int band_rows, int tzi = 0, int tze = 16) {
+                            int cpt, int tz, int table, int band_rows, int tzi = 0, int tze = 16,
+                            int tzd = 0) {
   ShotDev sd;
   std::memset(&sd, 0, sizeof(sd));
   const fomo_model_desc& d = m->d;
@@ -1106,7 +1139,13 @@ static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const 
       case TBL_IN0: sd.f[k] = w->buf[0][k]; sd.fo[k] = w->old[k]; break;
       case TBL_IN1: sd.f[k] = w->buf[1][k]; sd.fo[k] = w->old[k]; break;
       case TBL_FUSED0: sd.f[k] = w->buf[1][k]; sd.fo[k] = w->buf[0][k]; break;
-      default: sd.f[k] = w->buf[0][k]; sd.fo[k] = w->buf[1][k]; break;
+      case TBL_IN2: sd.f[k] = w->buf2[k]; sd.fo[k] = w->old[k]; break;
+      case TBL_DA0: sd.f[k] = w->buf2[k]; sd.fo[k] = w->buf[0][k]; break;
+      case TBL_DA1: sd.f[k] = w->buf2[k]; sd.fo[k] = w->buf[1][k]; break;
+      case TBL_DB0: sd.f[k] = w->buf[1][k]; sd.fo[k] = w->buf2[k]; break;
+      case TBL_DB1: sd.f[k] = w->buf[0][k]; sd.fo[k] = w->buf2[k]; break;
+      case TBL_DI0: sd.f[k] = w->buf[1][k]; sd.fo[k] = w->buf[0][k]; break;
+      default: sd.f[k] = w->buf[0][k]; sd.fo[k] = w->buf[1][k]; break;  // TBL_FUSED1, TBL_DI1
     }
   }
   // fused kernel tiling: lanes cover 64*cpt columns, the outer halo lanes
@@ -1126,6 +1165,15 @@ static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const 
   {
     const int mg = sd.band + 2 * M + 2;
     int xi0 = (mg + 1) & ~1, xi1 = (d.nx - mg) & ~1, zi0 = mg, zi1 = d.nz - mg;
+    // double step, first step: the edge tiles cover all but the interior
+    // shrunk by 2M, so that the second step's edge tiles (all but the
+    // interior) find every cell they read
+    if ((table == TBL_DA0 || table == TBL_DA1) && xi1 > xi0 && zi1 > zi0) {
+      xi0 += 2 * M;
+      xi1 -= 2 * M;
+      zi0 += 2 * M;
+      zi1 -= 2 * M;
+    }
     if (xi1 <= xi0 || zi1 <= zi0) {  // no interior: the edge tiles cover the grid
       xi0 = xi1 = 0;
       zi0 = zi1 = d.nz;
@@ -1145,6 +1193,14 @@ static ShotDev make_shotdev(const fomo_model* m, const fomo_wavefield* w, const 
     sd.e_mid = (zi1 - zi0 + tze - 1) / tze;
     sd.ntx_l = (xi0 + sd.ftw - 1) / sd.ftw;
     sd.ntx_r = (d.nx - xi1 + sd.ftw - 1) / sd.ftw;
+    if (table == TBL_DI0 || table == TBL_DI1) {
+      // two-step interior tiles (k_fused2: two columns per lane, 3M-column
+      // halo per side) of tzd rows over the same interior rectangle
+      sd.ftw = 128 - 2 * dbl_halo(M);
+      sd.tzi = std::max(1, tzd);
+      sd.nix = xi1 > xi0 ? (xi1 - xi0 + sd.ftw - 1) / sd.ftw : 0;
+      sd.niz = zi1 > zi0 ? (zi1 - zi0 + sd.tzi - 1) / sd.tzi : 0;
+    }
   }
   sd.bvx_x = m->coef[0];
   sd.bvx_z = m->coef[1];
@@ -1201,6 +1257,10 @@ static int part_items(const ShotDev& sd, int part) {
     case PART_BAND: return 2 * sd.nb_side + (sd.fs ? 1 : 2) * sd.nb_mid;
     case PART_FUSED_EDGE: return sd.e_top + sd.e_bot + (sd.ntx_l + sd.ntx_r) * sd.e_mid;
     case PART_FUSED_INT: return sd.nix * sd.niz;
+    // the double step's parts, counted on its TBL_DA / TBL_DB / TBL_DI rows
+    case PART_DBL_EDGE_A:
+    case PART_DBL_EDGE_B: return sd.e_top + sd.e_bot + (sd.ntx_l + sd.ntx_r) * sd.e_mid;
+    case PART_DBL_INT: return sd.nix * sd.niz;
   }
   return 0;
 }
@@ -1273,7 +1333,8 @@ static int stencil_slots(int storage, int M, int cpt, int cus, int mode, bool co
 // rows in four rounds); the two-pass kernels re-read about 4 halo rows per
 // tile and stay within [8, 64].
 static int auto_tz(const std::vector<int>& tiles_x, const std::vector<int>& nz, int slots,
-                   int M, int block) {
+                   int M, int block, int halo = -1) {
+  if (halo < 0) halo = 2 * M;  // rows a fused tile marches beyond its own
   int64_t rows = 0;
   for (size_t i = 0; i < nz.size(); ++i) rows += (int64_t)tiles_x[i] * nz[i];
   int best = 32;
@@ -1283,7 +1344,7 @@ static int auto_tz(const std::vector<int>& tiles_x, const std::vector<int>& nz, 
     int64_t it = 0;
     for (size_t i = 0; i < nz.size(); ++i) it += (int64_t)tiles_x[i] * ((nz[i] + tz - 1) / tz);
     const int64_t rounds = (it + slots - 1) / slots;
-    const double march = block > 0 ? (double)((tz + 2 * M + block - 1) / block * block) : tz + 4.0;
+    const double march = block > 0 ? (double)((tz + halo + block - 1) / block * block) : tz + 4.0;
     const double eff = (double)rows / ((double)rounds * slots * march);
     const double work = (double)it * march;
     if (eff > best_eff + 1e-9 || (eff > best_eff - 1e-9 && work < best_work)) {
@@ -1295,11 +1356,42 @@ static int auto_tz(const std::vector<int>& tiles_x, const std::vector<int>& nz, 
   return best;
 }
 
+// Orders the two-step interior kernel is built for.
+static bool dbl_order(int M) { return M == 2 || M == 4 || M == 6; }
+
+// Resident wavefronts of k_fused2 on the whole device.
+template <int M>
+static int dbl_slots_m(int cus) {
+  int bv = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused2<M, WPB>, 64 * WPB, 0) != hipSuccess) bv = 1;
+  return std::max(1, bv) * WPB * std::max(cus, 1);
+}
+static int dbl_slots(int M, int cus) {
+  return M == 2 ? dbl_slots_m<2>(cus) : M == 4 ? dbl_slots_m<4>(cus) : dbl_slots_m<6>(cus);
+}
+
+// The third state buffer of w (the double step), laid out like the other two
+// and zeroed once: its guard rows and padding are never written.
+static int ensure_buf2(fomo_wavefield* w) {
+  if (w->dbuf2) return FOMO_OK;
+  const fomo_model* m = w->m;
+  fomo_ctx* c = m->ctx;
+  const size_t bytes = 5 * m->fstride * m->fes;
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMalloc(&w->dbuf2, bytes));
+  HIP_TRY(c, hipMemsetAsync(w->dbuf2, 0, bytes, c->stream));
+  for (int k = 0; k < 5; ++k) w->buf2[k] = block_origin(w->dbuf2, m->fstride, k, m->pitch, m->fes);
+  w->bytes2 = (int64_t)bytes;
+  return FOMO_OK;
+}
+
 static void destroy_batch(fomo_batch* b) {
   if (!b) return;
   for (int q = 0; q < 2; ++q) {
     if (b->gexec[q]) (void)hipGraphExecDestroy(b->gexec[q]);
     if (b->graph[q]) (void)hipGraphDestroy(b->graph[q]);
+    if (b->gexec2[q]) (void)hipGraphExecDestroy(b->gexec2[q]);
+    if (b->graph2[q]) (void)hipGraphDestroy(b->graph2[q]);
   }
   (void)hipFree(b->ds);
   (void)hipFree(b->dn);
@@ -1319,8 +1411,8 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
   b->tz = c->tz;
   b->tz_req = c->tz;
   b->graph_steps = c->graph_steps;
-  b->split = c->step_mode == 2;
-  b->step_mode = c->step_mode == 2 ? 1 : c->step_mode;
+  b->split = c->step_mode >= 2;
+  b->step_mode = c->step_mode >= 2 ? 1 : c->step_mode;
   b->step_mode_req = c->step_mode;
   b->tze = c->tze;
   b->int_bpc = c->int_bpc;
@@ -1384,12 +1476,49 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
     if (b->int_bpc > 0) slots = std::min(slots, b->int_bpc * WPB * std::max(c->cus, 1));
     b->tzi = c->tz > 0 ? c->tz : auto_tz(tx, nzs, slots, b->M, fused_block(b->M));
   }
+  // Double step: FP32 with derived coefficients, an order k_fused2 is built
+  // for, and per member nbc >= M (every cell within 3M of the interior is
+  // then at least M from the grid's edge, which k_fused2 relies on) and an
+  // interior that stays non-empty shrunk by 2M, with no receiver inside the
+  // shrunk interior (its first step is recorded from the third buffer, which
+  // holds that step outside it).  Otherwise the split step runs.
+  if (b->split && b->step_mode == 1 && c->step_mode == 3 && b->storage == FOMO_STORAGE_F32 && b->compact &&
+      dbl_order(b->M)) {
+    bool ok = true;
+    std::vector<int> tx, nzs;
+    for (int i = 0; i < n && ok; ++i) {
+      const ShotDev g = make_shotdev(ms[i], ws[i], ss[i], b->ecpt, b->tz, TBL_DA0, BAND_ROWS_SINGLE,
+                                     b->tzi, b->tze);
+      if (g.xi1 <= g.xi0 || ms[i]->d.nbc < ms[i]->M) ok = false;
+      const std::vector<int>& r = ss[i]->hrec;
+      for (int q = 0; q < ss[i]->n_rec && ok; ++q) {
+        const int ri = r[3 * q], rj = r[3 * q + 1];
+        if (ri >= g.xi0 && ri < g.xi1 && rj >= g.zi0 && rj < g.zi1) ok = false;
+      }
+      const ShotDev h = make_shotdev(ms[i], ws[i], ss[i], b->cpt, b->tz, TBL_DI0, BAND_ROWS_SINGLE,
+                                     b->tzi, b->tze, 1);
+      tx.push_back(h.nix);
+      nzs.push_back(h.zi1 - h.zi0);
+    }
+    if (ok) {
+      for (int i = 0; i < n; ++i) {
+        int rc = ensure_buf2(ws[i]);
+        if (rc) return rc;
+      }
+      int slots = dbl_slots(b->M, c->cus);
+      if (b->int_bpc > 0) slots = std::min(slots, b->int_bpc * WPB * std::max(c->cus, 1));
+      b->tzd = c->tz > 0 ? c->tz : auto_tz(tx, nzs, slots, b->M, fused_block(b->M), 6 * b->M);
+      b->dbl = true;
+    }
+  }
   b->hs.resize((size_t)TBL_COUNT * n);
-  for (int t = 0; t < TBL_COUNT; ++t)
+  for (int t = 0; t < TBL_COUNT; ++t) {
+    const bool dedge = t == TBL_DA0 || t == TBL_DA1 || t == TBL_DB0 || t == TBL_DB1;
     for (int i = 0; i < n; ++i)
-      b->hs[(size_t)t * n + i] = make_shotdev(ms[i], ws[i], ss[i], b->cpt, b->tz, t,
+      b->hs[(size_t)t * n + i] = make_shotdev(ms[i], ws[i], ss[i], dedge ? b->ecpt : b->cpt, b->tz, t,
                                               n == 1 ? BAND_ROWS_SINGLE : BAND_ROWS_BATCH,
-                                              b->split ? b->tzi : b->tz, b->tze);
+                                              b->split ? b->tzi : b->tz, b->tze, b->tzd);
+  }
   if (b->split) {
     b->rec_in_frame = true;
     for (int i = 0; i < n; ++i) {
@@ -1408,7 +1537,9 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
     int acc = 0;
     for (int i = 0; i < n; ++i) {
       b->hprefix[(size_t)p * (n + 1) + i] = acc;
-      const int64_t next = (int64_t)acc + part_items(b->hs[i], p);
+      const int row = p == PART_DBL_EDGE_A ? TBL_DA0 : p == PART_DBL_EDGE_B ? TBL_DB0
+                      : p == PART_DBL_INT ? TBL_DI0 : TBL_IN0;
+      const int64_t next = (int64_t)acc + part_items(b->hs[(size_t)row * n + i], p);
       if (next > (1 << 30)) return fail(c, FOMO_ERR_ARG, "batch work list too large");
       acc = (int)next;
     }
@@ -1489,11 +1620,16 @@ template <int K, int M, int CPT>
 static void launch_stencil_mc(const fomo_batch* b, int table, int kind, int flags, int k_off) {
   dim3 block(64 * WPB);
   hipStream_t st = lstream(b);
-  if (kind == 2 || kind == 4 || kind == 5) {  // grid rounded up to whole XCD rounds (see xcd_block)
+  if (kind == 2 || kind == 4 || kind == 5 || kind == 6 || kind == 8) {  // grid in whole XCD rounds (xcd_block)
     // 4: the split step's edge tiles, 5: its interior tiles (at most int_bpc
-    // blocks per CU, so that the strip kernels beside them find free slots)
-    const int part = kind == 2 ? PART_FUSED : kind == 4 ? PART_FUSED_EDGE : PART_FUSED_INT;
-    if (kind == 4) flags |= FLAG_EDGE;
+    // blocks per CU, so that the strip kernels beside them find free slots),
+    // 6 / 8: the double step's edge tiles of step k / k + 1
+    const int part = kind == 2   ? PART_FUSED
+                     : kind == 4 ? PART_FUSED_EDGE
+                     : kind == 6 ? PART_DBL_EDGE_A
+                     : kind == 8 ? PART_DBL_EDGE_B
+                                 : PART_FUSED_INT;
+    if (kind == 4 || kind == 6 || kind == 8) flags |= FLAG_EDGE;
     if (kind == 5) flags |= FLAG_INT;
     int nb = blocks_for(b, part);
     if (kind == 5 && b->int_bpc > 0) nb = std::min(nb, b->int_bpc * std::max(b->ctx->cus, 1));
@@ -1528,7 +1664,8 @@ static void launch_stencil_mc(const fomo_batch* b, int table, int kind, int flag
 
 template <int K, int M>
 static void launch_stencil_m(const fomo_batch* b, int table, int kind, int flags, int k_off) {
-  const int cpt = kind == 3 ? 1 : b->cpt;  // the band kernel has one column per lane
+  // the band kernel has one column per lane; the double step's edge tiles ecpt
+  const int cpt = kind == 3 ? 1 : (kind == 6 || kind == 8) ? b->ecpt : b->cpt;
   switch (cpt) {
     case 1: launch_stencil_mc<K, M, 1>(b, table, kind, flags, k_off); break;
     case 2: launch_stencil_mc<K, M, 2>(b, table, kind, flags, k_off); break;
@@ -1560,6 +1697,21 @@ static void launch_stencil(const fomo_batch* b, int table, int kind, int flags, 
     case 6: launch_stencil_m<ST_F32, 6>(b, table, kind, flags, k_off); break;
     case 7: launch_stencil_m<ST_F32, 7>(b, table, kind, flags, k_off); break;
     default: launch_stencil_m<ST_F32, 8>(b, table, kind, flags, k_off); break;
+  }
+}
+
+// The double step's two-step interior tiles (FP32, derived coefficients),
+// at most int_bpc blocks per CU like the split step's interior.
+static void launch_dbl_interior(const fomo_batch* b, int table, int flags, int k_off) {
+  int nb = blocks_for(b, PART_DBL_INT);
+  if (b->int_bpc > 0) nb = std::min(nb, b->int_bpc * std::max(b->ctx->cus, 1));
+  const dim3 grid(round_up(nb, NUM_XCD)), block(64 * WPB);
+  const BatchArgs a = args_for(b, table, PART_DBL_INT, k_off);
+  hipStream_t st = lstream(b);
+  switch (b->M) {
+    case 2: hipLaunchKernelGGL((k_fused2<2, WPB>), grid, block, 0, st, a, flags); break;
+    case 4: hipLaunchKernelGGL((k_fused2<4, WPB>), grid, block, 0, st, a, flags); break;
+    default: hipLaunchKernelGGL((k_fused2<6, WPB>), grid, block, 0, st, a, flags); break;
   }
 }
 
@@ -1659,6 +1811,47 @@ static void launch_step(const fomo_batch* b, int parity, int k_off) {
   }
 }
 
+// Steps k_off and k_off + 1 of every shot from state buffer `parity` into
+// buffer 1 - parity (time_stepper.jl:39-63, twice).  On the aux stream, two
+// split-step chains: step k outside the interior shrunk by 2M into the third
+// buffer C, then step k + 1 outside the interior from C.  On the batch's
+// stream, beside them, the interior's two steps in one launch (k_fused2).
+// Why this is race-free:
+//   - A (buffer `parity`) is only read during the double step;
+//   - C is written by the first chain and read by the second, in stream
+//     order; the second chain reads C within 2M of the cells it writes, all
+//     of which the first chain wrote (the shrunk interior is 2M smaller);
+//   - B is written on disjoint cells: the interior by k_fused2, the rest by
+//     the second chain, whose strip kernels read only cells at least 2M + 2
+//     away from the interior;
+//   - the interior's two steps read A within 4M of the interior, whose
+//     step-k cells (within 2M) are outside every strip kernel's cells, so
+//     no HABC, band or free-surface update is missed.
+// Receivers are recorded from C for step k (none lies in the shrunk
+// interior) and from B for step k + 1.
+static void launch_double(const fomo_batch* b, int parity, int k_off) {
+  const int ta = TBL_DA0 + parity, tb = TBL_DB0 + parity, ti = TBL_DI0 + parity;
+  hipStream_t st = lstream(b), ax = b->ctx->aux;
+  (void)hipEventRecord(b->ev_edge, st);
+  (void)hipStreamWaitEvent(ax, b->ev_edge, 0);
+  b->cur = ax;
+  launch_stencil(b, ta, 6, FLAG_INJECT, k_off);
+  launch_habc(b, ta, false, k_off);
+  launch_stencil(b, ta, 3, FLAG_INJECT, k_off);
+  launch_habc(b, ta, true, k_off);
+  launch_fsrec(b, ta, MODE_FS | MODE_RECORD, k_off);
+  launch_stencil(b, tb, 8, FLAG_INJECT, k_off + 1);
+  launch_habc(b, tb, false, k_off + 1);
+  launch_stencil(b, tb, 3, FLAG_INJECT, k_off + 1);
+  launch_habc(b, tb, true, k_off + 1);
+  launch_fsrec(b, tb, b->rec_in_frame ? (MODE_FS | MODE_RECORD) : MODE_FS, k_off + 1);
+  (void)hipEventRecord(b->ev_chain, ax);
+  b->cur = nullptr;
+  launch_dbl_interior(b, ti, FLAG_INJECT, k_off);
+  (void)hipStreamWaitEvent(st, b->ev_chain, 0);
+  if (!b->rec_in_frame) launch_fsrec(b, tb, MODE_RECORD, k_off + 1);
+}
+
 static void launch_set_step(const fomo_batch* b, int k) {
   hipLaunchKernelGGL(k_set_step, dim3((b->n + 255) / 256), dim3(256), 0, lstream(b), b->ds,
                      b->dn, k);
@@ -1671,14 +1864,19 @@ static void launch_advance(const fomo_batch* b, int by) {
 
 // Captures one chunk of G whole steps, starting from state buffer p, into
 // b->gexec[p] (the step counters advance by G as the last node).
-static int capture_chunk(fomo_batch* b, int p) {
+// dbl: G / 2 double steps instead, into b->gexec2[p] (the chunk ends on
+// parity p + G / 2).
+static int capture_chunk(fomo_batch* b, int p, bool dbl = false) {
   API_LOCK;
   fomo_ctx* c = b->ctx;
   const int G = b->graph_steps;
   const int flip = b->step_mode == 1 ? 1 : 0;
   hipStream_t st = lstream(b);
   HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  for (int g = 0; g < G; ++g) launch_step(b, (p + flip * g) & 1, g);
+  if (dbl)
+    for (int g = 0; g < G / 2; ++g) launch_double(b, (p + g) & 1, 2 * g);
+  else
+    for (int g = 0; g < G; ++g) launch_step(b, (p + flip * g) & 1, g);
   launch_advance(b, G);
   const hipError_t le = hipGetLastError();
   hipGraph_t graph = nullptr;
@@ -1688,8 +1886,8 @@ static int capture_chunk(fomo_batch* b, int p) {
     return fail(c, FOMO_ERR_HIP, std::string("step capture: ") +
                                      hipGetErrorString(le != hipSuccess ? le : e));
   }
-  b->graph[p] = graph;
-  HIP_TRY(c, hipGraphInstantiate(&b->gexec[p], graph, nullptr, nullptr, 0));
+  (dbl ? b->graph2 : b->graph)[p] = graph;
+  HIP_TRY(c, hipGraphInstantiate(&(dbl ? b->gexec2 : b->gexec)[p], graph, nullptr, nullptr, 0));
   return FOMO_OK;
 }
 
@@ -1708,9 +1906,11 @@ static int refresh_backup(fomo_wavefield* w) {
   if (rc) return rc;
   HIP_TRY(w->m->ctx, hipSetDevice(w->m->ctx->device));
   launch_set_step(b, 1);
+  const int t = w->prev_buf2 ? TBL_IN2 : TBL_IN0 + (1 - w->cur);
   hipLaunchKernelGGL(k_backup<WPB>, dim3(blocks_for(b, PART_BACKUP)), dim3(64 * WPB), 0,
-                     w->m->ctx->stream, args_for(b, TBL_IN0 + (1 - w->cur), PART_BACKUP, 0));
+                     w->m->ctx->stream, args_for(b, t, PART_BACKUP, 0));
   w->backup_is_prev = false;
+  w->prev_buf2 = false;
   return check_launch(w->m->ctx, "refresh_backup");
 }
 
@@ -1733,6 +1933,37 @@ static int run_batch(fomo_batch* b, int k_first, int n_steps) {
   }
   HIP_TRY(c, hipSetDevice(c->device));
   launch_set_step(b, k_first);
+  // Double steps where every member is live for every step of the run (a
+  // double step needs both of its steps); an odd last step is a split step.
+  bool dbl = b->dbl && !b->frames_id && n_steps >= 2 && k_first >= 1;
+  for (int i = 0; i < b->n && dbl; ++i)
+    if ((int64_t)k_first + n_steps - 1 > b->ss[i]->nt) dbl = false;
+  if (dbl) {
+    const int G = b->graph_steps, H = G / 2;
+    const int pairs = n_steps / 2, single = n_steps & 1;
+    const int chunks = H > 0 ? pairs / H : 0, rest = pairs - chunks * H;
+    int q = parity;
+    for (int i = 0; i < chunks; ++i) {
+      if (!b->gexec2[q]) {
+        int rc = capture_chunk(b, q, true);
+        if (rc) return rc;
+      }
+      HIP_TRY(c, hipGraphLaunch(b->gexec2[q], c->stream));
+      q = (q + H) & 1;
+    }
+    for (int g = 0; g < rest; ++g, q ^= 1) launch_double(b, q, 2 * g);
+    if (single) {
+      launch_step(b, q, 2 * rest);
+      q ^= 1;
+    }
+    if (2 * rest + single) launch_advance(b, 2 * rest + single);
+    for (fomo_wavefield* w : b->ws) {
+      w->cur = q;
+      w->backup_is_prev = true;
+      w->prev_buf2 = !single;  // the last step was a double step's second
+    }
+    return check_launch(c, "time loop");
+  }
   const int G = b->graph_steps;  // even: a chunk ends on the parity it started from; 0: no graphs
   const int chunks = G > 0 ? n_steps / G : 0, rem = G > 0 ? n_steps % G : n_steps;
   const int flip = b->step_mode == 1 ? 1 : 0;
@@ -1747,6 +1978,7 @@ static int run_batch(fomo_batch* b, int k_first, int n_steps) {
     for (fomo_wavefield* w : b->ws) {
       w->cur = (parity + n_steps) & 1;
       w->backup_is_prev = true;
+      w->prev_buf2 = false;
     }
   }
   return check_launch(c, "time loop");
@@ -1763,7 +1995,7 @@ static void set_frames(fomo_batch* b, const fomo_frames* f) {
     b->gexec[q] = nullptr;
     b->graph[q] = nullptr;
   }
-  b->frames_id = id;
+  b->frames_id = id;  // double-step chunks hold no frame store (frames run split steps)
   b->fargs = f ? f->a : FrameArgs{};
 }
 
@@ -1792,6 +2024,10 @@ extern "C" int32_t fomo_run_time_loop_prepare(fomo_model* m, fomo_wavefield* w, 
     const int par = (w->cur + q) & 1;
     if (!b->gexec[par]) {
       rc = capture_chunk(b, par);
+      if (rc) return rc;
+    }
+    if (b->dbl && !b->gexec2[par]) {  // double-step chunks start from either parity
+      rc = capture_chunk(b, par, true);
       if (rc) return rc;
     }
   }
@@ -1918,6 +2154,7 @@ extern "C" int32_t fomo_time_step(fomo_model* m, fomo_wavefield* w, fomo_shot* s
   if (b->step_mode == 1) {
     w->cur ^= 1;
     w->backup_is_prev = true;
+    w->prev_buf2 = false;
   }
   return check_launch(m->ctx, "time_step");
 }
@@ -1978,7 +2215,7 @@ extern "C" int32_t fomo_record_receivers(fomo_model* m, fomo_wavefield* w, fomo_
 extern "C" int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t which,
                                      int32_t k, int32_t reps) {
   if (!m || !w || !s) return fail(m ? m->ctx : nullptr, FOMO_ERR_ARG, "NULL argument");
-  if (which < 0 || which > 10 || reps < 0) return fail(m->ctx, FOMO_ERR_ARG, "bad stage or reps");
+  if (which < 0 || which > 13 || reps < 0) return fail(m->ctx, FOMO_ERR_ARG, "bad stage or reps");
   if (which < 4 && m->storage != FOMO_STORAGE_F32)
     return fail(m->ctx, FOMO_ERR_UNSUPPORTED, "the in-place stages need FP32 storage");
   fomo_batch* b = nullptr;
@@ -1986,6 +2223,8 @@ extern "C" int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot
   if (rc) return rc;
   HIP_TRY(m->ctx, hipSetDevice(m->ctx->device));
   launch_set_step(b, k);
+  if (which >= 11 && !b->dbl)
+    return fail(m->ctx, FOMO_ERR_UNSUPPORTED, "stages 11-13 need a double-step run (step mode 3)");
   const int ti = TBL_IN0 + w->cur, tf = TBL_FUSED0 + w->cur;
   for (int r = 0; r < reps; ++r) {
     switch (which) {
@@ -2000,6 +2239,9 @@ extern "C" int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot
       case 8: launch_habc(b, tf, true, 0); break;
       case 9: launch_stencil(b, tf, 4, FLAG_INJECT, 0); break;
       case 10: launch_stencil(b, tf, 5, FLAG_INJECT, 0); break;
+      case 11: launch_dbl_interior(b, TBL_DI0 + w->cur, FLAG_INJECT, 0); break;
+      case 12: launch_stencil(b, TBL_DA0 + w->cur, 6, FLAG_INJECT, 0); break;
+      case 13: launch_stencil(b, TBL_DB0 + w->cur, 8, FLAG_INJECT, 1); break;
     }
   }
   return check_launch(m->ctx, "launch_stage");
@@ -2216,6 +2458,7 @@ extern "C" int32_t fomo_pool_remove(fomo_pool* p, fomo_shot* s) {
   fomo_wavefield* w = p->ws[i];
   w->cur = p->c0[i] ^ (done & 1);
   w->backup_is_prev = done > 0;
+  w->prev_buf2 = false;
   --p->ms[i]->pool_refs;
   --w->pool_refs;
   --s->pool_refs;

@@ -1307,6 +1307,397 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two time steps in one z-march (the interior tiles of the double step,
+// step mode 3): velocity and stress of step k from the state in A, then
+// velocity and stress of step k + 1 from those step-k values, which stay in
+// registers and LDS; only the step-(k+1) state is written (to B).  FP32
+// storage with derived coefficients and dtx == dtz (k_fused<M, 2, WPB,
+// ST_F32, true, true>), two columns per lane.
+//
+// Iteration jv computes velocity(k) of row jv, stress(k) of row jv - M,
+// velocity(k+1) of row jv - 2M and stress(k+1) of row jv - 3M.  In x each
+// half-step narrows the exact columns by M per side: velocity(k) is exact on
+// all 128 lane columns (halo lanes stage the old stress M beyond them),
+// stress(k) on [M, 127 - M], velocity(k+1) on [2M, 127 - 2M], stress(k+1) on
+// [3M, 127 - 3M]; the written columns are the inner 128 - 2 dbl_halo(M).
+// Lanes outside a half-step's exact range compute values that only other
+// inexact lanes read.
+//
+// The arithmetic of each step is k_fused's, operation for operation, so two
+// steps here equal two k_fused steps bit for bit:
+//   - the step-k pressure source is added to old txx / tzz at their first use
+//     (as k_fused); the step-(k+1) one to the step-k txx / tzz values of the
+//     source cell as they enter the rings, which every later reader sees;
+//   - the coefficients k_fused forms per cell (dtx/rho_vx, dtx/rho_vz,
+//     lam + 2mu, mu_txz) are formed once, at step k, and kept in rings for
+//     step k + 1 (same rows, so the same values).
+// All register rings have L = fused_block(M) slots, row r in slot
+// (r - jb) mod L; the longest live span (lam of rows jv - 3M .. jv + PD - M)
+// is 2M + 1 + PD <= L.
+// ---------------------------------------------------------------------------
+constexpr int dbl_halo(int M) { return (3 * M + 1) / 2 * 2; }  // per side, two columns per lane
+
+template <bool B>
+struct BoolTag {
+  static constexpr bool value = B;
+};
+
+template <int M, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
+  constexpr int CPT = 2, LC = 64 * CPT, LW = LC + 2 * M, PD = FUSED_PD, L = fused_block(M);
+  constexpr int HL = dbl_halo(M);
+  static_assert(2 * M + 1 + PD <= L, "ring too short");
+  __shared__ float lds[WPB][4][LW];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  float* s_txx = lds[wid][0];
+  float* s_txz = lds[wid][1];
+  float* s_vx = lds[wid][2];
+  float* s_vz = lds[wid][3];
+  const int nsh = *ba.n_shots_p;
+  const int total = ba.prefix[nsh];
+  for (int item = uniform(xcd_block() * WPB + wid); item < total;
+       item = uniform(item + gridDim.x * WPB)) {
+    const int s = uniform(find_shot(ba.prefix, nsh, item));
+    const ShotDev& sd = ba.shots[s];
+    const int k = shot_step(sd, ba.k_off);
+    // the host issues double steps only where both steps are live
+    if (k == 0 || shot_step(sd, ba.k_off + 1) == 0) continue;
+    const int t = item - ba.prefix[s];
+    const int P = uniform(sd.pitch);
+    int o0, xend, j0, j1;
+    fused_tile(sd, t, FLAG_INT, o0, xend, j0, j1);
+    const int ib0 = o0 - HL;    // first lane column of the tile
+    const int lo = lane * CPT;  // this lane's column offset in the tile
+    const int ib = ib0 + lo;    // this lane's first column
+    int hx = -1;  // LDS slot of this lane's halo column (velocity(k) x-derivatives)
+    if (lane < M) hx = lane;
+    else if (lane >= 64 - M) hx = LC + M + (lane - (64 - M));
+    const bool has_halo = hx >= 0;
+    const int hcol = ib0 - M + hx;
+    const Rsrc rA = make_rsrc(uniform_p(sd.bufA), (uint32_t)uniform((int)sd.recF));
+    const Rsrc rB = make_rsrc(uniform_p(sd.bufB), (uint32_t)uniform((int)sd.recF));
+    const Rsrc rC = make_rsrc(uniform_p(sd.bufC), (uint32_t)uniform((int)sd.recC));
+    const uint32_t fstr = (uint32_t)uniform((int)sd.fstr_b), forg = (uint32_t)uniform((int)sd.forg_b);
+    const uint32_t cstr = (uint32_t)uniform((int)sd.cstr_b), corg = (uint32_t)uniform((int)sd.corg_b);
+    constexpr int QLAM = 2, QRHO = 5, QMUX = 6;
+    const int nw = uniform(sd.nw), stype = uniform(sd.stype);
+    const bool inj = (flags & FLAG_INJECT) != 0;
+    const bool on1 = inj && k <= nw, on2 = inj && k + 1 <= nw;
+    const float w1 = on1 ? uniform_f(uniform_p(sd.wavelet)[k - 1]) : 0.f;
+    const float w2 = on2 ? uniform_f(uniform_p(sd.wavelet)[k]) : 0.f;
+    const bool p1 = on1 && stype == SRC_PRESSURE, p2 = on2 && stype == SRC_PRESSURE;
+    const bool fz1 = on1 && stype == SRC_FORCE_Z, fz2 = on2 && stype == SRC_FORCE_Z;
+    const bool f1 = fz1 || (on1 && stype == SRC_FORCE_X), f2 = fz2 || (on2 && stype == SRC_FORCE_X);
+    const int si = uniform(sd.si), sj = uniform(sd.sj);
+    const float src_scale = uniform_f(sd.src_scale);
+    const float dtx = uniform_f(sd.dtx);  // == dtz (derived coefficients need dtx == dtz)
+    float a[M];
+#pragma unroll
+    for (int l = 0; l < M; ++l) a[l] = uniform_f(sd.a[l]);
+    // ---- end of shot-table reads
+    const int jb = j0 - 3 * M;  // first velocity(k) row
+    const int jend = j1 + 3 * M;
+    // The source cell can change this tile's values only inside the columns
+    // its lanes and halo lanes hold and the rows it loads; other tiles run
+    // the march without any source code.
+    const bool src_tile = (on1 || on2) && si >= ib0 - M && si < ib0 + LC + M && sj >= jb - M - PD &&
+                          sj < jend + L + M + PD;
+    // byte offsets: per-tile constants + row * pitch (rows stay inside the
+    // guard rows: the interior is at least nbc + 3M + 3 rows from an edge)
+    const uint32_t P4 = (uint32_t)P * 4u;
+    const uint32_t lvo = (uint32_t)(lo * 4);
+    const uint32_t hvo = has_halo ? (uint32_t)(hx * 4) : OOB;
+    const uint32_t fb = forg + (uint32_t)ib0 * 4u, cb = corg + (uint32_t)ib0 * 4u;
+    const uint32_t hb = forg + (uint32_t)(ib0 - M) * 4u;
+    auto foff = [&](int c, int r) { return fb + (uint32_t)c * fstr + (uint32_t)r * P4; };
+    auto coff = [&](int q, int r) { return cb + (uint32_t)q * cstr + (uint32_t)r * P4; };
+    auto fld = [&](int c, int r) { return bload<ST_F32, CPT>(rA, lvo, foff(c, r), 1.f); };
+    auto cof = [&](int q, int r) { return bload<ST_F32, CPT>(rC, lvo, coff(q, r), 1.f); };
+    auto hal = [&](int c, int r) {
+      return bload<ST_F32, 1>(rA, hvo, hb + (uint32_t)c * fstr + (uint32_t)r * P4, 1.f).v[0];
+    };
+    // the march's loads and stores: offset of row jv + these constants
+    const uint32_t o_hxx = hb + F_TXX * fstr + PD * P4, o_hxz = hb + F_TXZ * fstr + PD * P4;
+    const uint32_t o_txx = fb + F_TXX * fstr + PD * P4;
+    const uint32_t o_txz = fb + F_TXZ * fstr + (uint32_t)(M + PD - 1) * P4;
+    const uint32_t o_tzz = fb + F_TZZ * fstr + (uint32_t)(M + PD) * P4;
+    const uint32_t o_vx = fb + F_VX * fstr + PD * P4, o_vz = fb + F_VZ * fstr + PD * P4;
+    const uint32_t o_rho = cb + QRHO * cstr + (uint32_t)(1 + PD) * P4;
+    const uint32_t o_mu = cb + QMUX * cstr - (uint32_t)(M - 1 - PD) * P4;
+    const uint32_t o_lam = cb + QLAM * cstr - (uint32_t)(M - PD) * P4;
+    const uint32_t w_vx = fb + F_VX * fstr - (uint32_t)(2 * M) * P4;
+    const uint32_t w_vz = fb + F_VZ * fstr - (uint32_t)(2 * M) * P4;
+    const uint32_t w_xx = fb + F_TXX * fstr - (uint32_t)(3 * M) * P4;
+    const uint32_t w_zz = fb + F_TZZ * fstr - (uint32_t)(3 * M) * P4;
+    const uint32_t w_xz = fb + F_TXZ * fstr - (uint32_t)(3 * M) * P4;
+    const bool own_lane = lo >= HL && lo < LC - HL;
+
+    auto march = [&](auto src_tag) {
+      constexpr bool SRC = decltype(src_tag)::value;
+      auto add_src1 = [&](V<CPT>& x, int r) {
+        if (SRC && p1 && r == sj) {
+#pragma unroll
+          for (int c = 0; c < CPT; ++c)
+            if (ib + c == si) x.v[c] = x.v[c] + w1;
+        }
+      };
+      V<CPT> rtxx[L], rtxz[L], rtzz[L];  // old (step k-1) stress
+      V<CPT> cvx[L], cvz[L];             // old velocity (slot of row jv)
+      float htxx[L], htxz[L];            // old stress of the halo columns
+      V<CPT> crho[L], cmux[L], clam[L];  // rho_vx rows, mu_txx rows, lam (slot jv: row jv - M)
+      V<CPT> imu;                        // 1 / mu_txx of the step-k stress row
+      V<CPT> qvx[L], qvz[L];             // step-k velocity
+      V<CPT> sxx[L], szz[L], sxz[L];     // step-k stress (+ the step-(k+1) pressure source)
+      V<CPT> wvx[L], wvz[L];             // step-(k+1) velocity
+      V<CPT> b1r[L], b3r[L];             // dtx / rho_vx, dtx / rho_vz of row jv
+      V<CPT> l2r[L], mtr[L];             // lam + 2 mu, mu_txz of row jv - M (slot jv)
+      // rows read before the first iteration (as k_fused, from jb)
+#pragma unroll
+      for (int q = 0; q < 2 * M + PD; ++q) {
+        const int r = jb - M + q, sl = (q - M + L) % L;
+        if (q < 2 * M + PD - 1) rtxz[sl] = fld(F_TXZ, r);
+        rtzz[sl] = fld(F_TZZ, r);
+        if (q < 2 * M) add_src1(rtzz[sl], r);
+        if (q < M + PD) {
+          rtxx[sl] = fld(F_TXX, r);
+          if (q < M) add_src1(rtxx[sl], r);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < PD; ++q) {
+        htxx[q] = hal(F_TXX, jb + q);
+        htxz[q] = hal(F_TXZ, jb + q);
+        cvx[q] = fld(F_VX, jb + q);
+        cvz[q] = fld(F_VZ, jb + q);
+        clam[q] = cof(QLAM, jb + q - M);
+      }
+#pragma unroll
+      for (int q = 0; q <= PD; ++q) {
+        crho[q] = cof(QRHO, jb + q);
+        cmux[(q - M + L) % L] = cof(QMUX, jb - M + q);
+      }
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) imu.v[c] = 1.0f / cmux[(L - M) % L].v[c];
+
+      for (int jv0 = jb; jv0 < jend; jv0 += L) {
+#pragma unroll
+        for (int u = 0; u < L; ++u) {
+          const int jv = jv0 + u;
+          const uint32_t rv = (uint32_t)jv * P4;
+          const int cu = u % L, nu = (u + PD) % L;
+          // ---- prefetch: the inputs of velocity(k) row jv+PD and stress(k) row jv+PD-M
+          htxx[nu] = bload<ST_F32, 1>(rA, hvo, o_hxx + rv, 1.f).v[0];
+          htxz[nu] = bload<ST_F32, 1>(rA, hvo, o_hxz + rv, 1.f).v[0];
+          rtxx[(u + PD) % L] = bload<ST_F32, CPT>(rA, lvo, o_txx + rv, 1.f);
+          rtxz[(u + M + PD - 1) % L] = bload<ST_F32, CPT>(rA, lvo, o_txz + rv, 1.f);
+          rtzz[(u + M + PD) % L] = bload<ST_F32, CPT>(rA, lvo, o_tzz + rv, 1.f);
+          cvx[nu] = bload<ST_F32, CPT>(rA, lvo, o_vx + rv, 1.f);
+          cvz[nu] = bload<ST_F32, CPT>(rA, lvo, o_vz + rv, 1.f);
+          crho[(u + 1 + PD) % L] = bload<ST_F32, CPT>(rC, lvo, o_rho + rv, 1.f);
+          cmux[(u - M + 1 + PD + L) % L] = bload<ST_F32, CPT>(rC, lvo, o_mu + rv, 1.f);
+          clam[nu] = bload<ST_F32, CPT>(rC, lvo, o_lam + rv, 1.f);
+          add_src1(rtxx[u % L], jv);
+          add_src1(rtzz[(u + M) % L], jv + M);
+
+          // ---- velocity(k), row jv, all lane columns
+          {
+            const V<CPT>& xr = rtxx[u % L];
+            const V<CPT>& zr = rtxz[u % L];
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) {
+              s_txx[M + lo + c] = xr.v[c];
+              s_txz[M + lo + c] = zr.v[c];
+            }
+            if (has_halo) {
+              float hxx = htxx[cu];
+              if (SRC && p1 && jv == sj && hcol == si) hxx = hxx + w1;
+              s_txx[hx] = hxx;
+              s_txz[hx] = htxz[cu];
+            }
+            wave_sync();
+            const float nb0 = from_upper_lane(crho[u % L].v[0]);
+            const float nb1 = from_upper_lane(crho[(u + 1) % L].v[0]);
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) {
+              const int x = M + lo + c;
+              float d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f;
+#pragma unroll
+              for (int l = 1; l <= M; ++l) {
+                d1 = fmaf_(a[l - 1], s_txx[x + l - 1] - s_txx[x - l], d1);
+                d2 = fmaf_(a[l - 1], rtxz[(u + l - 1) % L].v[c] - rtxz[(u - l + L) % L].v[c], d2);
+                d3 = fmaf_(a[l - 1], s_txz[x + l] - s_txz[x - l + 1], d3);
+                d4 = fmaf_(a[l - 1], rtzz[(u + l) % L].v[c] - rtzz[(u - l + 1 + L) % L].v[c], d4);
+              }
+              float vxs = cvx[cu].v[c], vzs = cvz[cu].v[c];
+              if (SRC && f1 && jv == sj && ib + c == si) {
+                if (fz1) vzs = vzs + w1 * src_scale;
+                else vxs = vxs + w1 * src_scale;
+              }
+              const float q0 = crho[u % L].v[c], q1 = crho[(u + 1) % L].v[c];
+              const float q0n = c + 1 < CPT ? crho[u % L].v[min(c + 1, CPT - 1)] : nb0;
+              const float q1n = c + 1 < CPT ? crho[(u + 1) % L].v[min(c + 1, CPT - 1)] : nb1;
+              const float b1 = dtx / q0;
+              const float b3 = dtx / (0.25f * (((q0 + q0n) + q1) + q1n));
+              b1r[cu].v[c] = b1;
+              b3r[cu].v[c] = b3;
+              const float ix = fmaf_(b1, d2, b1 * d1);
+              const float iz = fmaf_(b3, d4, b3 * d3);
+              qvx[cu].v[c] = vxs + ix;
+              qvz[cu].v[c] = vzs + iz;
+            }
+          }
+
+          // ---- stress(k), row j = jv - M (coefficients on every row: imu is carried)
+          {
+            const int j = jv - M;
+            const int sm = (u - M + L) % L;  // slot of row j
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) {
+              const float m0 = cmux[(u - M + L) % L].v[c], m1 = cmux[(u - M + 1 + L) % L].v[c];
+              const float i1 = 1.0f / m1;
+              mtr[cu].v[c] = 2.0f / (imu.v[c] + i1);
+              l2r[cu].v[c] = clam[cu].v[c] + 2.0f * m0;
+              imu.v[c] = i1;
+            }
+            if (j >= j0 - 2 * M && j < j1 + 2 * M) {  // rows velocity(k+1) reads
+#pragma unroll
+              for (int c = 0; c < CPT; ++c) {
+                s_vx[M + lo + c] = qvx[sm].v[c];
+                s_vz[M + lo + c] = qvz[sm].v[c];
+              }
+              wave_sync();
+#pragma unroll
+              for (int c = 0; c < CPT; ++c) {
+                const int x = M + lo + c;
+                float d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f;
+#pragma unroll
+                for (int l = 1; l <= M; ++l) {
+                  d1 = fmaf_(a[l - 1], s_vx[x + l] - s_vx[x - l + 1], d1);
+                  d2 = fmaf_(a[l - 1], qvz[(u - M + l - 1 + L) % L].v[c] - qvz[(u - M - l + L) % L].v[c], d2);
+                  d3 = fmaf_(a[l - 1], qvx[(u - M + l + L) % L].v[c] - qvx[(u - M - l + 1 + L) % L].v[c], d3);
+                  d4 = fmaf_(a[l - 1], s_vz[x + l - 1] - s_vz[x - l], d4);
+                }
+                const float xx = rtxx[sm].v[c], zz = rtzz[sm].v[c], xz = rtxz[sm].v[c];
+                const float t1 = d1 * dtx, t2 = d2 * dtx;
+                const float lv = clam[cu].v[c], l2 = l2r[cu].v[c];
+                const float ixx = fmaf_(lv, t2, l2 * t1);
+                const float izz = fmaf_(l2, t2, lv * t1);
+                const float sxzv = fmaf_(d4, dtx, d3 * dtx);
+                const float ixz = mtr[cu].v[c] * sxzv;
+                float oxx = xx + ixx;
+                float ozz = zz + izz;
+                if (SRC && p2 && j == sj && ib + c == si) {  // step k+1 reads txx, tzz + its source
+                  oxx = oxx + w2;
+                  ozz = ozz + w2;
+                }
+                sxx[sm].v[c] = oxx;
+                szz[sm].v[c] = ozz;
+                sxz[sm].v[c] = xz + ixz;
+              }
+            }
+          }
+
+          // ---- velocity(k+1), row jw = jv - 2M
+          {
+            const int jw = jv - 2 * M;
+            const int cw = (u - 2 * M + 2 * L) % L;  // slot of row jw
+            V<CPT> ovx, ovz;
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) ovx.v[c] = ovz.v[c] = 0.f;
+            if (jw >= j0 - M && jw < j1 + M) {  // rows stress(k+1) reads
+#pragma unroll
+              for (int c = 0; c < CPT; ++c) {
+                s_txx[M + lo + c] = sxx[cw].v[c];
+                s_txz[M + lo + c] = sxz[cw].v[c];
+              }
+              wave_sync();
+#pragma unroll
+              for (int c = 0; c < CPT; ++c) {
+                const int x = M + lo + c;
+                float d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f;
+#pragma unroll
+                for (int l = 1; l <= M; ++l) {
+                  d1 = fmaf_(a[l - 1], s_txx[x + l - 1] - s_txx[x - l], d1);
+                  d2 = fmaf_(a[l - 1], sxz[(u - 2 * M + l - 1 + 2 * L) % L].v[c] -
+                                           sxz[(u - 2 * M - l + 2 * L) % L].v[c], d2);
+                  d3 = fmaf_(a[l - 1], s_txz[x + l] - s_txz[x - l + 1], d3);
+                  d4 = fmaf_(a[l - 1], szz[(u - 2 * M + l + 2 * L) % L].v[c] -
+                                           szz[(u - 2 * M - l + 1 + 2 * L) % L].v[c], d4);
+                }
+                float vxs = qvx[cw].v[c], vzs = qvz[cw].v[c];
+                if (SRC && f2 && jw == sj && ib + c == si) {
+                  if (fz2) vzs = vzs + w2 * src_scale;
+                  else vxs = vxs + w2 * src_scale;
+                }
+                const float b1 = b1r[cw].v[c], b3 = b3r[cw].v[c];
+                const float ix = fmaf_(b1, d2, b1 * d1);
+                const float iz = fmaf_(b3, d4, b3 * d3);
+                ovx.v[c] = vxs + ix;
+                ovz.v[c] = vzs + iz;
+              }
+              wvx[cw] = ovx;
+              wvz[cw] = ovz;
+            }
+            const uint32_t vo = (jw >= j0 && jw < j1 && own_lane && ib < xend) ? lvo : OOB;
+            bstore2<ST_F32>(rB, vo, w_vx + rv, ovx.v[0], ovx.v[1], 1.f);
+            bstore2<ST_F32>(rB, vo, w_vz + rv, ovz.v[0], ovz.v[1], 1.f);
+          }
+
+          // ---- stress(k+1), row j2 = jv - 3M
+          {
+            const int j2 = jv - 3 * M;
+            const int c2 = (u - 3 * M + 4 * L) % L;  // slot of row j2
+            const int cw = (u - 2 * M + 2 * L) % L;  // slot holding lam, lam + 2mu, mu_txz of row j2
+            V<CPT> oxx, ozz, oxz;
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) oxx.v[c] = ozz.v[c] = oxz.v[c] = 0.f;
+            if (j2 >= j0 && j2 < j1) {
+#pragma unroll
+              for (int c = 0; c < CPT; ++c) {
+                s_vx[M + lo + c] = wvx[c2].v[c];
+                s_vz[M + lo + c] = wvz[c2].v[c];
+              }
+              wave_sync();
+#pragma unroll
+              for (int c = 0; c < CPT; ++c) {
+                const int x = M + lo + c;
+                float d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f;
+#pragma unroll
+                for (int l = 1; l <= M; ++l) {
+                  d1 = fmaf_(a[l - 1], s_vx[x + l] - s_vx[x - l + 1], d1);
+                  d2 = fmaf_(a[l - 1], wvz[(u - 3 * M + l - 1 + 4 * L) % L].v[c] -
+                                           wvz[(u - 3 * M - l + 4 * L) % L].v[c], d2);
+                  d3 = fmaf_(a[l - 1], wvx[(u - 3 * M + l + 4 * L) % L].v[c] -
+                                           wvx[(u - 3 * M - l + 1 + 4 * L) % L].v[c], d3);
+                  d4 = fmaf_(a[l - 1], s_vz[x + l - 1] - s_vz[x - l], d4);
+                }
+                const float xx = sxx[c2].v[c], zz = szz[c2].v[c], xz = sxz[c2].v[c];
+                const float t1 = d1 * dtx, t2 = d2 * dtx;
+                const float lv = clam[cw].v[c], l2 = l2r[cw].v[c];
+                const float ixx = fmaf_(lv, t2, l2 * t1);
+                const float izz = fmaf_(l2, t2, lv * t1);
+                const float sxzv = fmaf_(d4, dtx, d3 * dtx);
+                const float ixz = mtr[cw].v[c] * sxzv;
+                oxx.v[c] = xx + ixx;
+                ozz.v[c] = zz + izz;
+                oxz.v[c] = xz + ixz;
+              }
+            }
+            const uint32_t vo = (j2 >= j0 && j2 < j1 && own_lane && ib < xend) ? lvo : OOB;
+            bstore2<ST_F32>(rB, vo, w_xx + rv, oxx.v[0], oxx.v[1], 1.f);
+            bstore2<ST_F32>(rB, vo, w_zz + rv, ozz.v[0], ozz.v[1], 1.f);
+            bstore2<ST_F32>(rB, vo, w_xz + rv, oxz.v[0], oxz.v[1], 1.f);
+          }
+        }
+      }
+    };
+    if (src_tile) march(BoolTag<true>{});
+    else march(BoolTag<false>{});
+  }
+}
+
 // Stress in the bands next to the HABC strips, after the velocity HABC:
 // new velocities from B (f[]), old stress from A (fo[]), output to B.  One
 // wavefront per (band, 64-column chunk, band_rows-row segment); the same

@@ -302,7 +302,17 @@ int32_t fomo_pool_next_step(const fomo_pool* p, const fomo_shot* s, int32_t* k_n
  *   context's stream while, on a second stream of the context, the fused pass
  *   over the tiles near the edges and then the HABC, band stress and free
  *   surface run; the two are joined before the next step.  Batches only
- *   (pools run mode 1).
+ *   (pools run mode 1);
+ * 3 = double: pairs of steps.  One pass over the interior tiles advances
+ *   them two steps (the first step's values stay on chip), while beside it
+ *   two split-step chains advance the rest: step k outside the interior
+ *   shrunk by 2M into a third state buffer, then step k + 1 outside the
+ *   interior.  FP32 storage with the derived coefficients of
+ *   fomo_model_create (dx == dz), fd_order 4, 8 or 12, nbc >= fd_order / 2,
+ *   and no receiver inside the shrunk interior; other runs, frame stores,
+ *   and an odd last step take mode 2.  The third buffer (5 fields) is
+ *   allocated with the first such run and counted in
+ *   fomo_wavefield_device_bytes.
  * All give identical results. */
 int32_t fomo_ctx_set_step_mode(fomo_ctx* ctx, int32_t mode);
 /* Split step shape (mode 2): rows per edge tile (1..256, default 16) and the
@@ -324,7 +334,9 @@ int32_t fomo_ctx_set_tiling(fomo_ctx* ctx, int32_t cols_per_lane, int32_t rows_p
  * 4 = free surface + record, 5 = fused velocity+stress pass, 6 = band
  * stress, 7 = HABC velocity, 8 = HABC stress (fused-step buffer roles),
  * 9 = fused pass over the split step's edge tiles, 10 = over its interior
- * tiles (mode 2's geometry). */
+ * tiles (mode 2's geometry); for a double-step run (mode 3): 11 = the
+ * two-step pass over the interior (steps k and k + 1), 12 = the step-k edge
+ * tiles, 13 = the step-(k+1) edge tiles. */
 int32_t fomo_launch_stage(fomo_model* m, fomo_wavefield* w, fomo_shot* s, int32_t stage,
                           int32_t k, int32_t reps);
 

@@ -97,6 +97,12 @@ def test_roofline_check_refuses_idle_kernels():
                             (0.5, 0.31, 0.30), (0.5, 0.0, 0.3)):
         with pytest.raises(SystemExit):
             bench.check_roofline(frac, kms, step)
+    # the double step's interior launch advances two steps: up to two step times
+    bench.check_roofline(0.5, 0.37, 0.193, 2)
+    with pytest.raises(SystemExit):
+        bench.check_roofline(0.5, 0.39, 0.193, 2)
+    with pytest.raises(SystemExit):
+        bench.check_roofline(0.5, 0.37, 0.193)
 
 
 def test_cpu_share_is_the_usable_cpus():

@@ -1,0 +1,177 @@
+"""Step mode 3 (double step) against the CPU oracle, bit for bit, through the
+C ABI: two time steps per pass over the interior (k_fused2), the split step's
+edge tiles and strip kernels for step k (outside the interior shrunk by 2M,
+into the third buffer) and step k + 1 (outside the interior) beside it.
+
+Every case also checks that the double step really ran (the third buffer was
+allocated) or, where it cannot run (a receiver inside the shrunk interior),
+that the split step took over."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from fomo_julia_amd import _lib
+from fomo_julia_amd import elastic2d as E
+from tests.helpers import Case
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+@pytest.fixture(scope="module")
+def b():
+    bk = E.backend("mi355x")
+    yield bk
+    bk.set_step_mode(1)
+    bk.set_split(16, 0)
+    bk.set_tiling(1, 0, 16)
+
+
+def _bytes(W):
+    return int(_lib.load().fomo_wavefield_device_bytes(W._h))
+
+
+def _state_equal(W, r, what):
+    for f in range(10):
+        got, want = W.download(f), r.field(f)
+        if not np.array_equal(got, want):
+            bad = np.argwhere(got != want)
+            i, j = bad[0]
+            raise AssertionError(f"{what} field {f}: {len(bad)} cells differ, first (i={i}, j={j}) "
+                                 f"got {got[i, j]!r} want {want[i, j]!r}")
+
+
+def _run(b, c, cpt=2, gsteps=16, edge_rows=16, bpc=1):
+    b.set_tiling(cpt, 0, gsteps)
+    b.set_step_mode(3)
+    b.set_split(edge_rows, bpc)
+    try:
+        W = E.Wavefield(c.M.nx, c.M.nz, b)
+        res = E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p)
+        b.synchronize()
+        fresh = E.Wavefield(c.M.nx, c.M.nz, b)  # the same model's wavefield, never run
+        fresh._bind(W._dm)
+        grew = _bytes(W) > _bytes(fresh)
+        fresh.close()
+    finally:
+        b.set_step_mode(1)
+        b.set_split(16, 0)
+        b.set_tiling(1, 0, 16)
+    return res, W, grew
+
+
+def _xz(c, i, j):
+    """Physical source coordinates of padded cell (i, j) (0-based)."""
+    return (F32((i - c.M.pad) * float(c.M.dx)), F32((j - c.M.pad) * float(c.M.dz)))
+
+
+# Case(nx_in=151, nz_in=97, nbc=14, fd_order=8): padded 187 x 133, band 19,
+# interior [30, 158) x [29, 104), shrunk interior [38, 150) x [37, 96).
+SRC_CELLS = {"frame": None, "deep": (94, 66), "interior_edge": (32, 60), "ring": (27, 60),
+             "shrunk_edge": (38, 37)}
+
+
+@pytest.mark.parametrize("where", list(SRC_CELLS))
+@pytest.mark.parametrize("src_type", ["pressure", "force_z", "force_x"])
+def test_double_step_sources_bit_exact(b, src_type, where):
+    base = Case(nx_in=151, nz_in=97, nbc=14, nt=75)
+    cell = SRC_CELLS[where]
+    c = Case(nx_in=151, nz_in=97, nbc=14, nt=75, src_type=src_type,
+             src_xz=None if cell is None else _xz(base, *cell))
+    if cell is not None:  # Source indices are 1-based, as in the reference
+        assert (int(c.src.i) - 1, int(c.src.j) - 1) == cell
+    print(f"case double src={src_type} at {where} ({int(c.src.i)},{int(c.src.j)}) nt={c.nt}", flush=True)
+    res, W, grew = _run(b, c)
+    assert grew, "the double step did not run"
+    r = c.oracle()
+    r.run(1, c.nt)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    _state_equal(W, r, "double")
+
+
+@pytest.mark.parametrize("fd_order", [4, 8, 12])
+@pytest.mark.parametrize("cpt,gsteps,edge_rows,bpc", [(1, 6, 7, 1), (2, 16, 16, 1), (2, 4, 16, 0),
+                                                      (4, 0, 3, 2)])
+def test_double_step_orders_and_shapes_bit_exact(b, fd_order, cpt, gsteps, edge_rows, bpc):
+    """Orders with k_fused2 instances, edge-tile shapes, graph chunks of an odd
+    number of double steps (6 steps), no graphs, even and odd nt."""
+    nt = 64 if gsteps == 16 else 71
+    c = Case(nx_in=160, nz_in=110, nbc=12, fd_order=fd_order, nt=nt, fs=fd_order != 12,
+             src_xz=None if fd_order == 4 else (F32(700.0), F32(500.0)))
+    print(f"case double order={fd_order} cpt={cpt} gsteps={gsteps} edge_rows={edge_rows} "
+          f"bpc={bpc} nt={nt}", flush=True)
+    res, W, grew = _run(b, c, cpt, gsteps, edge_rows, bpc)
+    assert grew, "the double step did not run"
+    r = c.oracle()
+    r.run(1, c.nt)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    _state_equal(W, r, "double")
+
+
+@pytest.mark.parametrize("rec_where", ["interior_ring", "shrunk_interior"])
+def test_double_step_receivers(b, rec_where):
+    """Receivers inside the interior but outside the shrunk interior are
+    recorded after the join (step k + 1) and from the third buffer (step k);
+    one inside the shrunk interior makes the run take split steps."""
+    c = Case(nx_in=151, nz_in=97, nbc=14, nt=70, rec_types=("vz", "vx", "p"))
+    j = 33 if rec_where == "interior_ring" else 60  # padded rows; shrunk interior rows [37, 96)
+    xr = np.linspace(0.0, float(c.M.x_max), 37).astype(F32)
+    zr = np.full(37, (j - c.M.pad) * float(c.M.dz), F32)
+    c.rec = E.setup_receivers(xr, zr, c.M, type=[("vz", "vx", "p")[k % 3] for k in range(37)])
+    assert int(c.rec.j[0]) - 1 == j  # 1-based, as in the reference
+    print(f"case double receivers {rec_where} (row {j})", flush=True)
+    res, W, grew = _run(b, c)
+    assert grew == (rec_where == "interior_ring")
+    r = c.oracle()
+    r.run(1, c.nt)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    _state_equal(W, r, "double receivers")
+
+
+def test_double_step_batch_in_pieces_bit_exact(b):
+    """A batch of uneven members with interiors, and one with a member too
+    small for a shrunk interior (the whole batch then takes split steps),
+    each run in pieces that end on a single and on a double step."""
+    lib = _lib.load()
+    groups = {
+        "interiors": [Case(nx_in=150, nz_in=100, nt=90, seed=1, nbc=10),
+                      Case(nx_in=200, nz_in=120, nt=90, seed=2, fs=False, nbc=10,
+                           src_type="force_z", src_xz=(F32(900.0), F32(600.0))),
+                      Case(nx_in=170, nz_in=140, nt=90, seed=4, nbc=10, src_type="force_x")],
+        "one_small": [Case(nx_in=150, nz_in=100, nt=90, seed=1, nbc=10),
+                      Case(nx_in=40, nz_in=40, nt=90, seed=3, nbc=10)],
+    }
+    for name, cases in groups.items():
+        print(f"case double batch {name}: {len(cases)} members, runs 51 + 39", flush=True)
+        b.set_tiling(2, 0, 8)
+        b.set_step_mode(3)
+        b.set_split(16, 1)
+        hs = []
+        try:
+            for c in cases:
+                dm = E._model(b, c.M, c.H, c.a, c.p)
+                W = E.Wavefield(c.M.nx, c.M.nz, b)
+                W._bind(dm)
+                ds = E._DeviceShot(dm, c.src, c.rec, c.nt)
+                hs.append((dm, W, ds, _bytes(W)))
+            n = len(hs)
+            arr = lambda xs: (ctypes.c_void_p * n)(*[x.value for x in xs])
+            bh = ctypes.c_void_p()
+            b.check(lib.fomo_batch_create(b.ctx, n, arr([h[0].h for h in hs]), arr([h[1]._h for h in hs]),
+                                          arr([h[2].h for h in hs]), ctypes.byref(bh)), "batch_create")
+            b.check(lib.fomo_batch_run(bh, 1, 51), "batch_run")
+            b.check(lib.fomo_batch_run(bh, 52, 39), "batch_run")
+            b.synchronize()
+            lib.fomo_batch_destroy(bh)
+        finally:
+            b.set_step_mode(1)
+            b.set_split(16, 0)
+            b.set_tiling(1, 0, 16)
+        grew = [_bytes(h[1]) > h[3] for h in hs]
+        assert all(grew) if name == "interiors" else not any(grew)
+        for c, (dm, W, ds, _) in zip(cases, hs):
+            r = c.oracle()
+            r.run(1, c.nt)
+            assert np.array_equal(ds.gather(), r.gather_nt_by_nrec())
+            _state_equal(W, r, f"double batch {name}")
