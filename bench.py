@@ -263,9 +263,10 @@ def bench_batch(args) -> dict:
     wav = I.ricker_wavelet(F32(25.0), dt, nt)
     b = E.backend("mi355x")
     split = args.split not in (None, "", "off")
+    double = split and args.double and args.storage == "f32"
     if split:
         er, bpc = (int(v) for v in args.split.split(","))
-        b.set_step_mode(2)
+        b.set_step_mode(3 if double else 2)
         b.set_split(er, bpc)
     W = E.Wavefield(M.nx, M.nz, b)
     dm = E.bind(b, W, M, H, a, p)
@@ -311,7 +312,8 @@ def bench_batch(args) -> dict:
                                    f"fd_order=12, nbc=40, nt={nt}, 25 Hz Ricker, 1700 receivers, "
                                    "one persistent launch per kernel, captured steps",
                        "padded_cells_per_shot": M.nx * M.nz,
-                       "step_mode": ("split fused " + args.split) if split else "fused"},
+                       "step_mode": ("double, split edges " + args.split) if double else
+                                    ("split fused " + args.split) if split else "fused"},
             "mcell_updates_per_s": round(cells / gpu_s / 1e6, 1),
             "ms_per_step": round(gpu_s * 1e3 / nt, 4), "gpu_seconds": round(gpu_s, 3),
             "wall_seconds_with_gathers": round(wall, 3),

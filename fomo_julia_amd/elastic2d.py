@@ -97,7 +97,9 @@ class MI355XBackend:
 
     def set_step_mode(self, mode: int = 1) -> None:
         """1 = fused single-pass step on ping-pong buffers (default), 0 = two-pass in
-        place, 2 = fused with the interior tiles beside the strip kernels."""
+        place, 2 = fused with the interior tiles beside the strip kernels, 3 = pairs
+        of steps with the interior advanced two steps per pass (FP32; runs that
+        cannot take it use mode 2).  All give identical results."""
         self.check(self.lib.fomo_ctx_set_step_mode(self.ctx, int(mode)), "set_step_mode")
 
     def set_split(self, edge_rows: int = 16, interior_blocks_per_cu: int = 0) -> None:

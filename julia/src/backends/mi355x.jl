@@ -66,14 +66,33 @@ end
 # ------------------------------------------------------------------ context
 const FOMO_CTX = Ref{Ptr{Cvoid}}(C_NULL)
 
+# Step forms (fomo_ctx_set_step_mode): single shots take the double step (two
+# steps per pass over the interior, split steps where a run cannot), batches
+# the whole-grid fused step; both give the same results.
+const FOMO_STEP_FUSED = Int32(1)
+const FOMO_STEP_DOUBLE = Int32(3)
+
 function fomo_ctx()
     if FOMO_CTX[] == C_NULL
         h = Ref{Ptr{Cvoid}}(C_NULL)
         rc = ccall((:fomo_ctx_create, LIBFOMO), Int32, (Int32, Ref{Ptr{Cvoid}}), Int32(0), h)
         fomo_check(rc, "fomo_ctx_create")
         FOMO_CTX[] = h[]
+        # two columns per lane, automatic tile height, 16 steps per graph;
+        # split-step edge chains of 16-row tiles, one interior block per CU
+        rc = ccall((:fomo_ctx_set_tiling, LIBFOMO), Int32, (Ptr{Cvoid}, Int32, Int32, Int32),
+                   h[], Int32(2), Int32(0), Int32(16))
+        fomo_check(rc, "fomo_ctx_set_tiling")
+        rc = ccall((:fomo_ctx_set_split, LIBFOMO), Int32, (Ptr{Cvoid}, Int32, Int32), h[], Int32(16), Int32(1))
+        fomo_check(rc, "fomo_ctx_set_split")
+        fomo_step_mode!(FOMO_STEP_DOUBLE)
     end
     return FOMO_CTX[]
+end
+
+function fomo_step_mode!(mode::Int32)
+    rc = ccall((:fomo_ctx_set_step_mode, LIBFOMO), Int32, (Ptr{Cvoid}, Int32), FOMO_CTX[], mode)
+    fomo_check(rc, "fomo_ctx_set_step_mode")
 end
 
 function fomo_check(rc::Integer, what::AbstractString)
@@ -414,10 +433,15 @@ function run_shots!(backend::MI355XBackend, W::Wavefield{HipMatrix}, M::Medium, 
                 shs[t] = fomo_shot_new(d.model, srcs[t], rec_template, nt)
             end
             models = fill(d.model, n)
-            rc = ccall((:fomo_batch_create, LIBFOMO), Int32,
-                       (Ptr{Cvoid}, Int32, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
-                        Ref{Ptr{Cvoid}}),
-                       fomo_ctx(), Int32(n), models, wfs, shs, bh)
+            fomo_step_mode!(FOMO_STEP_FUSED)  # a batch's strips are too large a share to split
+            rc = try
+                ccall((:fomo_batch_create, LIBFOMO), Int32,
+                      (Ptr{Cvoid}, Int32, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
+                       Ref{Ptr{Cvoid}}),
+                      fomo_ctx(), Int32(n), models, wfs, shs, bh)
+            finally
+                fomo_step_mode!(FOMO_STEP_DOUBLE)
+            end
             fomo_check(rc, "fomo_batch_create")
             rc = ccall((:fomo_batch_run, LIBFOMO), Int32, (Ptr{Cvoid}, Int32, Int32),
                        bh[], Int32(1), Int32(nt))

@@ -41,7 +41,8 @@ def _f24_rne(v):
 
 
 @pytest.mark.parametrize("storage,mode", [("f32", 1), ("bf16", 1), ("f16", 1), ("f16c", 1),
-                                          ("f24", 1), ("f32", 2), ("bf16", 2), ("f24", 2)])
+                                          ("f24", 1), ("f32", 2), ("bf16", 2), ("f24", 2),
+                                          ("f32", 3)])
 def test_seam_random_state_bit_exact(storage, mode):
     import oracle as O
     dx = F32(12.5)
@@ -64,6 +65,8 @@ def test_seam_random_state_bit_exact(storage, mode):
     b = E.backend("mi355x")
     b.set_tiling(2, 0, 16)
     b.set_step_mode(mode)
+    if mode == 3:  # the benchmark's double step: one interior block per CU
+        b.set_split(16, 1)
     W = E.Wavefield(M.nx, M.nz, b)
     dm = E.bind(b, W, M, H, a, p)
     scales = np.ones(5, F32)
@@ -102,9 +105,16 @@ def test_seam_random_state_bit_exact(storage, mode):
         del v
     try:
         E.run_time_loop(b, W, M, H, a, src, rec, p)
+        b.synchronize()
     finally:
         b.set_tiling(1, 0, 16)
         b.set_step_mode(1)
+        b.set_split(16, 0)
+    if mode == 3:  # the double step ran: its third state buffer exists
+        fresh = E.Wavefield(M.nx, M.nz, b)
+        fresh._bind(dm)
+        assert b.lib.fomo_wavefield_device_bytes(W._h) > b.lib.fomo_wavefield_device_bytes(fresh._h)
+        fresh.close()
     r.run(1, NT)
     assert np.array_equal(rec.data, r.gather_nt_by_nrec())
     for f in range(5):
@@ -116,7 +126,7 @@ def test_seam_random_state_bit_exact(storage, mode):
                                  f"got {got[i, j]!r} want {want[i, j]!r}")
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_marmousi_batch_of_64_random_states_bit_exact(mode):
     """The batched workload at its full size: 64 Marmousi-shaped shots (1700 x
     350, nbc 40, order 12, padded 1792 x 442) in one batch, two columns per
@@ -124,7 +134,8 @@ def test_marmousi_batch_of_64_random_states_bit_exact(mode):
     source position; after 20 steps every shot's fields and gather equal its
     own oracle run.  Mode 1 is the whole-grid fused step, mode 2 the split
     step (interior tiles of all 64 shots beside their edge tiles and strip
-    kernels, 16 edge rows, one interior block per CU)."""
+    kernels, 16 edge rows, one interior block per CU), mode 3 the double
+    step (two steps per interior launch, order 12)."""
     import ctypes
     import oracle as O
     dx = F32(10.0)
