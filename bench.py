@@ -863,6 +863,7 @@ def main():
                      "algorithmic_bytes_per_launch": main_bytes * main_cells},
         "kernels_ms": {k: round(v, 5) for k, v in kernels.items()},
         "tiling": ({"cols_per_lane": tuning["tiling"][0], "rows_per_tile": tuning["tiling"][1],
+                    "steps_per_graph": b.tiling[2],
                     "autotuned": True, "trials_ms_per_step": tuning["trials"]} if tuning else
                    {"fixed": args.tiling or "library default"}),
         "step_mode": ("double, split edges " + args.split) if double else

@@ -1507,7 +1507,7 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
       }
       int slots = dbl_slots(b->M, c->cus);
       if (b->int_bpc > 0) slots = std::min(slots, b->int_bpc * WPB * std::max(c->cus, 1));
-      b->tzd = c->tz > 0 ? c->tz : auto_tz(tx, nzs, slots, b->M, fused_block(b->M), 6 * b->M);
+      b->tzd = c->tz > 0 ? c->tz : auto_tz(tx, nzs, slots, b->M, dbl_block(b->M), 6 * b->M);
       b->dbl = true;
     }
   }

@@ -1337,6 +1337,11 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
 // is 2M + 1 + PD <= L.
 // ---------------------------------------------------------------------------
 constexpr int dbl_halo(int M) { return (3 * M + 1) / 2 * 2; }  // per side, two columns per lane
+// Rows k_fused2 loads ahead of their first use, and its ring length / unroll.
+#ifndef DBL_PD
+#define DBL_PD 1
+#endif
+constexpr int dbl_block(int M) { return 2 * M + 2 * DBL_PD; }
 
 template <bool B>
 struct BoolTag {
@@ -1345,7 +1350,7 @@ struct BoolTag {
 
 template <int M, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
-  constexpr int CPT = 2, LC = 64 * CPT, LW = LC + 2 * M, PD = FUSED_PD, L = fused_block(M);
+  constexpr int CPT = 2, LC = 64 * CPT, LW = LC + 2 * M, PD = DBL_PD, L = dbl_block(M);
   constexpr int HL = dbl_halo(M);
   static_assert(2 * M + 1 + PD <= L, "ring too short");
   __shared__ float lds[WPB][4][LW];
