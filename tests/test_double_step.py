@@ -175,3 +175,68 @@ def test_double_step_batch_in_pieces_bit_exact(b):
             r.run(1, c.nt)
             assert np.array_equal(ds.gather(), r.gather_nt_by_nrec())
             _state_equal(W, r, f"double batch {name}")
+
+
+def test_double_step_then_stages_bit_exact(b):
+    """After a run that ends on a double step, the strip backup the per-stage
+    API reads comes from the third buffer: a single time_step and then the
+    per-stage kernels continue the shot bit for bit."""
+    c = Case(nx_in=151, nz_in=97, nbc=14, nt=44, src_type="force_x", src_xz=(F32(900.0), F32(500.0)))
+    print("case double then time_step and stages", flush=True)
+    b.set_tiling(2, 0, 16)
+    b.set_step_mode(3)
+    b.set_split(16, 1)
+    try:
+        W = E.Wavefield(c.M.nx, c.M.nz, b)
+        dm = E.bind(b, W, c.M, c.H, c.a, c.p)
+        ds = E._shot_for(dm, c.src, c.rec, c.nt)
+        b.check(b.lib.fomo_run_time_loop(dm.h, W._h, ds.h, 1, 40), "run 40")
+        r = c.oracle()
+        r.run(1, 40)
+        _state_equal(W, r, "after 40 double-step steps")  # old fields from the third buffer
+        E.time_step(b, W, c.M, c.H, c.a, c.src, c.rec, 41, c.p)
+        E.backup_boundary(b, W)
+        E.update_velocity(b, W)
+        E.apply_habc_velocity(b, W)
+        b.synchronize()
+    finally:
+        b.set_step_mode(1)
+        b.set_split(16, 0)
+        b.set_tiling(1, 0, 16)
+    import oracle as O
+    r.run(41, 1)
+    r.run(42, 1, O.STAGES["backup"] | O.STAGES["velocity"] | O.STAGES["habc_v"])
+    _state_equal(W, r, "time_step and stages after double steps")
+
+
+def test_double_step_frames_and_batched_run_shots(b):
+    """On-step frames in step mode 3 take split steps and equal the oracle's;
+    run_shots batched in mode 3 equals run_shots one shot at a time in mode 1."""
+    from fomo_julia_amd import video as V
+    from tests.test_frames import _check, _oracle_frames
+    names = ["p", "vz"]
+    c = Case(nx_in=151, nz_in=97, nbc=14, nt=42)
+    print("case double frames + batched run_shots", flush=True)
+    xs = [F32(500.0), F32(750.0), F32(1000.0)]
+    cfg = E.MultiShotConfig(xs, [F32(40.0)] * 3, c.src.wavelet)
+    b.set_tiling(2, 0, 16)
+    b.set_step_mode(3)
+    b.set_split(16, 1)
+    try:
+        rec = V.MultiFieldRecorder(c.M.nx, c.M.nz, c.dt, V.VideoConfig(fields=names, skip=6, downsample=2))
+        W = E.Wavefield(c.M.nx, c.M.nz, b)
+        res = E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p, on_step=rec)
+        Wb = E.Wavefield(c.M.nx, c.M.nz, b)
+        bat = E.run_shots(b, Wb, c.M, c.H, c.a, c.rec, cfg, c.p, batch=3)
+    finally:
+        b.set_step_mode(1)
+        b.set_split(16, 0)
+        b.set_tiling(1, 0, 16)
+    want, r = _oracle_frames(c, names, 6, 2, c.M.nx, c.M.nz)
+    _check(rec, want, names)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    Ws = E.Wavefield(c.M.nx, c.M.nz, b)
+    seq = E.run_shots(b, Ws, c.M, c.H, c.a, c.rec, cfg, c.p, batch=1)
+    for s, t in zip(seq, bat):
+        assert (s.src_i, s.src_j) == (t.src_i, t.src_j)
+        assert np.array_equal(s.gather, t.gather)
