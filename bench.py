@@ -767,14 +767,17 @@ def main():
             main_name = (f"k_fused over the split step's interior tiles, {form} ({main_bytes} "
                          f"B/cell-update x {main_cells} interior cells per launch)")
         if double:
-            # two steps per launch over the interior: one read of the 8 arrays
-            # and one write of the 5 fields per cell and launch
+            # two steps per launch over the interior, with the five stored
+            # coefficient arrays (the library's default form for k_fused2):
+            # one read of 5 fields + 5 coefficients and one write of the 5
+            # fields per cell and launch
             kernels["dbl_interior_2steps"] = main_ms = stage_ms(11, reps)
             kernels["dbl_edge_step_k"] = stage_ms(12, reps)
             kernels["dbl_edge_step_k1"] = stage_ms(13, reps)
+            main_bytes = BYTES_FUSED
             main_name = (f"k_fused2 over the double step's interior tiles (two time steps per "
-                         f"launch), {form} ({main_bytes} B per cell and launch x {main_cells} "
-                         "interior cells)")
+                         f"launch), stored coefficients ({main_bytes} B per cell and launch x "
+                         f"{main_cells} interior cells)")
     else:
         main_ms = stage_ms(1, reps)
         kernels = {"velocity": stage_ms(0, reps), "stress": main_ms,
@@ -782,7 +785,8 @@ def main():
                    "fs_record": stage_ms(4, reps)}
         main_bytes, main_name = BYTES_STRESS, "k_stress (44 B/cell-update x padded cells per launch)"
     main_gbs = main_bytes * main_cells / (main_ms / 1e3) / 1e9
-    step_bytes = main_bytes if fused else BYTES_VELOCITY + BYTES_STRESS
+    # the step as a whole on the single-step model of the medium's form
+    step_bytes = (BYTES_FUSED_DERIVED if double else main_bytes) if fused else BYTES_VELOCITY + BYTES_STRESS
     step_gbs = step_bytes * cells / (ms_per_step / 1e3) / 1e9
     check_roofline(main_gbs / HBM_PEAK_GBS, main_ms, ms_per_step, 2 if double else 1)
     traffic = None

@@ -177,6 +177,7 @@ struct fomo_batch {
   bool split = false, rec_in_frame = false;
   int tzi = 0, tze = 16, int_bpc = 0;
   hipEvent_t ev_edge = nullptr, ev_chain = nullptr;
+  hipEvent_t ev_dedge = nullptr;  // double step: the step-k edge tiles are done
   mutable hipStream_t cur = nullptr;  // set while the strip kernels are queued on the aux stream
   // frame store written after every step (one-shot batches); 0: none.  The
   // captured chunks hold the store's arguments, so they are keyed by its id.
@@ -1358,16 +1359,31 @@ static int auto_tz(const std::vector<int>& tiles_x, const std::vector<int>& nz, 
 
 // Orders the two-step interior kernel is built for.
 static bool dbl_order(int M) { return M == 2 || M == 4 || M == 6; }
+// The two-step kernel's coefficient form: read from the five stored arrays
+// (0, every FP32 model holds them) or formed from rho_vx, lam, mu_txx (1,
+// models that hold those too).  The stored form issues 17 % fewer
+// instructions (no divisions) for 8 more bytes per cell and two steps: its
+// launch takes 259 us against 309 us on the SEAM-shaped shot, and the double
+// step 0.182 against 0.190 ms per step.
+#ifndef DBL_DERIVED
+#define DBL_DERIVED 0
+#endif
+static bool dbl_cmp(const fomo_batch* b) { return b->compact && DBL_DERIVED; }
 
 // Resident wavefronts of k_fused2 on the whole device.
-template <int M>
+template <int M, bool CMP>
 static int dbl_slots_m(int cus) {
   int bv = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused2<M, WPB>, 64 * WPB, 0) != hipSuccess) bv = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k_fused2<M, WPB, CMP>, 64 * WPB, 0) != hipSuccess)
+    bv = 1;
   return std::max(1, bv) * WPB * std::max(cus, 1);
 }
-static int dbl_slots(int M, int cus) {
-  return M == 2 ? dbl_slots_m<2>(cus) : M == 4 ? dbl_slots_m<4>(cus) : dbl_slots_m<6>(cus);
+template <bool CMP>
+static int dbl_slots_c(int M, int cus) {
+  return M == 2 ? dbl_slots_m<2, CMP>(cus) : M == 4 ? dbl_slots_m<4, CMP>(cus) : dbl_slots_m<6, CMP>(cus);
+}
+static int dbl_slots(const fomo_batch* b, int cus) {
+  return dbl_cmp(b) ? dbl_slots_c<true>(b->M, cus) : dbl_slots_c<false>(b->M, cus);
 }
 
 // The third state buffer of w (the double step), laid out like the other two
@@ -1398,6 +1414,7 @@ static void destroy_batch(fomo_batch* b) {
   (void)hipFree(b->dprefix);
   if (b->ev_edge) (void)hipEventDestroy(b->ev_edge);
   if (b->ev_chain) (void)hipEventDestroy(b->ev_chain);
+  if (b->ev_dedge) (void)hipEventDestroy(b->ev_dedge);
   delete b;
 }
 
@@ -1482,7 +1499,7 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
   // interior that stays non-empty shrunk by 2M, with no receiver inside the
   // shrunk interior (its first step is recorded from the third buffer, which
   // holds that step outside it).  Otherwise the split step runs.
-  if (b->split && b->step_mode == 1 && c->step_mode == 3 && b->storage == FOMO_STORAGE_F32 && b->compact &&
+  if (b->split && b->step_mode == 1 && c->step_mode == 3 && b->storage == FOMO_STORAGE_F32 && b->same &&
       dbl_order(b->M)) {
     bool ok = true;
     std::vector<int> tx, nzs;
@@ -1505,7 +1522,7 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
         int rc = ensure_buf2(ws[i]);
         if (rc) return rc;
       }
-      int slots = dbl_slots(b->M, c->cus);
+      int slots = dbl_slots(b.get(), c->cus);
       if (b->int_bpc > 0) slots = std::min(slots, b->int_bpc * WPB * std::max(c->cus, 1));
       b->tzd = c->tz > 0 ? c->tz : auto_tz(tx, nzs, slots, b->M, dbl_block(b->M), 6 * b->M);
       b->dbl = true;
@@ -1531,6 +1548,7 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
     }
     HIP_TRY(c, hipEventCreateWithFlags(&b->ev_edge, hipEventDisableTiming));
     HIP_TRY(c, hipEventCreateWithFlags(&b->ev_chain, hipEventDisableTiming));
+    HIP_TRY(c, hipEventCreateWithFlags(&b->ev_dedge, hipEventDisableTiming));
   }
   b->hprefix.assign((size_t)PART_COUNT * (n + 1), 0);
   for (int p = 0; p < PART_COUNT; ++p) {
@@ -1708,10 +1726,18 @@ static void launch_dbl_interior(const fomo_batch* b, int table, int flags, int k
   const dim3 grid(round_up(nb, NUM_XCD)), block(64 * WPB);
   const BatchArgs a = args_for(b, table, PART_DBL_INT, k_off);
   hipStream_t st = lstream(b);
-  switch (b->M) {
-    case 2: hipLaunchKernelGGL((k_fused2<2, WPB>), grid, block, 0, st, a, flags); break;
-    case 4: hipLaunchKernelGGL((k_fused2<4, WPB>), grid, block, 0, st, a, flags); break;
-    default: hipLaunchKernelGGL((k_fused2<6, WPB>), grid, block, 0, st, a, flags); break;
+  if (dbl_cmp(b)) {
+    switch (b->M) {
+      case 2: hipLaunchKernelGGL((k_fused2<2, WPB, true>), grid, block, 0, st, a, flags); break;
+      case 4: hipLaunchKernelGGL((k_fused2<4, WPB, true>), grid, block, 0, st, a, flags); break;
+      default: hipLaunchKernelGGL((k_fused2<6, WPB, true>), grid, block, 0, st, a, flags); break;
+    }
+  } else {
+    switch (b->M) {
+      case 2: hipLaunchKernelGGL((k_fused2<2, WPB, false>), grid, block, 0, st, a, flags); break;
+      case 4: hipLaunchKernelGGL((k_fused2<4, WPB, false>), grid, block, 0, st, a, flags); break;
+      default: hipLaunchKernelGGL((k_fused2<6, WPB, false>), grid, block, 0, st, a, flags); break;
+    }
   }
 }
 
@@ -1815,7 +1841,12 @@ static void launch_step(const fomo_batch* b, int parity, int k_off) {
 // buffer 1 - parity (time_stepper.jl:39-63, twice).  On the aux stream, two
 // split-step chains: step k outside the interior shrunk by 2M into the third
 // buffer C, then step k + 1 outside the interior from C.  On the batch's
-// stream, beside them, the interior's two steps in one launch (k_fused2).
+// stream, beside them, the interior's two steps in one launch (k_fused2),
+// started once the step-k edge tiles are done: those latency-bound tiles
+// then run without the interior's waves on their SIMDs, and the rest of the
+// chains still fit beside the interior launch (0.182 ms per step against
+// 0.193 with both started together; waiting also for the velocity HABC, or
+// for the whole step-k chain, measured 0.196 and 0.202 ms on SEAM).
 // Why this is race-free:
 //   - A (buffer `parity`) is only read during the double step;
 //   - C is written by the first chain and read by the second, in stream
@@ -1836,6 +1867,7 @@ static void launch_double(const fomo_batch* b, int parity, int k_off) {
   (void)hipStreamWaitEvent(ax, b->ev_edge, 0);
   b->cur = ax;
   launch_stencil(b, ta, 6, FLAG_INJECT, k_off);
+  (void)hipEventRecord(b->ev_dedge, ax);
   launch_habc(b, ta, false, k_off);
   launch_stencil(b, ta, 3, FLAG_INJECT, k_off);
   launch_habc(b, ta, true, k_off);
@@ -1847,6 +1879,7 @@ static void launch_double(const fomo_batch* b, int parity, int k_off) {
   launch_fsrec(b, tb, b->rec_in_frame ? (MODE_FS | MODE_RECORD) : MODE_FS, k_off + 1);
   (void)hipEventRecord(b->ev_chain, ax);
   b->cur = nullptr;
+  (void)hipStreamWaitEvent(st, b->ev_dedge, 0);
   launch_dbl_interior(b, ti, FLAG_INJECT, k_off);
   (void)hipStreamWaitEvent(st, b->ev_chain, 0);
   if (!b->rec_in_frame) launch_fsrec(b, tb, MODE_RECORD, k_off + 1);

@@ -1348,7 +1348,11 @@ struct BoolTag {
   static constexpr bool value = B;
 };
 
-template <int M, int WPB>
+// CMP: coefficients formed from rho_vx, lam, mu_txx (k_fused's derived form);
+// otherwise read from the five stored arrays (dt/rho_vx, dt/rho_vz, lam,
+// lam + 2mu, mu_txz), the same values (the model holds both only when they
+// agree bit for bit), with no divisions.
+template <int M, int WPB, bool CMP>
 __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
   constexpr int CPT = 2, LC = 64 * CPT, LW = LC + 2 * M, PD = DBL_PD, L = dbl_block(M);
   constexpr int HL = dbl_halo(M);
@@ -1386,7 +1390,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
     const Rsrc rC = make_rsrc(uniform_p(sd.bufC), (uint32_t)uniform((int)sd.recC));
     const uint32_t fstr = (uint32_t)uniform((int)sd.fstr_b), forg = (uint32_t)uniform((int)sd.forg_b);
     const uint32_t cstr = (uint32_t)uniform((int)sd.cstr_b), corg = (uint32_t)uniform((int)sd.corg_b);
-    constexpr int QLAM = 2, QRHO = 5, QMUX = 6;
+    constexpr int QB1 = 0, QB3 = 1, QLAM = 2, QL2M = 3, QMU = 4, QRHO = 5, QMUX = 6;
     const int nw = uniform(sd.nw), stype = uniform(sd.stype);
     const bool inj = (flags & FLAG_INJECT) != 0;
     const bool on1 = inj && k <= nw, on2 = inj && k + 1 <= nw;
@@ -1432,6 +1436,9 @@ __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
     const uint32_t o_rho = cb + QRHO * cstr + (uint32_t)(1 + PD) * P4;
     const uint32_t o_mu = cb + QMUX * cstr - (uint32_t)(M - 1 - PD) * P4;
     const uint32_t o_lam = cb + QLAM * cstr - (uint32_t)(M - PD) * P4;
+    const uint32_t o_b1 = cb + QB1 * cstr + PD * P4, o_b3 = cb + QB3 * cstr + PD * P4;
+    const uint32_t o_l2m = cb + QL2M * cstr - (uint32_t)(M - PD) * P4;
+    const uint32_t o_mt = cb + QMU * cstr - (uint32_t)(M - PD) * P4;
     const uint32_t w_vx = fb + F_VX * fstr - (uint32_t)(2 * M) * P4;
     const uint32_t w_vz = fb + F_VZ * fstr - (uint32_t)(2 * M) * P4;
     const uint32_t w_xx = fb + F_TXX * fstr - (uint32_t)(3 * M) * P4;
@@ -1458,6 +1465,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
       V<CPT> wvx[L], wvz[L];             // step-(k+1) velocity
       V<CPT> b1r[L], b3r[L];             // dtx / rho_vx, dtx / rho_vz of row jv
       V<CPT> l2r[L], mtr[L];             // lam + 2 mu, mu_txz of row jv - M (slot jv)
+      // (formed at step k with CMP, loaded with the stored form)
       // rows read before the first iteration (as k_fused, from jb)
 #pragma unroll
       for (int q = 0; q < 2 * M + PD; ++q) {
@@ -1477,14 +1485,22 @@ __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
         cvx[q] = fld(F_VX, jb + q);
         cvz[q] = fld(F_VZ, jb + q);
         clam[q] = cof(QLAM, jb + q - M);
+        if constexpr (!CMP) {
+          b1r[q] = cof(QB1, jb + q);
+          b3r[q] = cof(QB3, jb + q);
+          l2r[q] = cof(QL2M, jb + q - M);
+          mtr[q] = cof(QMU, jb + q - M);
+        }
       }
+      if constexpr (CMP) {
 #pragma unroll
-      for (int q = 0; q <= PD; ++q) {
-        crho[q] = cof(QRHO, jb + q);
-        cmux[(q - M + L) % L] = cof(QMUX, jb - M + q);
+        for (int q = 0; q <= PD; ++q) {
+          crho[q] = cof(QRHO, jb + q);
+          cmux[(q - M + L) % L] = cof(QMUX, jb - M + q);
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) imu.v[c] = 1.0f / cmux[(L - M) % L].v[c];
       }
-#pragma unroll
-      for (int c = 0; c < CPT; ++c) imu.v[c] = 1.0f / cmux[(L - M) % L].v[c];
 
       for (int jv0 = jb; jv0 < jend; jv0 += L) {
 #pragma unroll
@@ -1500,8 +1516,15 @@ __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
           rtzz[(u + M + PD) % L] = bload<ST_F32, CPT>(rA, lvo, o_tzz + rv, 1.f);
           cvx[nu] = bload<ST_F32, CPT>(rA, lvo, o_vx + rv, 1.f);
           cvz[nu] = bload<ST_F32, CPT>(rA, lvo, o_vz + rv, 1.f);
-          crho[(u + 1 + PD) % L] = bload<ST_F32, CPT>(rC, lvo, o_rho + rv, 1.f);
-          cmux[(u - M + 1 + PD + L) % L] = bload<ST_F32, CPT>(rC, lvo, o_mu + rv, 1.f);
+          if constexpr (CMP) {
+            crho[(u + 1 + PD) % L] = bload<ST_F32, CPT>(rC, lvo, o_rho + rv, 1.f);
+            cmux[(u - M + 1 + PD + L) % L] = bload<ST_F32, CPT>(rC, lvo, o_mu + rv, 1.f);
+          } else {
+            b1r[nu] = bload<ST_F32, CPT>(rC, lvo, o_b1 + rv, 1.f);
+            b3r[nu] = bload<ST_F32, CPT>(rC, lvo, o_b3 + rv, 1.f);
+            l2r[nu] = bload<ST_F32, CPT>(rC, lvo, o_l2m + rv, 1.f);
+            mtr[nu] = bload<ST_F32, CPT>(rC, lvo, o_mt + rv, 1.f);
+          }
           clam[nu] = bload<ST_F32, CPT>(rC, lvo, o_lam + rv, 1.f);
           add_src1(rtxx[u % L], jv);
           add_src1(rtzz[(u + M) % L], jv + M);
@@ -1522,8 +1545,11 @@ __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
               s_txz[hx] = htxz[cu];
             }
             wave_sync();
-            const float nb0 = from_upper_lane(crho[u % L].v[0]);
-            const float nb1 = from_upper_lane(crho[(u + 1) % L].v[0]);
+            float nb0 = 0.f, nb1 = 0.f;  // CMP: rho right of this lane's last column, rows jv, jv+1
+            if constexpr (CMP) {
+              nb0 = from_upper_lane(crho[u % L].v[0]);
+              nb1 = from_upper_lane(crho[(u + 1) % L].v[0]);
+            }
 #pragma unroll
             for (int c = 0; c < CPT; ++c) {
               const int x = M + lo + c;
@@ -1540,13 +1566,19 @@ __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
                 if (fz1) vzs = vzs + w1 * src_scale;
                 else vxs = vxs + w1 * src_scale;
               }
-              const float q0 = crho[u % L].v[c], q1 = crho[(u + 1) % L].v[c];
-              const float q0n = c + 1 < CPT ? crho[u % L].v[min(c + 1, CPT - 1)] : nb0;
-              const float q1n = c + 1 < CPT ? crho[(u + 1) % L].v[min(c + 1, CPT - 1)] : nb1;
-              const float b1 = dtx / q0;
-              const float b3 = dtx / (0.25f * (((q0 + q0n) + q1) + q1n));
-              b1r[cu].v[c] = b1;
-              b3r[cu].v[c] = b3;
+              float b1, b3;
+              if constexpr (CMP) {
+                const float q0 = crho[u % L].v[c], q1 = crho[(u + 1) % L].v[c];
+                const float q0n = c + 1 < CPT ? crho[u % L].v[min(c + 1, CPT - 1)] : nb0;
+                const float q1n = c + 1 < CPT ? crho[(u + 1) % L].v[min(c + 1, CPT - 1)] : nb1;
+                b1 = dtx / q0;
+                b3 = dtx / (0.25f * (((q0 + q0n) + q1) + q1n));
+                b1r[cu].v[c] = b1;
+                b3r[cu].v[c] = b3;
+              } else {
+                b1 = b1r[cu].v[c];
+                b3 = b3r[cu].v[c];
+              }
               const float ix = fmaf_(b1, d2, b1 * d1);
               const float iz = fmaf_(b3, d4, b3 * d3);
               qvx[cu].v[c] = vxs + ix;
@@ -1558,13 +1590,15 @@ __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
           {
             const int j = jv - M;
             const int sm = (u - M + L) % L;  // slot of row j
+            if constexpr (CMP) {
 #pragma unroll
-            for (int c = 0; c < CPT; ++c) {
-              const float m0 = cmux[(u - M + L) % L].v[c], m1 = cmux[(u - M + 1 + L) % L].v[c];
-              const float i1 = 1.0f / m1;
-              mtr[cu].v[c] = 2.0f / (imu.v[c] + i1);
-              l2r[cu].v[c] = clam[cu].v[c] + 2.0f * m0;
-              imu.v[c] = i1;
+              for (int c = 0; c < CPT; ++c) {
+                const float m0 = cmux[(u - M + L) % L].v[c], m1 = cmux[(u - M + 1 + L) % L].v[c];
+                const float i1 = 1.0f / m1;
+                mtr[cu].v[c] = 2.0f / (imu.v[c] + i1);
+                l2r[cu].v[c] = clam[cu].v[c] + 2.0f * m0;
+                imu.v[c] = i1;
+              }
             }
             if (j >= j0 - 2 * M && j < j1 + 2 * M) {  // rows velocity(k+1) reads
 #pragma unroll

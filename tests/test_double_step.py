@@ -240,3 +240,20 @@ def test_double_step_frames_and_batched_run_shots(b):
     for s, t in zip(seq, bat):
         assert (s.src_i, s.src_j) == (t.src_i, t.src_j)
         assert np.array_equal(s.gather, t.gather)
+
+
+def test_double_step_stored_coefficient_model(b):
+    """A medium whose mu_txz is not init_medium's harmonic average in one
+    interior cell keeps only the five stored coefficient arrays; the double
+    step reads those (as it does for every FP32 medium) and stays bit-exact,
+    the perturbed cell included."""
+    from tests.test_gpu_parity import _perturbed_case
+    c = _perturbed_case(nx_in=151, nz_in=97, nbc=14, nt=50, src_type="force_z",
+                        src_xz=(F32(760.0), F32(480.0)))
+    print("case double step, stored-coefficient medium", flush=True)
+    res, W, grew = _run(b, c)
+    assert grew, "the double step did not run"
+    r = c.oracle()
+    r.run(1, c.nt)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    _state_equal(W, r, "double, stored coefficients")
