@@ -1357,8 +1357,11 @@ static int auto_tz(const std::vector<int>& tiles_x, const std::vector<int>& nz, 
   return best;
 }
 
-// Orders the two-step interior kernel is built for.
-static bool dbl_order(int M) { return M == 2 || M == 4 || M == 6; }
+// Orders the double step runs at.  At fd_order 12 k_fused2 takes 492 of a
+// SIMD lane's 512 registers, so no edge or strip kernel fits beside it and
+// the chains run after it: the 64-shot order-12 batch took 0.997 ms per step
+// that way against 0.790 whole-grid, and such runs take the split step.
+static bool dbl_order(int M) { return M == 2 || M == 4; }
 // The two-step kernel's coefficient form: read from the five stored arrays
 // (0, every FP32 model holds them) or formed from rho_vx, lam, mu_txx (1,
 // models that hold those too).  The stored form issues 17 % fewer
@@ -1380,7 +1383,7 @@ static int dbl_slots_m(int cus) {
 }
 template <bool CMP>
 static int dbl_slots_c(int M, int cus) {
-  return M == 2 ? dbl_slots_m<2, CMP>(cus) : M == 4 ? dbl_slots_m<4, CMP>(cus) : dbl_slots_m<6, CMP>(cus);
+  return M == 2 ? dbl_slots_m<2, CMP>(cus) : dbl_slots_m<4, CMP>(cus);
 }
 static int dbl_slots(const fomo_batch* b, int cus) {
   return dbl_cmp(b) ? dbl_slots_c<true>(b->M, cus) : dbl_slots_c<false>(b->M, cus);
@@ -1727,17 +1730,11 @@ static void launch_dbl_interior(const fomo_batch* b, int table, int flags, int k
   const BatchArgs a = args_for(b, table, PART_DBL_INT, k_off);
   hipStream_t st = lstream(b);
   if (dbl_cmp(b)) {
-    switch (b->M) {
-      case 2: hipLaunchKernelGGL((k_fused2<2, WPB, true>), grid, block, 0, st, a, flags); break;
-      case 4: hipLaunchKernelGGL((k_fused2<4, WPB, true>), grid, block, 0, st, a, flags); break;
-      default: hipLaunchKernelGGL((k_fused2<6, WPB, true>), grid, block, 0, st, a, flags); break;
-    }
+    if (b->M == 2) hipLaunchKernelGGL((k_fused2<2, WPB, true>), grid, block, 0, st, a, flags);
+    else hipLaunchKernelGGL((k_fused2<4, WPB, true>), grid, block, 0, st, a, flags);
   } else {
-    switch (b->M) {
-      case 2: hipLaunchKernelGGL((k_fused2<2, WPB, false>), grid, block, 0, st, a, flags); break;
-      case 4: hipLaunchKernelGGL((k_fused2<4, WPB, false>), grid, block, 0, st, a, flags); break;
-      default: hipLaunchKernelGGL((k_fused2<6, WPB, false>), grid, block, 0, st, a, flags); break;
-    }
+    if (b->M == 2) hipLaunchKernelGGL((k_fused2<2, WPB, false>), grid, block, 0, st, a, flags);
+    else hipLaunchKernelGGL((k_fused2<4, WPB, false>), grid, block, 0, st, a, flags);
   }
 }
 

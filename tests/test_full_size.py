@@ -134,8 +134,8 @@ def test_marmousi_batch_of_64_random_states_bit_exact(mode):
     source position; after 20 steps every shot's fields and gather equal its
     own oracle run.  Mode 1 is the whole-grid fused step, mode 2 the split
     step (interior tiles of all 64 shots beside their edge tiles and strip
-    kernels, 16 edge rows, one interior block per CU), mode 3 the double
-    step (two steps per interior launch, order 12)."""
+    kernels, 16 edge rows, one interior block per CU); mode 3 asks for the
+    double step, which order 12 does not take (the split step runs)."""
     import ctypes
     import oracle as O
     dx = F32(10.0)
