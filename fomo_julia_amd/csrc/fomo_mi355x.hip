@@ -1880,6 +1880,9 @@ static void launch_double(const fomo_batch* b, int parity, int k_off) {
   launch_dbl_interior(b, ti, FLAG_INJECT, k_off);
   (void)hipStreamWaitEvent(st, b->ev_chain, 0);
   if (!b->rec_in_frame) launch_fsrec(b, tb, MODE_RECORD, k_off + 1);
+  // frames: only step k + 1 is whole in memory; run_batch takes double steps
+  // with a frame store only when every frame step is the second of a pair
+  if (b->frames_id) launch_frame(b, b->fargs, tb, k_off + 1, -1);
 }
 
 static void launch_set_step(const fomo_batch* b, int k) {
@@ -1965,7 +1968,10 @@ static int run_batch(fomo_batch* b, int k_first, int n_steps) {
   launch_set_step(b, k_first);
   // Double steps where every member is live for every step of the run (a
   // double step needs both of its steps); an odd last step is a split step.
-  bool dbl = b->dbl && !b->frames_id && n_steps >= 2 && k_first >= 1;
+  // With a frame store, every frame step (a multiple of its even skip) must
+  // be the second step of a pair, which holds the whole state in memory.
+  bool dbl = b->dbl && n_steps >= 2 && k_first >= 1 &&
+             (!b->frames_id || (b->fargs.skip % 2 == 0 && k_first % 2 == 1));
   for (int i = 0; i < b->n && dbl; ++i)
     if ((int64_t)k_first + n_steps - 1 > b->ss[i]->nt) dbl = false;
   if (dbl) {
@@ -2022,10 +2028,12 @@ static void set_frames(fomo_batch* b, const fomo_frames* f) {
   for (int q = 0; q < 2; ++q) {
     if (b->gexec[q]) (void)hipGraphExecDestroy(b->gexec[q]);
     if (b->graph[q]) (void)hipGraphDestroy(b->graph[q]);
-    b->gexec[q] = nullptr;
-    b->graph[q] = nullptr;
+    if (b->gexec2[q]) (void)hipGraphExecDestroy(b->gexec2[q]);
+    if (b->graph2[q]) (void)hipGraphDestroy(b->graph2[q]);
+    b->gexec[q] = b->gexec2[q] = nullptr;
+    b->graph[q] = b->graph2[q] = nullptr;
   }
-  b->frames_id = id;  // double-step chunks hold no frame store (frames run split steps)
+  b->frames_id = id;
   b->fargs = f ? f->a : FrameArgs{};
 }
 

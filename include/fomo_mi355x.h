@@ -309,7 +309,9 @@ int32_t fomo_pool_next_step(const fomo_pool* p, const fomo_shot* s, int32_t* k_n
  *   shrunk by 2M into a third state buffer, then step k + 1 outside the
  *   interior.  FP32 storage with dtx == dtz, fd_order 4 or 8,
  *   nbc >= fd_order / 2, and no receiver inside the shrunk interior; other
- *   runs, frame stores, and an odd last step take mode 2.  The third buffer (5 fields) is
+ *   runs, runs with a frame store whose frame steps are not all second
+ *   steps of pairs (an odd skip, or an even first step), and an odd last
+ *   step take mode 2.  The third buffer (5 fields) is
  *   allocated with the first such run and counted in
  *   fomo_wavefield_device_bytes.
  * All give identical results. */

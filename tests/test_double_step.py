@@ -210,21 +210,25 @@ def test_double_step_then_stages_bit_exact(b):
     _state_equal(W, r, "time_step and stages after double steps")
 
 
-def test_double_step_frames_and_batched_run_shots(b):
-    """On-step frames in step mode 3 take split steps and equal the oracle's;
-    run_shots batched in mode 3 equals run_shots one shot at a time in mode 1."""
+@pytest.mark.parametrize("skip", [6, 5])
+def test_double_step_frames_and_batched_run_shots(b, skip):
+    """On-step frames in step mode 3: with an even skip every frame step is the
+    second step of a pair and the frames are taken inside the double steps;
+    with an odd skip the run takes split steps.  Both equal the oracle's
+    frames.  run_shots batched in mode 3 equals run_shots one shot at a time
+    in mode 1."""
     from fomo_julia_amd import video as V
     from tests.test_frames import _check, _oracle_frames
     names = ["p", "vz"]
     c = Case(nx_in=151, nz_in=97, nbc=14, nt=42)
-    print("case double frames + batched run_shots", flush=True)
+    print(f"case double frames (skip {skip}) + batched run_shots", flush=True)
     xs = [F32(500.0), F32(750.0), F32(1000.0)]
     cfg = E.MultiShotConfig(xs, [F32(40.0)] * 3, c.src.wavelet)
     b.set_tiling(2, 0, 16)
     b.set_step_mode(3)
     b.set_split(16, 1)
     try:
-        rec = V.MultiFieldRecorder(c.M.nx, c.M.nz, c.dt, V.VideoConfig(fields=names, skip=6, downsample=2))
+        rec = V.MultiFieldRecorder(c.M.nx, c.M.nz, c.dt, V.VideoConfig(fields=names, skip=skip, downsample=2))
         W = E.Wavefield(c.M.nx, c.M.nz, b)
         res = E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p, on_step=rec)
         Wb = E.Wavefield(c.M.nx, c.M.nz, b)
@@ -233,9 +237,14 @@ def test_double_step_frames_and_batched_run_shots(b):
         b.set_step_mode(1)
         b.set_split(16, 0)
         b.set_tiling(1, 0, 16)
-    want, r = _oracle_frames(c, names, 6, 2, c.M.nx, c.M.nz)
+    want, r = _oracle_frames(c, names, skip, 2, c.M.nx, c.M.nz)
+    done = (c.nt // skip) * skip  # the oracle ran up to the last frame step
+    if done < c.nt:
+        r.run(done + 1, c.nt - done)
+    assert rec.device_frames == c.nt // skip
     _check(rec, want, names)
     assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    _state_equal(W, r, f"frames skip {skip}")
     Ws = E.Wavefield(c.M.nx, c.M.nz, b)
     seq = E.run_shots(b, Ws, c.M, c.H, c.a, c.rec, cfg, c.p, batch=1)
     for s, t in zip(seq, bat):
