@@ -1034,13 +1034,20 @@ __global__ __launch_bounds__(64 * WPB) void k_fused(BatchArgs ba, int flags) {
     const int zlast = nz + GUARD_ROWS - 1;  // last row a load may touch (zero guard)
     // byte offsets: uniform part (block + row + tile column ib0), lane part
     const uint32_t lvo = (uint32_t)(lo * FES), lco = (uint32_t)(lo * CES);
-    const uint32_t hvo = has_halo ? (uint32_t)(hx * FES) : OOB;  // halo lanes read column ib0-M+hx
+    // Loads: the written columns [o0, min(o0 + ftw, xend)) need velocity
+    // within M of them and that velocity the previous state within M more,
+    // so lanes (and halo lanes) wholly right of the written columns + 2M load
+    // nothing (the last tile of a side strip or of a row reads only what it
+    // uses); their values are never written nor read by a written column.
+    const int cend = min(o0 + uniform(sd.ftw), xend) + 2 * M;
+    const uint32_t lvl = ib < cend ? lvo : OOB, lcl = ib < cend ? lco : OOB;
+    const uint32_t hvo = has_halo && hcol < cend ? (uint32_t)(hx * FES) : OOB;  // column ib0-M+hx
     auto frow = [&](int c, int r, int col) {
       return (uint32_t)c * fstr + forg + (uint32_t)((min(r, zlast) * P + col) * FES);
     };
-    auto fld = [&](int c, int r, auto sc) { return bload<KF, CPT>(rA, lvo, frow(c, r, ib0), sc); };
+    auto fld = [&](int c, int r, auto sc) { return bload<KF, CPT>(rA, lvl, frow(c, r, ib0), sc); };
     auto cof = [&](int q, int r, float sc) {
-      return bload<KC, CPT>(rC, lco, (uint32_t)q * cstr + corg + (uint32_t)((min(r, zlast) * P + ib0) * CES),
+      return bload<KC, CPT>(rC, lcl, (uint32_t)q * cstr + corg + (uint32_t)((min(r, zlast) * P + ib0) * CES),
                             sc);
     };
     auto hal = [&](int c, int r, auto sc) { return bload<KF, 1>(rA, hvo, frow(c, r, ib0 - M), sc).v[0]; };
@@ -1417,13 +1424,19 @@ __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
     // guard rows: the interior is at least nbc + 3M + 3 rows from an edge)
     const uint32_t P4 = (uint32_t)P * 4u;
     const uint32_t lvo = (uint32_t)(lo * 4);
-    const uint32_t hvo = has_halo ? (uint32_t)(hx * 4) : OOB;
+    // Loads: the written columns need the previous state within 4M of them
+    // (two steps of velocity and stress, M each), so lanes and halo lanes
+    // wholly right of the written columns + 4M (in the last tile of a row)
+    // load nothing; their values are never written nor read by a written column.
+    const int cend = min(o0 + uniform(sd.ftw), xend) + 4 * M;
+    const uint32_t lvl = ib < cend ? lvo : OOB;
+    const uint32_t hvo = has_halo && hcol < cend ? (uint32_t)(hx * 4) : OOB;
     const uint32_t fb = forg + (uint32_t)ib0 * 4u, cb = corg + (uint32_t)ib0 * 4u;
     const uint32_t hb = forg + (uint32_t)(ib0 - M) * 4u;
     auto foff = [&](int c, int r) { return fb + (uint32_t)c * fstr + (uint32_t)r * P4; };
     auto coff = [&](int q, int r) { return cb + (uint32_t)q * cstr + (uint32_t)r * P4; };
-    auto fld = [&](int c, int r) { return bload<ST_F32, CPT>(rA, lvo, foff(c, r), 1.f); };
-    auto cof = [&](int q, int r) { return bload<ST_F32, CPT>(rC, lvo, coff(q, r), 1.f); };
+    auto fld = [&](int c, int r) { return bload<ST_F32, CPT>(rA, lvl, foff(c, r), 1.f); };
+    auto cof = [&](int q, int r) { return bload<ST_F32, CPT>(rC, lvl, coff(q, r), 1.f); };
     auto hal = [&](int c, int r) {
       return bload<ST_F32, 1>(rA, hvo, hb + (uint32_t)c * fstr + (uint32_t)r * P4, 1.f).v[0];
     };
@@ -1511,21 +1524,21 @@ __global__ __launch_bounds__(64 * WPB) void k_fused2(BatchArgs ba, int flags) {
           // ---- prefetch: the inputs of velocity(k) row jv+PD and stress(k) row jv+PD-M
           htxx[nu] = bload<ST_F32, 1>(rA, hvo, o_hxx + rv, 1.f).v[0];
           htxz[nu] = bload<ST_F32, 1>(rA, hvo, o_hxz + rv, 1.f).v[0];
-          rtxx[(u + PD) % L] = bload<ST_F32, CPT>(rA, lvo, o_txx + rv, 1.f);
-          rtxz[(u + M + PD - 1) % L] = bload<ST_F32, CPT>(rA, lvo, o_txz + rv, 1.f);
-          rtzz[(u + M + PD) % L] = bload<ST_F32, CPT>(rA, lvo, o_tzz + rv, 1.f);
-          cvx[nu] = bload<ST_F32, CPT>(rA, lvo, o_vx + rv, 1.f);
-          cvz[nu] = bload<ST_F32, CPT>(rA, lvo, o_vz + rv, 1.f);
+          rtxx[(u + PD) % L] = bload<ST_F32, CPT>(rA, lvl, o_txx + rv, 1.f);
+          rtxz[(u + M + PD - 1) % L] = bload<ST_F32, CPT>(rA, lvl, o_txz + rv, 1.f);
+          rtzz[(u + M + PD) % L] = bload<ST_F32, CPT>(rA, lvl, o_tzz + rv, 1.f);
+          cvx[nu] = bload<ST_F32, CPT>(rA, lvl, o_vx + rv, 1.f);
+          cvz[nu] = bload<ST_F32, CPT>(rA, lvl, o_vz + rv, 1.f);
           if constexpr (CMP) {
-            crho[(u + 1 + PD) % L] = bload<ST_F32, CPT>(rC, lvo, o_rho + rv, 1.f);
-            cmux[(u - M + 1 + PD + L) % L] = bload<ST_F32, CPT>(rC, lvo, o_mu + rv, 1.f);
+            crho[(u + 1 + PD) % L] = bload<ST_F32, CPT>(rC, lvl, o_rho + rv, 1.f);
+            cmux[(u - M + 1 + PD + L) % L] = bload<ST_F32, CPT>(rC, lvl, o_mu + rv, 1.f);
           } else {
-            b1r[nu] = bload<ST_F32, CPT>(rC, lvo, o_b1 + rv, 1.f);
-            b3r[nu] = bload<ST_F32, CPT>(rC, lvo, o_b3 + rv, 1.f);
-            l2r[nu] = bload<ST_F32, CPT>(rC, lvo, o_l2m + rv, 1.f);
-            mtr[nu] = bload<ST_F32, CPT>(rC, lvo, o_mt + rv, 1.f);
+            b1r[nu] = bload<ST_F32, CPT>(rC, lvl, o_b1 + rv, 1.f);
+            b3r[nu] = bload<ST_F32, CPT>(rC, lvl, o_b3 + rv, 1.f);
+            l2r[nu] = bload<ST_F32, CPT>(rC, lvl, o_l2m + rv, 1.f);
+            mtr[nu] = bload<ST_F32, CPT>(rC, lvl, o_mt + rv, 1.f);
           }
-          clam[nu] = bload<ST_F32, CPT>(rC, lvo, o_lam + rv, 1.f);
+          clam[nu] = bload<ST_F32, CPT>(rC, lvl, o_lam + rv, 1.f);
           add_src1(rtxx[u % L], jv);
           add_src1(rtzz[(u + M) % L], jv + M);
 
