@@ -1368,6 +1368,9 @@ static bool dbl_order(int M) { return M == 2 || M == 4; }
 // instructions (no divisions) for 8 more bytes per cell and two steps: its
 // launch takes 259 us against 309 us on the SEAM-shaped shot, and the double
 // step 0.182 against 0.190 ms per step.
+// Height of the double step's step-(k+1) edge tiles in edge rows (SEAM-shaped
+// shot: 0.1829 ms per step with 1, 0.1815 with 2, 0.1781 with 3, 0.1822 with 4).
+constexpr int DBL_B_ROWS = 3;
 #ifndef DBL_DERIVED
 #define DBL_DERIVED 0
 #endif
@@ -1534,10 +1537,14 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
   b->hs.resize((size_t)TBL_COUNT * n);
   for (int t = 0; t < TBL_COUNT; ++t) {
     const bool dedge = t == TBL_DA0 || t == TBL_DA1 || t == TBL_DB0 || t == TBL_DB1;
+    // the double step's step-(k+1) edge tiles run beside the interior launch,
+    // where the bytes they move cost more than their latency: DBL_B_ROWS
+    // times taller tiles (fewer halo rows re-read)
+    const int tze = (t == TBL_DB0 || t == TBL_DB1) ? b->tze * DBL_B_ROWS : b->tze;
     for (int i = 0; i < n; ++i)
       b->hs[(size_t)t * n + i] = make_shotdev(ms[i], ws[i], ss[i], dedge ? b->ecpt : b->cpt, b->tz, t,
                                               n == 1 ? BAND_ROWS_SINGLE : BAND_ROWS_BATCH,
-                                              b->split ? b->tzi : b->tz, b->tze, b->tzd);
+                                              b->split ? b->tzi : b->tz, tze, b->tzd);
   }
   if (b->split) {
     b->rec_in_frame = true;
