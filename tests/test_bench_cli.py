@@ -36,6 +36,16 @@ def test_committed_traffic_is_near_the_algorithmic_bytes():
     assert 0.85 * bench.BYTES_FUSED_DERIVED < per_cell < 1.1 * bench.BYTES_FUSED_DERIVED
 
 
+def test_committed_double_step_traffic_is_near_its_algorithmic_bytes():
+    # the two-step interior launch reads 5 fields and 5 stored coefficient
+    # arrays and writes 5 fields once per interior cell (60 B); its tiles
+    # re-read halo columns and rows, but not a whole field
+    lines = [l for l in open(bench.TRAFFIC_FILE_DOUBLE) if l.startswith("traffic_bytes_per_launch:")]
+    assert len(lines) == 1
+    per_cell = int(lines[0].split(":")[1]) / bench.split_interior_cells(4552, 3927, 50, 4)
+    assert 60.0 < per_cell < 1.25 * 60.0
+
+
 @pytest.mark.parametrize("storage", ["f32", "bf16"])
 def test_batch_line_is_self_consistent(storage):
     d = json.load(open(os.path.join(ROOT, "profiles", "batch_marmousi64_bench.json")))[storage]
