@@ -179,6 +179,15 @@ struct fomo_batch {
   hipEvent_t ev_edge = nullptr, ev_chain = nullptr;
   hipEvent_t ev_dedge = nullptr;  // double step: the step-k edge tiles are done
   mutable hipStream_t cur = nullptr;  // set while the strip kernels are queued on the aux stream
+  // Two halves (whole-grid step, two or more members): members [0, nh) step
+  // on the batch's stream and [nh, n) on the context's aux stream.
+  bool alt = false;
+  int nh = 0;
+  mutable int half = -1;  // launches cover this half's members (-1: all)
+  int* dprefix_h[2] = {nullptr, nullptr};  // [PART_COUNT][members of the half + 1]
+  int* dn_h[2] = {nullptr, nullptr};
+  std::vector<int> hprefix_h[2];
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // frame store written after every step (one-shot batches); 0: none.  The
   // captured chunks hold the store's arguments, so they are keyed by its id.
   uint64_t frames_id = 0;
@@ -1368,6 +1377,11 @@ static bool dbl_order(int M) { return M == 2 || M == 4; }
 // instructions (no divisions) for 8 more bytes per cell and two steps: its
 // launch takes 259 us against 309 us on the SEAM-shaped shot, and the double
 // step 0.182 against 0.190 ms per step.
+// Batches of two or more shots in the whole-grid step run as two halves on
+// two streams (launch_alt_steps); 0 runs them on one stream.
+#ifndef FOMO_ALT
+#define FOMO_ALT 1
+#endif
 // Height of the double step's step-(k+1) edge tiles in edge rows (SEAM-shaped
 // shot: 0.1829 ms per step with 1, 0.1815 with 2, 0.1781 with 3, 0.1822 with 4).
 constexpr int DBL_B_ROWS = 3;
@@ -1421,6 +1435,12 @@ static void destroy_batch(fomo_batch* b) {
   if (b->ev_edge) (void)hipEventDestroy(b->ev_edge);
   if (b->ev_chain) (void)hipEventDestroy(b->ev_chain);
   if (b->ev_dedge) (void)hipEventDestroy(b->ev_dedge);
+  for (int h = 0; h < 2; ++h) {
+    (void)hipFree(b->dprefix_h[h]);
+    (void)hipFree(b->dn_h[h]);
+  }
+  for (hipEvent_t e : {b->ev_fork, b->ev_join})
+    if (e) (void)hipEventDestroy(e);
   delete b;
 }
 
@@ -1581,6 +1601,30 @@ static int build_batch(fomo_ctx* c, int n, fomo_model* const* ms, fomo_wavefield
   HIP_TRY(c, hipMalloc(&b->dprefix, b->hprefix.size() * sizeof(int)));
   HIP_TRY(c, hipMemcpy(b->dprefix, b->hprefix.data(), b->hprefix.size() * sizeof(int),
                        hipMemcpyHostToDevice));
+  if (FOMO_ALT && n >= 2 && !b->split && b->step_mode == 1) {
+    // halves of about equal fused work: half 0 ends at the first member
+    // whose fused tiles reach half of the batch's
+    const int* fp = b->hprefix.data() + (size_t)PART_FUSED * (n + 1);
+    int nh = 1;
+    while (nh < n - 1 && 2 * fp[nh] < fp[n]) ++nh;
+    b->nh = nh;
+    for (int h = 0; h < 2; ++h) {
+      const int lo = h ? nh : 0, hi = h ? n : nh, cnt = hi - lo;
+      std::vector<int>& pre = b->hprefix_h[h];
+      pre.assign((size_t)PART_COUNT * (cnt + 1), 0);
+      for (int q = 0; q < PART_COUNT; ++q) {
+        const int* full = b->hprefix.data() + (size_t)q * (n + 1);
+        for (int i = 0; i <= cnt; ++i) pre[(size_t)q * (cnt + 1) + i] = full[lo + i] - full[lo];
+      }
+      HIP_TRY(c, hipMalloc(&b->dprefix_h[h], pre.size() * sizeof(int)));
+      HIP_TRY(c, hipMemcpy(b->dprefix_h[h], pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+      HIP_TRY(c, hipMalloc(&b->dn_h[h], sizeof(int)));
+      HIP_TRY(c, hipMemcpy(b->dn_h[h], &cnt, sizeof(int), hipMemcpyHostToDevice));
+    }
+    for (hipEvent_t* e : {&b->ev_fork, &b->ev_join})
+      HIP_TRY(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
+    b->alt = true;
+  }
   *out = b.release();
   return FOMO_OK;
 }
@@ -1628,10 +1672,18 @@ static int single_batch(fomo_model* m, fomo_wavefield* w, fomo_shot* s, fomo_bat
 }
 
 // ------------------------------------------------------------------ launches
+// Work items of a part over the members launches cover (b->half).
+static int part_total(const fomo_batch* b, int part) {
+  if (b->half >= 0) {
+    const int cnt = b->half ? b->n - b->nh : b->nh;
+    return b->hprefix_h[b->half][(size_t)part * (cnt + 1) + cnt];
+  }
+  return b->hprefix[(size_t)part * (b->n + 1) + b->n];
+}
+
 static int blocks_for(const fomo_batch* b, int part, int wpb = WPB) {
   if (b->grid_fixed[part] > 0) return b->grid_fixed[part];
-  const int total = b->hprefix[(size_t)part * (b->n + 1) + b->n];
-  return std::max(1, (total + wpb - 1) / wpb);
+  return std::max(1, (part_total(b, part) + wpb - 1) / wpb);
 }
 
 static BatchArgs args_for(const fomo_batch* b, int table, int part, int k_off) {
@@ -1639,6 +1691,12 @@ static BatchArgs args_for(const fomo_batch* b, int table, int part, int k_off) {
   a.shots = b->ds + (size_t)table * b->n;
   a.prefix = b->dprefix + (size_t)part * (b->n + 1);
   a.n_shots_p = b->dn;
+  if (b->half >= 0) {
+    const int cnt = b->half ? b->n - b->nh : b->nh;
+    a.shots += b->half ? b->nh : 0;
+    a.prefix = b->dprefix_h[b->half] + (size_t)part * (cnt + 1);
+    a.n_shots_p = b->dn_h[b->half];
+  }
   a.k_off = k_off;
   return a;
 }
@@ -1747,7 +1805,7 @@ static void launch_dbl_interior(const fomo_batch* b, int table, int flags, int k
 
 static void launch_habc(const fomo_batch* b, int table, bool stress, int k_off) {
   const int part = stress ? PART_HABC_S : PART_HABC_V;
-  const int items = b->hprefix[(size_t)part * (b->n + 1) + b->n];
+  const int items = part_total(b, part);
   const BatchArgs a = args_for(b, table, part, k_off);
   const dim3 g(b->grid_fixed[part] > 0 ? b->grid_fixed[part] : std::max(1, items));
   const int rows = b->habc_rows;
@@ -1897,6 +1955,36 @@ static void launch_set_step(const fomo_batch* b, int k) {
                      b->dn, k);
 }
 // Reads the kp of rows [0, n) of table TBL_IN0 (filled for batches and pools).
+// `count` whole-grid steps of a two-stream batch from state buffer `parity`:
+// members [0, nh) on the batch's stream, [nh, n) on the context's aux stream,
+// each half's steps in stream order and the halves unordered (they share no
+// member).  A half's latency-bound strip kernels then overlap the other
+// half's bandwidth-bound fused pass part of the time (64 Marmousi-shaped
+// shots: 0.747 ms per step against 0.787 on one stream; ordering the fused
+// passes so that they never overlap measured 0.944: one half's fused pass
+// alone does not fill the GPU).  The aux stream joins at the end.
+static void launch_alt_steps(const fomo_batch* b, int parity, int count) {
+  hipStream_t st = lstream(b), ax = b->ctx->aux;
+  (void)hipEventRecord(b->ev_fork, st);
+  (void)hipStreamWaitEvent(ax, b->ev_fork, 0);
+  for (int g = 0; g < count; ++g) {
+    const int t = TBL_FUSED0 + ((parity + g) & 1);
+    for (int h = 0; h < 2; ++h) {
+      b->half = h;
+      b->cur = h ? ax : nullptr;
+      launch_stencil(b, t, 2, FLAG_INJECT, g);
+      launch_habc(b, t, false, g);
+      launch_stencil(b, t, 3, FLAG_INJECT, g);
+      launch_habc(b, t, true, g);
+      launch_fsrec(b, t, MODE_FS | MODE_RECORD, g);
+    }
+  }
+  b->half = -1;
+  b->cur = nullptr;
+  (void)hipEventRecord(b->ev_join, ax);
+  (void)hipStreamWaitEvent(st, b->ev_join, 0);
+}
+
 static void launch_advance(const fomo_batch* b, int by) {
   hipLaunchKernelGGL(k_advance, dim3((b->n + 255) / 256), dim3(256), 0, lstream(b), b->ds,
                      b->dn, by);
@@ -1915,6 +2003,8 @@ static int capture_chunk(fomo_batch* b, int p, bool dbl = false) {
   HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   if (dbl)
     for (int g = 0; g < G / 2; ++g) launch_double(b, (p + g) & 1, 2 * g);
+  else if (b->alt)
+    launch_alt_steps(b, p, G);
   else
     for (int g = 0; g < G; ++g) launch_step(b, (p + flip * g) & 1, g);
   launch_advance(b, G);
@@ -1958,10 +2048,17 @@ static int run_batch(fomo_batch* b, int k_first, int n_steps) {
   fomo_ctx* c = b->ctx;
   if (n_steps < 0) return fail(c, FOMO_ERR_ARG, "n_steps < 0");
   if (n_steps == 0) return FOMO_OK;
-  const int parity = b->ws[0]->cur;
-  for (fomo_wavefield* w : b->ws)
-    if (w->cur != parity)
+  // A member steps while k <= its nt.  Members that step in this run must
+  // hold their state in the same buffer; a member whose shot has ended is
+  // skipped by every kernel and may sit in either.
+  int parity = -1;
+  for (int i = 0; i < b->n; ++i) {
+    if (k_first > b->ss[i]->nt) continue;
+    if (parity < 0) parity = b->ws[i]->cur;
+    else if (b->ws[i]->cur != parity)
       return fail(c, FOMO_ERR_STATE, "batch wavefields are at different buffer parities");
+  }
+  if (parity < 0) parity = b->ws[0]->cur;
   if (b->step_mode == 0)
     for (fomo_wavefield* w : b->ws) {
       int rc = refresh_backup(w);
@@ -2015,11 +2112,19 @@ static int run_batch(fomo_batch* b, int k_first, int n_steps) {
     if (rc) return rc;
   }
   for (int i = 0; i < chunks; ++i) HIP_TRY(c, hipGraphLaunch(b->gexec[parity], c->stream));
-  for (int g = 0; g < rem; ++g) launch_step(b, (parity + flip * g) & 1, g);
+  if (b->alt && rem)
+    launch_alt_steps(b, parity, rem);
+  else
+    for (int g = 0; g < rem; ++g) launch_step(b, (parity + flip * g) & 1, g);
   if (rem) launch_advance(b, rem);
   if (flip) {
-    for (fomo_wavefield* w : b->ws) {
-      w->cur = (parity + n_steps) & 1;
+    for (int i = 0; i < b->n; ++i) {
+      // the state of a member whose shot ends inside the run is in the
+      // buffer of its own last step
+      const int64_t live = std::min<int64_t>(n_steps, std::max<int64_t>(0, (int64_t)b->ss[i]->nt - k_first + 1));
+      if (live == 0) continue;
+      fomo_wavefield* w = b->ws[i];
+      w->cur = (parity + (int)live) & 1;
       w->backup_is_prev = true;
       w->prev_buf2 = false;
     }

@@ -1806,7 +1806,12 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
     if (lane < M) hx = lane;
     else if (lane >= 64 - M) hx = 64 + M + (lane - (64 - M));
     const bool has_halo = hx >= 0;
-    const int hcol = c0 - M + hx;
+    // Loads: the written columns [c0, c1) need velocity up to column c1+M-1
+    // and their own old stress and coefficients; lanes right of that read
+    // the last needed column again (no further lines), and their values are
+    // never written nor read by a written column.
+    const int ibv = min(ib, c1 + M - 1), ibs = min(ib, c1 - 1);
+    const int hcol = min(c0 - M + hx, c1 + M - 1);
     const T* vx = (const T*)uniform_p(sd.f[F_VX]);
     const T* vz = (const T*)uniform_p(sd.f[F_VZ]);
     const T* atxx = (const T*)uniform_p(sd.fo[F_TXX]);
@@ -1839,26 +1844,26 @@ __global__ __launch_bounds__(64 * WPB) void k_band(BatchArgs ba, int flags) {
     float wvz[2 * M];  // vz rows j-M .. j+M-1
 #pragma unroll
     for (int q = 0; q < 2 * M; ++q) {
-      wvx[q] = ld1<KF>(rowt(vx, r0 - M + 1 + q, P) + ib, rvx);
-      wvz[q] = ld1<KF>(rowt(vz, r0 - M + q, P) + ib, rvz);
+      wvx[q] = ld1<KF>(rowt(vx, r0 - M + 1 + q, P) + ibv, rvx);
+      wvz[q] = ld1<KF>(rowt(vz, r0 - M + q, P) + ibv, rvz);
     }
     // row r0's pointwise operands; each iteration prefetches the next row's
     float hvx = has_halo ? ld1<KF>(rowt(vx, r0, P) + hcol, rvx) : 0.f;
     float hvz = has_halo ? ld1<KF>(rowt(vz, r0, P) + hcol, rvz) : 0.f;
-    float lv = ld1<KC>(rowt(lamp, r0, P) + ib, rl), l2 = ld1<KC>(rowt(l2mp, r0, P) + ib, rl2);
-    float mu = ld1<KC>(rowt(mup, r0, P) + ib, rmu);
-    float oxx = ld1<KF>(rowt(atxx, r0, P) + ib, rxx), ozz = ld1<KF>(rowt(atzz, r0, P) + ib, rzz);
-    float oxz = ld1<KF>(rowt(atxz, r0, P) + ib, rxz);
+    float lv = ld1<KC>(rowt(lamp, r0, P) + ibs, rl), l2 = ld1<KC>(rowt(l2mp, r0, P) + ibs, rl2);
+    float mu = ld1<KC>(rowt(mup, r0, P) + ibs, rmu);
+    float oxx = ld1<KF>(rowt(atxx, r0, P) + ibs, rxx), ozz = ld1<KF>(rowt(atzz, r0, P) + ibs, rzz);
+    float oxz = ld1<KF>(rowt(atxz, r0, P) + ibs, rxz);
     for (int j = r0; j < r1; ++j) {
       const int jn = j + 1;
       const float n_hvx = has_halo ? ld1<KF>(rowt(vx, jn, P) + hcol, rvx) : 0.f;
       const float n_hvz = has_halo ? ld1<KF>(rowt(vz, jn, P) + hcol, rvz) : 0.f;
-      const float n_lv = ld1<KC>(rowt(lamp, jn, P) + ib, rl), n_l2 = ld1<KC>(rowt(l2mp, jn, P) + ib, rl2);
-      const float n_mu = ld1<KC>(rowt(mup, jn, P) + ib, rmu);
-      const float n_oxx = ld1<KF>(rowt(atxx, jn, P) + ib, rxx), n_ozz = ld1<KF>(rowt(atzz, jn, P) + ib, rzz);
-      const float n_oxz = ld1<KF>(rowt(atxz, jn, P) + ib, rxz);
-      const float n_wvx = ld1<KF>(rowt(vx, j + M + 1, P) + ib, rvx);
-      const float n_wvz = ld1<KF>(rowt(vz, j + M, P) + ib, rvz);
+      const float n_lv = ld1<KC>(rowt(lamp, jn, P) + ibs, rl), n_l2 = ld1<KC>(rowt(l2mp, jn, P) + ibs, rl2);
+      const float n_mu = ld1<KC>(rowt(mup, jn, P) + ibs, rmu);
+      const float n_oxx = ld1<KF>(rowt(atxx, jn, P) + ibs, rxx), n_ozz = ld1<KF>(rowt(atzz, jn, P) + ibs, rzz);
+      const float n_oxz = ld1<KF>(rowt(atxz, jn, P) + ibs, rxz);
+      const float n_wvx = ld1<KF>(rowt(vx, j + M + 1, P) + ibv, rvx);
+      const float n_wvz = ld1<KF>(rowt(vz, j + M, P) + ibv, rvz);
       s_vx[M + lane] = wvx[M - 1];
       s_vz[M + lane] = wvz[M];
       if (has_halo) {

@@ -254,7 +254,14 @@ int32_t fomo_batch_create(fomo_ctx* ctx, int32_t n, fomo_model* const* models,
                           fomo_wavefield* const* wavefields, fomo_shot* const* shots,
                           fomo_batch** out);
 int32_t fomo_batch_destroy(fomo_batch* b);
-/* Steps k_first .. k_first+n_steps-1 for every shot of the batch. */
+/* Steps k_first .. k_first+n_steps-1 for every shot of the batch; a shot
+ * steps while k <= its nt and is skipped after that (its wavefield keeps the
+ * state of its own last step).  The shots that step in a run must be at the
+ * same buffer parity (FOMO_ERR_STATE otherwise): true for every batch whose
+ * runs stay within every shot's nt.  In the whole-grid step a batch of two
+ * or more shots runs as two halves on two streams (the context's stream and
+ * its second stream), joined before the call's work ends on the context's
+ * stream. */
 int32_t fomo_batch_run(fomo_batch* b, int32_t k_first, int32_t n_steps);
 
 /* ---- pools: continuous batching for a serving loop ----------------------

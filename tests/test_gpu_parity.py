@@ -283,6 +283,48 @@ def test_batched_uneven_grids_bit_exact(b, mode):
     b.set_split(16, 0)
 
 
+@pytest.mark.parametrize("nts", [(60, 35), (40, 40, 70)])
+def test_two_stream_batch_members_that_end_apart(b, nts):
+    """A whole-grid batch of two or more members steps as two halves on two
+    streams; a member whose shot ends first idles in its half while the
+    other half runs on, and every member still equals its own oracle run,
+    its state read from the buffer of its own last step (runs that go on
+    past a member's end leave the members at different buffer parities)."""
+    import ctypes
+    from fomo_julia_amd import _lib
+    lib = _lib.load()
+    cases = [Case(nx_in=70 + 20 * q, nz_in=48 + 6 * q, nt=nt, seed=q + 5) for q, nt in enumerate(nts)]
+    print(f"case two-stream batch, nt {nts}", flush=True)
+    b.set_tiling(2, 0, 8)
+    b.set_step_mode(1)
+    hs = []
+    for c in cases:
+        dm = E._model(b, c.M, c.H, c.a, c.p)
+        W = E.Wavefield(c.M.nx, c.M.nz, b)
+        W._bind(dm)
+        hs.append((dm, W, E._DeviceShot(dm, c.src, c.rec, c.nt)))
+    n = len(hs)
+    arr = lambda xs: (ctypes.c_void_p * n)(*[x.value for x in xs])
+    bh = ctypes.c_void_p()
+    b.check(lib.fomo_batch_create(b.ctx, n, arr([h[0].h for h in hs]), arr([h[1]._h for h in hs]),
+                                  arr([h[2].h for h in hs]), ctypes.byref(bh)), "batch_create")
+    try:
+        # two chunks of 8 and 5 steps; then runs in which members end at
+        # different buffer parities, and one in which only some still step
+        b.check(lib.fomo_batch_run(bh, 1, 21), "batch_run")
+        b.check(lib.fomo_batch_run(bh, 22, 20), "batch_run")
+        b.check(lib.fomo_batch_run(bh, 42, max(nts) - 41), "batch_run")
+        b.synchronize()
+    finally:
+        lib.fomo_batch_destroy(bh)
+        b.set_tiling(1, 0, 16)
+    for c, (dm, W, ds) in zip(cases, hs):
+        r = c.oracle()
+        r.run(1, c.nt)
+        assert np.array_equal(ds.gather(), r.gather_nt_by_nrec())
+        assert_same_state(W, r, f"two-stream batch member with nt {c.nt}")
+
+
 def test_run_shots_batched_equals_sequential(b):
     c = Case(nx_in=90, nz_in=50, nt=60)
     xs = [F32(100.0), F32(300.0), F32(550.0)]
