@@ -1961,8 +1961,9 @@ static void launch_set_step(const fomo_batch* b, int k) {
 // member).  A half's latency-bound strip kernels then overlap the other
 // half's bandwidth-bound fused pass part of the time (64 Marmousi-shaped
 // shots: 0.747 ms per step against 0.787 on one stream; ordering the fused
-// passes so that they never overlap measured 0.944: one half's fused pass
-// alone does not fill the GPU).  The aux stream joins at the end.
+// passes so that they never overlap measured 0.944, and 0.824 at best with
+// shorter tiles: a half's strip kernels are as long as the whole batch's,
+// being latency-bound).  The aux stream joins at the end.
 static void launch_alt_steps(const fomo_batch* b, int parity, int count) {
   hipStream_t st = lstream(b), ax = b->ctx->aux;
   (void)hipEventRecord(b->ev_fork, st);
