@@ -210,6 +210,41 @@ def test_double_step_then_stages_bit_exact(b):
     _state_equal(W, r, "time_step and stages after double steps")
 
 
+@pytest.mark.parametrize("skip,slots", [(4, 2), (6, 3), (3, 2)])
+def test_double_step_frames_ring_wraps(b, monkeypatch, skip, slots):
+    """A frame ring of a few slots splits the run into segments of
+    slots * skip steps; with an even skip every segment starts on an odd
+    step and takes double steps, its last segment ending on an even or odd
+    step; with an odd skip the segments take split steps.  Frames, gather
+    and final state equal the oracle's."""
+    from fomo_julia_amd import video as V
+    from tests.test_frames import _check, _oracle_frames
+    names = ["vx", "p"]
+    c = Case(nx_in=151, nz_in=97, nbc=14, nt=41)
+    rec = V.MultiFieldRecorder(c.M.nx, c.M.nz, c.dt, V.VideoConfig(fields=names, skip=skip, downsample=3))
+    per = len(names) * rec.nx_d * rec.nz_d * 4
+    monkeypatch.setattr(V, "RING_BYTES", slots * per)
+    print(f"case double frames ring skip={skip} slots={slots}", flush=True)
+    b.set_tiling(2, 0, 8)
+    b.set_step_mode(3)
+    b.set_split(16, 1)
+    try:
+        W = E.Wavefield(c.M.nx, c.M.nz, b)
+        res = E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p, on_step=rec)
+    finally:
+        b.set_step_mode(1)
+        b.set_split(16, 0)
+        b.set_tiling(1, 0, 16)
+    want, r = _oracle_frames(c, names, skip, 3, c.M.nx, c.M.nz)
+    done = (c.nt // skip) * skip
+    if done < c.nt:
+        r.run(done + 1, c.nt - done)
+    assert rec.device_frames == c.nt // skip
+    _check(rec, want, names)
+    assert np.array_equal(res.gather, r.gather_nt_by_nrec())
+    _state_equal(W, r, f"frames ring skip {skip}")
+
+
 @pytest.mark.parametrize("skip", [6, 5])
 def test_double_step_frames_and_batched_run_shots(b, skip):
     """On-step frames in step mode 3: with an even skip every frame step is the
