@@ -384,8 +384,10 @@ def bench_edges(args) -> dict:
     - on-step frames: the replayed loop writing a 4x-decimated pressure frame
       every 10 steps into a device ring (fomo_run_time_loop_frames), against
       the plain loop; HIP-event time per step on the library's stream, both
-      from step 1 of a reset shot, 2 columns per lane.  Draining the ring to
-      the host is not timed."""
+      from step 1 of a reset shot, 2 columns per lane, with the whole-grid
+      step and with the double step (whose frames are taken after the
+      second step of each pair).  Draining the ring to the host is not
+      timed."""
     import torch
     from fomo_julia_amd import elastic2d as E
     from fomo_julia_amd import init as I
@@ -417,47 +419,60 @@ def bench_edges(args) -> dict:
             best = t if best is None else min(best, t)
         setup[name] = round(best, 3)
     # the device-built model (the last one created) runs the shot, at the
-    # tiling the SEAM autotune picks (2 columns per lane, automatic rows)
-    b.set_tiling(2, 0, 16)
-    W = E.Wavefield(M.nx, M.nz, b)
-    dm = E.bind(b, W, M, H, a, p)
+    # tiling the SEAM autotune picks (2 columns per lane, automatic rows),
+    # with the whole-grid step and with the double step (split edges 16,1)
     wav = I.ricker_wavelet(F32(10.0), dt, steps)
-    src = E.Source(I.coord_to_index(F32(M.x_max / F32(2)), M.dx, M.pad),
-                   I.coord_to_index(F32(0.0), M.dz, M.pad), wav, "pressure")
     rec = E.setup_receivers(np.arange(vp.shape[0], dtype=F32) * dx, np.zeros(vp.shape[0], F32), M,
                             type="vz")
-    ds = E._shot_for(dm, src, rec, steps)
-    store = V._FrameStore(b, M.nx, M.nz, [V.FIELDS["p"]], skip, dsamp, steps // skip)
     stream = torch.cuda.ExternalStream(b.stream_handle())
 
-    def run(frames: bool, k0: int, n: int):
-        if frames:
-            b.check(lib.fomo_run_time_loop_frames(dm.h, W._h, ds.h, k0, n, store.h), "loop with frames")
-        else:
-            b.check(lib.fomo_run_time_loop(dm.h, W._h, ds.h, k0, n), "loop")
+    def frames_cost(step_mode: int) -> dict:
+        b.set_tiling(2, 0, 16)
+        b.set_step_mode(step_mode)
+        b.set_split(16, 1 if step_mode >= 2 else 0)
+        try:
+            W = E.Wavefield(M.nx, M.nz, b)
+            dm = E.bind(b, W, M, H, a, p)
+            src = E.Source(I.coord_to_index(F32(M.x_max / F32(2)), M.dx, M.pad),
+                           I.coord_to_index(F32(0.0), M.dz, M.pad), wav, "pressure")
+            ds = E._shot_for(dm, src, rec, steps)
+            store = V._FrameStore(b, M.nx, M.nz, [V.FIELDS["p"]], skip, dsamp, steps // skip)
 
-    per_step = {}
-    for frames in (False, True):
-        run(frames, 1, 64)  # warm-up: graph capture
-        E.reset(b, W)
-        b.check(lib.fomo_shot_reset(ds.h), "shot reset")
-        b.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        run(frames, 1, steps)
-        e1.record(stream)
-        b.synchronize()
-        per_step["frames" if frames else "plain"] = e0.elapsed_time(e1) / steps
-    frame = store.download(b, steps // skip - 1, 1)
+            def run(frames: bool, k0: int, n: int):
+                if frames:
+                    b.check(lib.fomo_run_time_loop_frames(dm.h, W._h, ds.h, k0, n, store.h), "loop with frames")
+                else:
+                    b.check(lib.fomo_run_time_loop(dm.h, W._h, ds.h, k0, n), "loop")
+
+            per_step = {}
+            for frames in (False, True):
+                run(frames, 1, 64)  # warm-up: graph capture
+                E.reset(b, W)
+                b.check(lib.fomo_shot_reset(ds.h), "shot reset")
+                b.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                run(frames, 1, steps)
+                e1.record(stream)
+                b.synchronize()
+                per_step["frames" if frames else "plain"] = e0.elapsed_time(e1) / steps
+            frame = store.download(b, steps // skip - 1, 1)
+            return {"field": "p", "skip": skip, "downsample": dsamp, "steps": steps,
+                    "step_mode": {1: "whole-grid", 3: "double, split edges 16,1"}[step_mode],
+                    "frame_shape": list(store.shape[1:]),
+                    "ms_per_step_plain": round(per_step["plain"], 4),
+                    "ms_per_step_with_frames": round(per_step["frames"], 4),
+                    "overhead": round(per_step["frames"] / per_step["plain"] - 1.0, 4),
+                    "last_frame_finite_nonzero": bool(np.isfinite(frame).all() and np.abs(frame).max() > 0)}
+        finally:
+            b.set_step_mode(1)
+            b.set_split(16, 0)
+
     return {"metric": "edges of the path, SEAM-shaped FP32 shot, 1 GPU",
             "setup_seconds": setup,
             "setup_speedup": round(setup["host"] / setup["device"], 2),
-            "frames": {"field": "p", "skip": skip, "downsample": dsamp, "steps": steps,
-                       "frame_shape": list(store.shape[1:]),
-                       "ms_per_step_plain": round(per_step["plain"], 4),
-                       "ms_per_step_with_frames": round(per_step["frames"], 4),
-                       "overhead": round(per_step["frames"] / per_step["plain"] - 1.0, 4),
-                       "last_frame_finite_nonzero": bool(np.isfinite(frame).all() and np.abs(frame).max() > 0)},
+            "frames": frames_cost(1),
+            "frames_double": frames_cost(3),
             "data": "synthetic (seeded SEAM-like medium)"}
 
 

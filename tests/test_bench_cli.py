@@ -69,10 +69,13 @@ def test_edges_line_is_self_consistent():
     d = json.load(open(os.path.join(ROOT, "profiles", "edges_seam_bench.json")))
     s = d["setup_seconds"]
     assert d["setup_speedup"] == pytest.approx(s["host"] / s["device"], rel=2e-2)
-    f = d["frames"]
-    assert f["overhead"] == pytest.approx(f["ms_per_step_with_frames"] / f["ms_per_step_plain"] - 1, abs=2e-3)
-    assert f["frame_shape"] == [-(-3927 // f["downsample"]), -(-4552 // f["downsample"])]
-    assert f["last_frame_finite_nonzero"]
+    for key in ("frames", "frames_double"):
+        f = d[key]
+        assert f["overhead"] == pytest.approx(f["ms_per_step_with_frames"] / f["ms_per_step_plain"] - 1, abs=2e-3)
+        assert f["frame_shape"] == [-(-3927 // f["downsample"]), -(-4552 // f["downsample"])]
+        assert f["last_frame_finite_nonzero"]
+    # the double step takes its frames inside double steps only for an even skip
+    assert d["frames_double"]["skip"] % 2 == 0
 
 
 def test_timed_kernels_run_at_a_step_inside_their_shot():
