@@ -279,13 +279,17 @@ def test_c1_accuracy_vs_float64(b, storage):
 
 
 @gpu
-@pytest.mark.parametrize("storage,mode", [("bf16", 1), ("f24", 1), ("f24", 2)])
-def test_reduced_batch_uneven_matches_single(b, storage, mode):
+@pytest.mark.parametrize("storage,mode,nts", [("bf16", 1, (50, 50, 50)), ("f24", 1, (50, 50, 50)),
+                                              ("f24", 2, (50, 50, 50)), ("bf16", 1, (50, 37, 44)),
+                                              ("f24", 1, (31, 50, 44))])
+def test_reduced_batch_uneven_matches_single(b, storage, mode, nts):
     """A batch of uneven grids equals each shot run alone (and, for F24, the
-    oracle's emulation), in the whole-grid and the split step."""
-    cases = [Case(nx_in=90, nz_in=60, nt=50, storage=storage, seed=1),
-             Case(nx_in=130, nz_in=45, nt=50, storage=storage, seed=2, src_type="force_z"),
-             Case(nx_in=200, nz_in=110, nt=50, storage=storage, seed=3, src_type="force_x")]
+    oracle's emulation), in the whole-grid and the split step; whole-grid
+    batches step as two halves on two streams, and shots that end before
+    the batch's run (nts) keep the state of their own last step."""
+    cases = [Case(nx_in=90, nz_in=60, nt=nts[0], storage=storage, seed=1),
+             Case(nx_in=130, nz_in=45, nt=nts[1], storage=storage, seed=2, src_type="force_z"),
+             Case(nx_in=200, nz_in=110, nt=nts[2], storage=storage, seed=3, src_type="force_x")]
     print(f"case uneven batch {storage} mode {mode}", flush=True)
     b.set_step_mode(mode)
     b.set_split(7, 1)
@@ -311,7 +315,7 @@ def test_reduced_batch_uneven_matches_single(b, storage, mode):
     b.check(lib.fomo_batch_create(b.ctx, len(cases), arr(models), arr(ws), arr(shots),
                                   ctypes.byref(hb)), "batch_create")
     try:
-        b.check(lib.fomo_batch_run(hb, 1, cases[0].nt), "batch_run")
+        b.check(lib.fomo_batch_run(hb, 1, max(nts)), "batch_run")
         b.synchronize()
     finally:
         lib.fomo_batch_destroy(hb)
@@ -319,6 +323,12 @@ def test_reduced_batch_uneven_matches_single(b, storage, mode):
         b.set_split(16, 0)
     for q, want in enumerate(singles):
         assert np.array_equal(keep[2 * q + 1].gather(), want)
+    if len(set(nts)) > 1:  # each shot's final state against its solo run's
+        for q, c in enumerate(cases):
+            W = E.Wavefield(c.M.nx, c.M.nz, b)
+            E.run_shot(b, W, c.M, c.H, c.a, c.src, c.rec, c.p)
+            for f in range(5):
+                assert np.array_equal(keep[2 * q].download(f), W.download(f)), (q, f)
     if storage == "f24":
         for q, c in enumerate(cases):
             r = c.oracle(storage="f24")
